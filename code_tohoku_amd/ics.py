@@ -1,0 +1,173 @@
+"""Seeded initial states for the per-timestep MHD update (host side, numpy).
+
+All generators return the global state array described in ``restart.py``
+(16 fields on the (-1..N+2)^3 Fortran index box, x fastest) with the
+interior cells 1..N set, the N+1 planes set (face B on the far faces, periodic
+images elsewhere) and the remaining ghosts left to the solver's own fills.
+
+Face B is built from a staggered vector potential, so the discrete
+divergence the constrained-transport update preserves,
+  (Bx(i+1)-Bx(i))/dx(i) + (By(j+1)-By(j))/dy(j) + (Bz(k+1)-Bz(k))/dz(k),
+vanishes to round-off.  Units are the code units of the reference
+(1 pc, 1 Myr, m_p cm^-3; kb = 8.63359, see MCF3DSGincfca3.f90:11).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .restart import Input3D, empty_state
+
+KB = 8.63359
+# composition of the reference's inflow boundary (MCF3DSGincfca3.f90:182-190)
+_CM = None
+
+
+def bbrv_cm(p: Input3D) -> np.ndarray:
+    xc = 1.4e-4
+    cm = np.zeros(8)
+    cm[0] = 0.91 / 1.27
+    cm[1] = 0.91 * p.pini / p.Hini / 1.27
+    cm[2] = 0.91 * p.H2ini / p.Hini / 1.27
+    cm[3] = 0.09 / 1.27
+    cm[4] = 0.09 * p.Hepini / p.Heini / 1.27
+    cm[7] = xc / 1.27
+    cm[5] = xc * p.Cini / p.Cpini / 1.27
+    cm[6] = xc * p.COini / p.Cpini / 1.27
+    return cm
+
+
+def grid(p: Input3D):
+    """Cell edges x(i) and widths dx(i) for i = -1..N+2 (INITIA M3:255-298)."""
+    out = []
+    np1 = (p.np1x, p.np1y, p.np1z)
+    ql1 = (p.ql1x, p.ql1y, p.ql1z)
+    for d, n in enumerate(p.n):
+        idx = np.arange(-1, n + 3)
+        if d == 0:
+            dx = np.where(idx <= p.np1x, p.ql1x / p.np1x, p.ql2x / p.np2x)
+        else:
+            dx = np.full(n + 4, ql1[d] / np1[d])
+        x = np.empty(n + 4)
+        x[0] = -dx[1]
+        for i in range(1, n + 4):
+            x[i] = x[i - 1] + dx[i]
+        out.append((x, dx))
+    return out
+
+
+def _modes(rng, nmodes, kmax, periodic_x):
+    ks = []
+    while len(ks) < nmodes:
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        if not periodic_x:
+            k = k.astype(float)
+            k[0] = rng.uniform(-kmax, kmax)
+        if np.any(k != 0):
+            ks.append(k)
+    return np.array(ks, dtype=float), rng.uniform(0, 2 * np.pi, nmodes), rng.normal(size=nmodes)
+
+
+def _field(mods, X, Y, Z, L):
+    ks, ph, amp = mods
+    f = np.zeros(np.broadcast_shapes(X.shape, Y.shape, Z.shape))
+    for k, p0, a in zip(ks, ph, amp):
+        f += a * np.cos(2 * np.pi * (k[0] * X / L[0] + k[1] * Y / L[1] + k[2] * Z / L[2]) + p0)
+    return f / np.sqrt(0.5 * np.sum(amp ** 2))
+
+
+def _wrap_periodic(st, n, periodic):
+    """Set the N+1 plane of periodic axes to the image of plane 1."""
+    nx, ny, nz = n
+    if periodic[0]:
+        st[:, :, :, nx + 2] = st[:, :, :, 2]
+    if periodic[1]:
+        st[:, :, ny + 2, :] = st[:, :, 2, :]
+    if periodic[2]:
+        st[:, nz + 2, :, :] = st[:, 2, :, :]
+
+
+def random_state(p: Input3D, seed: int = 1, vamp: float = 8.0, bamp: float = 2.0,
+                 b0=(4.0, 4.0, 1.0), rho0: float = 1.3, pt_range=(2.5, 12.0),
+                 converging: bool | None = None) -> np.ndarray:
+    """Smooth random state (density, velocity, pressure, composition and
+    face B all perturbed) on the grid of ``p``.  Thermal pressure n*T (in
+    10^3 K cm^-3) spans ``pt_range``, which puts cells on both sides of the
+    limiter's T = 3 switch and in both the explicit and implicit cooling
+    regimes.  With ``converging`` (default: when x has inflow boundaries) vx
+    follows -v0*tanh((x-Lx/2)/w) like the colliding-flow problem."""
+    rng = np.random.default_rng(seed)
+    n = p.n
+    nx, ny, nz = n
+    periodic = (p.bcx1 == 3 and p.bcx2 == 3, True, True)
+    if converging is None:
+        converging = not periodic[0]
+    (xe, dxe), (ye, dye), (ze, dze) = grid(p)
+    L = (xe[nx + 1] - xe[1], ye[ny + 1] - ye[1], ze[nz + 1] - ze[1])
+    # cell centres for slots 0..N+3 (index -1..N+2)
+    xc = xe - 0.5 * dxe
+    yc = ye - 0.5 * dye
+    zc = ze - 0.5 * dze
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    st = empty_state(n)
+    mk = lambda: _modes(rng, 6, 3, periodic[0])  # noqa: E731
+    f_rho, f_pt, f_h2, f_vx, f_vy, f_vz = (_field(mk(), X, Y, Z, L) for _ in range(6))
+    rho = rho0 * np.exp(0.35 * f_rho)
+    cm = bbrv_cm(p)
+    h2frac = 0.15 * (1.0 + np.tanh(f_h2))          # molecular fraction 0..0.3
+    sp = np.empty((8,) + rho.shape)
+    for s in range(8):
+        sp[s] = cm[s] * rho
+    sp[2] = h2frac * 0.91 / 1.27 * rho * 0.5
+    sp[0] = cm[0] * rho * (1.0 - h2frac)
+    lo, hi = np.log(pt_range[0]), np.log(pt_range[1])
+    pt = np.exp(lo + (hi - lo) * 0.5 * (1.0 + np.tanh(f_pt)))
+    ntot = sp[1] + sp[0] + sp[2] + sp[3] + sp[4]
+    nnum = rho / 1.27
+    pres = KB * ntot * (pt / nnum)                  # p = kb * sum(n) * T, T = (nT)/n
+    vx = vamp * f_vx
+    if converging:
+        w = 0.1 * L[0]
+        vx = -abs(p.vinitx1) * np.tanh((X - (xe[1] + 0.5 * L[0])) / w) + 0.3 * vamp * f_vx
+    vy = vamp * f_vy
+    vz = vamp * f_vz
+    st[0], st[1], st[2], st[3], st[4] = rho, vx, vy, vz, pres
+    st[8:16] = sp
+    # face B from a staggered vector potential: A_d on the edges
+    ex, ey, ez = xe, ye, ze              # edge coordinates x(i-1) sit at slot i
+    xs = np.concatenate(([ex[0] - dxe[0]], ex[:-1]))  # x(i-1) for slot i
+    ys = np.concatenate(([ey[0] - dye[0]], ey[:-1]))
+    zs = np.concatenate(([ez[0] - dze[0]], ez[:-1]))
+    Zc, Ys, Xs = np.meshgrid(zc, ys, xs, indexing="ij")   # Az: edge (x(i-1), y(j-1)), z centre
+    Zs, Yc, Xs2 = np.meshgrid(zs, yc, xs, indexing="ij")  # Ay: edge (x(i-1), z(k-1))
+    Zs2, Ys2, Xc = np.meshgrid(zs, ys, xc, indexing="ij")  # Ax: edge (y(j-1), z(k-1))
+    scale = bamp * L[0] / (2 * np.pi * 2.0)
+    Ax = scale * _field(mk(), Xc, Ys2, Zs2, L)
+    Ay = scale * _field(mk(), Xs2, Yc, Zs, L)
+    Az = scale * _field(mk(), Xs, Ys, Zc, L)
+    dxa = dxe[None, None, :]
+    dya = dye[None, :, None]
+    dza = dze[:, None, None]
+    Bx = np.zeros_like(rho); By = np.zeros_like(rho); Bz = np.zeros_like(rho)
+    Bx[:, :-1, :] = (Az[:, 1:, :] - Az[:, :-1, :]) / dya[:, :-1, :]
+    Bx[:-1, :, :] -= (Ay[1:, :, :] - Ay[:-1, :, :]) / dza[:-1, :, :]
+    By[:-1, :, :] = (Ax[1:, :, :] - Ax[:-1, :, :]) / dza[:-1, :, :]
+    By[:, :, :-1] -= (Az[:, :, 1:] - Az[:, :, :-1]) / dxa[:, :, :-1]
+    Bz[:, :, :-1] = (Ay[:, :, 1:] - Ay[:, :, :-1]) / dxa[:, :, :-1]
+    Bz[:, :-1, :] -= (Ax[:, 1:, :] - Ax[:, :-1, :]) / dya[:, :-1, :]
+    st[5] = b0[0] + Bx
+    st[6] = b0[1] + By
+    st[7] = b0[2] + Bz
+    _wrap_periodic(st, n, periodic)
+    return st
+
+
+def face_divergence(st: np.ndarray, p: Input3D) -> np.ndarray:
+    """Discrete div B on interior cells, the quantity CC(3) preserves."""
+    nx, ny, nz = p.n
+    (_, dxe), (_, dye), (_, dze) = grid(p)
+    I = slice(2, nx + 2); J = slice(2, ny + 2); K = slice(2, nz + 2)
+    bx, by, bz = st[5], st[6], st[7]
+    d = (bx[K, J, 3:nx + 3] - bx[K, J, I]) / dxe[None, None, I]
+    d += (by[K, 3:ny + 3, I] - by[K, J, I]) / dye[None, J, None]
+    d += (bz[3:nz + 3, J, I] - bz[K, J, I]) / dze[K, None, None]
+    return d

@@ -1,0 +1,31 @@
+#!/bin/bash
+# Builds the reference MPI code (3DMHD-CfCA/MCF3DSGincfca3.f90, linked with
+# 3DMHD-CfCA/selfG_MPIfull8.f90 which only provides GRAVTY) from the sources
+# where they lie under /root/reference, with AMD flang and the MPICH of
+# /opt/conda.  Test infrastructure: the binary goes to oracle/_ref/bin/ and is
+# used only to write golden fixtures in this container.
+#
+# usage: oracle/build_ref.sh [NDX]     (NDX = compile-time block size, default 18)
+#
+# -O2 without -march: baseline x86-64 has no FMA, so nothing is fused.
+set -euo pipefail
+NDX=${1:-18}
+REF=/root/reference/3DMHD-CfCA
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+FC=/opt/rocm/bin/amdflang
+MPI=/opt/conda
+if [ ! -f "$REF/MCF3DSGincfca3.f90" ]; then
+  echo "reference sources not present; skipping reference build" >&2
+  exit 0
+fi
+mkdir -p "$OUT/bin"
+TMP=$(mktemp -d "$OUT/tmp_build.XXXXXX")
+trap 'rm -rf "$TMP"' EXIT
+python3 "$HERE/patch_ref.py" "$REF/MCF3DSGincfca3.f90" "$TMP/m3.f90" "$NDX"
+cp "$REF/selfG_MPIfull8.f90" "$TMP/sg8.f90"
+cd "$TMP"
+$FC -O2 -I$MPI/include -c m3.f90 -o m3.o 2>build.log || { cat build.log; exit 1; }
+$FC -O2 -I$MPI/include -c sg8.f90 -o sg8.o 2>>build.log || { cat build.log; exit 1; }
+$FC -O2 -o "$OUT/bin/mcf_ref_ndx$NDX" m3.o sg8.o -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi
+echo "built $OUT/bin/mcf_ref_ndx$NDX"

@@ -1,0 +1,201 @@
+"""ctypes binding of the CPU restatement (oracle/mcf_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, and only as the checker / the timed CPU
+baseline.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmcf_oracle.so")
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("np1x", "np2x", "np1y", "np2y", "np1z", "np2z")] + \
+        [(n, C.c_double) for n in ("ql1x", "ql2x", "ql1y", "ql2y", "ql1z", "ql2z",
+                                   "vinitx1", "vinitx2", "vinity1", "vinity2", "vinitz1", "vinitz2",
+                                   "binitx1", "binitx2", "binity1", "binity2", "binitz1", "binitz2",
+                                   "cfl", "facdep")] + \
+        [("maxstp", C.c_int), ("nitera", C.c_int), ("tfinal", C.c_double)] + \
+        [(n, C.c_int) for n in ("bcx1", "bcx2", "bcy1", "bcy2", "bcz1", "bcz2",
+                                "ifchem", "ifthrm", "ifrad", "ifgrv")] + \
+        [(n, C.c_double) for n in ("pinit_over_kb", "Hini", "pini", "H2ini", "Heini", "Hepini",
+                                   "Cini", "COini", "Cpini")] + \
+        [(n, C.c_int) for n in ("nsplitx", "nsplity", "nsplitz")]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.c_double
+        pd = C.POINTER(C.c_double)
+        L.mcfo_create.restype = P
+        L.mcfo_create.argtypes = [C.POINTER(Params)]
+        L.mcfo_destroy.argtypes = [P]
+        L.mcfo_field.restype = pd
+        L.mcfo_field.argtypes = [P, C.c_int]
+        L.mcfo_grid.restype = pd
+        L.mcfo_grid.argtypes = [P, C.c_int, C.c_int]
+        L.mcfo_get_time.argtypes = [P, pd, C.POINTER(C.c_int)]
+        L.mcfo_set_time.argtypes = [P, D, C.c_int]
+        L.mcfo_bbrv.argtypes = [P, pd, pd, pd]
+        L.mcfo_cc4.argtypes = [P]
+        L.mcfo_couran.restype = D
+        L.mcfo_couran.argtypes = [P]
+        L.mcfo_stblty.restype = D
+        L.mcfo_stblty.argtypes = [P]
+        L.mcfo_timestep.restype = D
+        L.mcfo_timestep.argtypes = [P, D, C.POINTER(C.c_int)]
+        for nm in ("mcfo_source", "mcfo_ct", "mcfo_step"):
+            getattr(L, nm).argtypes = [P, D]
+        L.mcfo_sweep.argtypes = [P, C.c_int, D]
+        L.mcfo_dissip.argtypes = [P]
+        L.mcfo_run.restype = C.c_int
+        L.mcfo_run.argtypes = [P, C.c_int, pd, C.POINTER(C.c_int)]
+        L.mcfo_rieman.argtypes = [pd, D, pd, pd]
+        L.mcfo_chareq.argtypes = [pd, pd, pd]
+        L.mcfo_fcool_ki.restype = D
+        L.mcfo_fcool_ki.argtypes = [D, D]
+        L.mcfo_imc.restype = D
+        L.mcfo_imc.argtypes = [D, D, D, D, D]
+        L.mcfo_vlimit1d.argtypes = [C.c_int, pd, pd, pd, pd]
+        _lib = L
+    return _lib
+
+
+def to_params(p, nsplit=(2, 2, 2)) -> Params:
+    out = Params()
+    for name, _ in Params._fields_:
+        if name.startswith("nsplit"):
+            continue
+        setattr(out, name, getattr(p, name))
+    out.nsplitx, out.nsplity, out.nsplitz = nsplit
+    return out
+
+
+class Oracle:
+    """One single-block CPU instance of the reference step."""
+
+    def __init__(self, p, state: np.ndarray | None = None, time: float = 0.0, ifevo: int = 1,
+                 nsplit=(2, 2, 2)):
+        self.p = p
+        self.n = p.n
+        self._params = to_params(p, nsplit)
+        self.h = lib().mcfo_create(C.byref(self._params))
+        nx, ny, nz = self.n
+        self.shape = (nz + 4, ny + 4, nx + 4)
+        if state is not None:
+            self.set_state(state)
+            lib().mcfo_cc4(self.h)
+        lib().mcfo_set_time(self.h, time, ifevo)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mcfo_destroy(self.h)
+            self.h = None
+
+    def field(self, fid: int) -> np.ndarray:
+        ptr = lib().mcfo_field(self.h, fid)
+        return np.ctypeslib.as_array(ptr, shape=self.shape)
+
+    def set_state(self, st: np.ndarray):
+        for f in range(16):
+            self.field(f + 1)[...] = st[f]
+
+    def state(self) -> np.ndarray:
+        return np.stack([self.field(f + 1).copy() for f in range(16)])
+
+    def bcc(self) -> np.ndarray:
+        return np.stack([self.field(17 + c).copy() for c in range(3)])
+
+    @property
+    def time(self):
+        t = C.c_double()
+        e = C.c_int()
+        lib().mcfo_get_time(self.h, C.byref(t), C.byref(e))
+        return t.value, e.value
+
+    def run(self, nsteps: int):
+        dts = np.zeros(nsteps)
+        tags = np.zeros(nsteps, dtype=np.int32)
+        n = lib().mcfo_run(self.h, nsteps, dts.ctypes.data_as(C.POINTER(C.c_double)),
+                           tags.ctypes.data_as(C.POINTER(C.c_int)))
+        return dts[:n], tags[:n]
+
+    def step(self, dt: float):
+        lib().mcfo_step(self.h, dt)
+
+    def timestep(self, tsave: float = 1e30):
+        st = C.c_int()
+        dt = lib().mcfo_timestep(self.h, tsave, C.byref(st))
+        return dt, st.value
+
+    def couran(self):
+        return lib().mcfo_couran(self.h)
+
+    def stblty(self):
+        return lib().mcfo_stblty(self.h)
+
+    def source(self, dt):
+        lib().mcfo_source(self.h, dt)
+
+    def sweep(self, d, dt):
+        lib().mcfo_sweep(self.h, d, dt)
+
+    def ct(self, dt):
+        lib().mcfo_ct(self.h, dt)
+
+    def dissip(self):
+        lib().mcfo_dissip(self.h)
+
+
+def rieman(ql, qr, gamma):
+    a = np.array(list(ql) + list(qr), dtype=np.float64)
+    pa = C.c_double(); va = C.c_double()
+    lib().mcfo_rieman(a.ctypes.data_as(C.POINTER(C.c_double)), gamma, C.byref(pa), C.byref(va))
+    return pa.value, va.value
+
+
+def chareq(ql, qr):
+    a = np.array(list(ql) + list(qr), dtype=np.float64)
+    b = C.c_double(); v = C.c_double()
+    lib().mcfo_chareq(a.ctypes.data_as(C.POINTER(C.c_double)), C.byref(b), C.byref(v))
+    return b.value, v.value
+
+
+def fcool_ki(rho, T):
+    return lib().mcfo_fcool_ki(rho, T)
+
+
+def imc(P, n, dt, rho, gammi1):
+    return lib().mcfo_imc(P, n, dt, rho, gammi1)
+
+
+def vlimit1d(q, T, dxx):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.size - 4
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    dxx = np.ascontiguousarray(dxx, dtype=np.float64)
+    out = np.zeros(n + 2)
+    pd = C.POINTER(C.c_double)
+    lib().mcfo_vlimit1d(n, q.ctypes.data_as(pd), T.ctypes.data_as(pd), dxx.ctypes.data_as(pd),
+                        out.ctypes.data_as(pd))
+    return out

@@ -1,0 +1,53 @@
+#!/usr/bin/env python3
+"""Scratch-copy edits for building the reference 3DMHD-CfCA/MCF3DSGincfca3.f90.
+
+Test infrastructure: writes an edited COPY of the reference source into a
+temporary build directory (deleted by build_ref.sh after compiling); the
+reference tree itself is never modified and no reference source is kept in
+this repository.
+
+Edits (each recorded in the fixture manifest written by ref_runner.py):
+  1. block size: the compile-time ndx/ndy/ndz/ndmax (M3:3) is set to NDX, so
+     that N/NSPLT <= NDX-2 holds for the grid being run;
+  2. run directory: the hard-coded '/work/maedarn/3DMHD/samplecnv2/' prefix
+     (M3:134,302,309,...) becomes the process working directory;
+  3. cooling: the chemistry-network Fcool body (M3:2825-2881) is replaced by
+     the Koyama-Inutsuka form of 1DMHDKIcool/1DMHDwCC3KIcool.f90:1165-1171
+     with N = U(i,j,k,1)/1.27, the same substitution the reference's own
+     3DMHD-feedback/MCF3DSGincfca3-feedback.f90:3586 makes.
+"""
+import re
+import sys
+
+KI_FCOOL = """SUBROUTINE Fcool(CooL,T,i,j,k)
+USE comvar
+double precision :: CooL,T,N,AbsL,T0,Laml,Lamc
+AbsL = 3.94656d1
+T0   = 1.0d3
+N    = U(i,j,k,1)/1.27d0
+Laml = 1.0d7 * dexp( -1.184d5/(T*T0+1.0d3) )
+Laml = dmin1(Laml,5.d3)
+Lamc = 1.4d-2 * dsqrt(T*T0) * dexp( -9.2d1/(T*T0) )
+CooL = ( N**2.d0 * ( Laml + Lamc ) - N ) * AbsL
+END SUBROUTINE Fcool
+"""
+
+
+def patch(text: str, ndx: int) -> str:
+    text, n = re.subn(r"^INTEGER, parameter :: ndx=\d+, ndy=\d+, ndz=\d+, ndmax=\d+, Dim=3.*$",
+                      f"INTEGER, parameter :: ndx={ndx}, ndy={ndx}, ndz={ndx}, ndmax={ndx}, Dim=3",
+                      text, count=1, flags=re.M)
+    assert n == 1, "block-size line not found"
+    text = text.replace("'/work/maedarn/3DMHD/samplecnv2/", "'")
+    start = text.index("SUBROUTINE Fcool(CooL,T,i,j,k)")
+    end = text.index("END SUBROUTINE Fcool", start) + len("END SUBROUTINE Fcool")
+    text = text[:start] + KI_FCOOL.rstrip("\n") + text[end:]
+    return text
+
+
+if __name__ == "__main__":
+    src, dst, ndx = sys.argv[1], sys.argv[2], int(sys.argv[3])
+    with open(src) as f:
+        t = f.read()
+    with open(dst, "w") as f:
+        f.write(patch(t, ndx))

@@ -1,0 +1,71 @@
+"""Drive the reference MPI build (oracle/_ref/bin) on an injected state.
+
+TEST INFRASTRUCTURE (fixture generation in this container only; the
+reference never travels).  The state enters through the reference's own
+restart path: tsave.DAT with nunit >= 2 makes INITIA read 000{rank}.dat
+(3DMHD-CfCA/MCF3DSGincfca3.f90:454-465), and the final SAVEU writes the
+result back in the same format (:749-757).  INPUT3D.DAT carries maxstp=1 and
+nitera=nsteps, so EVOLVE runs exactly nsteps inner iterations unless the
+tsave/tfinal clips (:603-604) stop it earlier.
+"""
+from __future__ import annotations
+
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from code_tohoku_amd import restart as rs  # noqa: E402
+
+REF_BIN = os.path.join(HERE, "_ref", "bin")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def ref_binary(ndx: int) -> str:
+    path = os.path.join(REF_BIN, f"mcf_ref_ndx{ndx}")
+    if not os.path.exists(path):
+        subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(ndx)], check=True)
+    return path
+
+
+def run_reference(params: rs.Input3D, state: np.ndarray, nsteps: int, nranks: int = 8,
+                  tag: str = "run", time0: float = 0.0) -> dict:
+    nsplit = rs.NSPLIT_TABLE[nranks]
+    nx, ny, nz = params.n
+    cx, cy, cz = nx // nsplit[0], ny // nsplit[1], nz // nsplit[2]
+    assert cx * nsplit[0] == nx and cy * nsplit[1] == ny and cz * nsplit[2] == nz
+    ndx = max(cx, cy, cz) + 2
+    exe = ref_binary(ndx)
+    run = os.path.join(HERE, "_ref", "runs", tag)
+    if os.path.isdir(run):
+        shutil.rmtree(run)
+    os.makedirs(run)
+    p = rs.Input3D(**{**params.__dict__})
+    p.maxstp, p.nitera = 1, nsteps
+    with open(os.path.join(run, "INPUT3D.DAT"), "w") as f:
+        f.write(p.to_text())
+    rs.write_counters(run, time=time0, nunit=2)
+    for r, b in enumerate(rs.split_state(state, nsplit)):
+        rs.write_restart_rank(os.path.join(run, f"000{r:03d}.dat"), b)
+    cmd = f"ulimit -s unlimited; {MPIEXEC} -n {nranks} {exe}"
+    res = subprocess.run(["bash", "-c", cmd], cwd=run, capture_output=True, text=True, timeout=1800)
+    if res.returncode != 0:
+        raise RuntimeError(f"reference run failed rc={res.returncode}\n{res.stdout[-3000:]}\n{res.stderr[-3000:]}")
+    dts = []
+    for ln in res.stdout.splitlines():
+        m = re.match(r"^\s*(\d+)\s+([-+0-9.EeDd]+)\s+([-+0-9.EeDd]+)\s*$", ln)
+        if m:
+            dts.append(float(m.group(3).replace("D", "E").replace("d", "e")))
+    blocks = []
+    for r in range(nranks):
+        b, _ = rs.read_restart_rank(os.path.join(run, f"000{r:03d}.dat"), cx, cy, cz)
+        blocks.append(b)
+    out = rs.assemble_state(blocks, (nx, ny, nz), nsplit)
+    with open(os.path.join(run, "tsave.DAT")) as f:
+        t_end = float(f.readline().replace("D", "E"))
+    return {"state": out, "dts": dts, "time": t_end, "stdout": res.stdout, "run_dir": run}
