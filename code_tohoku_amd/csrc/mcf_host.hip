@@ -1,0 +1,666 @@
+// mcf_host.hip -- host side of the C ABI (include/mcf_abi.h): the device
+// context, the ghost-fill schedule of the reference's BC_MPI/BC_MPI_OT calls
+// and the stage order of EVOLVE (3DMHD-CfCA/MCF3DSGincfca3.f90, "M3").
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mcf_abi.h"
+#include "mcf_kernels.h"
+
+using namespace mcf;
+
+static thread_local std::string g_err;
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) {                                                                        \
+      g_err = std::string(#x) + ": " + hipGetErrorString(e_);                                     \
+      return MCF_E_HIP;                                                                            \
+    }                                                                                              \
+  } while (0)
+#define CHK(x)                                                                                     \
+  do {                                                                                             \
+    int r_ = (x);                                                                                  \
+    if (r_ != MCF_OK) return r_;                                                                   \
+  } while (0)
+
+enum { F_RHO = 0, F_VX, F_VY, F_VZ, F_P, F_BX, F_BY, F_BZ };
+
+struct mcf_ctx {
+  mcf_config cfg;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  Geo g{};
+  size_t ncell = 0;
+  double *pool = nullptr;
+  double *U[8]{}, *sp[8]{}, *bcc[3]{};
+  // sweep scratch
+  double *u5e = nullptr, *blg[2]{}, *xlag = nullptr, *dxlagM = nullptr, *pa = nullptr, *va = nullptr;
+  double *mf[8]{}, *fl[15]{};
+  // CT / source scratch (aliases the sweep scratch: the stages never overlap)
+  double *vfc[3]{}, *dq[4]{}, *emf[3]{}, *Tn = nullptr, *Pn = nullptr;
+  double *divv = nullptr, *red = nullptr;
+  double *gx[3]{}, *gdx[3]{};
+  std::vector<double> hx[3], hdx[3];
+  double bbrv[11][3][3]{}; // BBRV(k, mode, side), 1-based
+  double bbrv_cm[8]{};
+  double dtf = 0, pmin = 0, pmax = 0, rmin = 0, spmin[8]{};
+  int nsp[3]{1, 1, 1};
+  double time = 0.0;
+  int ifevo = 1;
+  double *hred = nullptr; // pinned 2 doubles
+  // per-kernel event timing (mcf_profile)
+  bool prof = false;
+  struct Rec { const char *name; hipEvent_t a, b; double bytes; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evpool;
+};
+
+static hipEvent_t prof_event(mcf_ctx *x) {
+  if (!x->evpool.empty()) { hipEvent_t e = x->evpool.back(); x->evpool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+extern "C" void mcf_default_config(mcf_config *c) {
+  std::memset(c, 0, sizeof(*c));
+  c->np1x = c->np2x = c->np1y = c->np2y = c->np1z = c->np2z = 64;
+  c->ql1x = c->ql2x = c->ql1y = c->ql2y = c->ql1z = c->ql2z = 10.0;
+  c->vinitx1 = 9.778; c->vinitx2 = -9.778;
+  c->binity1 = c->binity2 = 2.2307;
+  c->cfl = 0.49; c->facdep = 1.0;
+  c->maxstp = 4; c->nitera = 5000; c->tfinal = 100.0;
+  c->bcx1 = c->bcx2 = 4; c->bcy1 = c->bcy2 = c->bcz1 = c->bcz2 = 3;
+  c->ifchem = 0; c->ifthrm = 2; c->ifrad = 0; c->ifgrv = 0;
+  c->pinit_over_kb = 8.810807e3;
+  c->Hini = 0.9219098; c->pini = 0.9503446e-2; c->H2ini = 0.9465513e-8;
+  c->Heini = 0.9155226e-1; c->Hepini = 0.5655353e-3;
+  c->Cini = 0.1565848e-8; c->COini = 0.2202631e-20; c->Cpini = 0.1433520e-3;
+  c->nsplitx = c->nsplity = c->nsplitz = 2;
+  c->precision = MCF_PREC_FP64;
+  c->cooling = MCF_COOL_KI;
+}
+
+extern "C" const char *mcf_last_error(void) { return g_err.c_str(); }
+
+// INITIA (M3:155-298): boundary states, floors, grid
+static void setup_constants(mcf_ctx *x) {
+  const mcf_config &p = x->cfg;
+  const double mH = 1.0, mHe = 4.0, mH2 = 2.0, xc = 1.4e-4;
+  double pinit1 = p.pinit_over_kb * KB * 1.0e-3;
+  double dinit1 = mH * p.Hini + mH * p.pini + mH2 * p.H2ini + mHe * p.Heini + mHe * p.Hepini;
+  double s4 = p.Hini + p.pini + p.Heini + p.Hepini;
+  double gamma = (5.0 * s4 + 7.0 * p.H2ini) / (3.0 * s4 + 5.0 * p.H2ini);
+  double gammi1 = gamma - 1.0;
+  x->pmin = 1.829797 * 8.6336;
+  x->pmax = 1.0e10;
+  x->rmin = 0.1949628;
+  double smin[8] = {x->rmin * 0.91, 1.0e-20, 1.0e-20, x->rmin * 0.09, 1.0e-20, 1.0e-20, 1.0e-20, 1.0e-20};
+  std::memcpy(x->spmin, smin, sizeof(smin));
+  double *cm = x->bbrv_cm;
+  cm[0] = 0.91 / 1.27;
+  cm[1] = 0.91 * p.pini / p.Hini / 1.27;
+  cm[2] = 0.91 * p.H2ini / p.Hini / 1.27;
+  cm[3] = 0.09 / 1.27;
+  cm[4] = 0.09 * p.Hepini / p.Heini / 1.27;
+  cm[7] = xc / 1.27;
+  cm[5] = xc * p.Cini / p.Cpini / 1.27;
+  cm[6] = xc * p.COini / p.Cpini / 1.27;
+  double dBC = mH * cm[0] + mH * cm[1] + mH2 * cm[2] + mHe * cm[3] + mHe * cm[4];
+  auto &B = x->bbrv;
+  B[1][1][1] = B[1][2][1] = B[1][1][2] = B[1][2][2] = dBC;
+  B[2][1][1] = p.vinitx1; B[2][2][1] = dBC * p.vinitx1; B[2][1][2] = -p.vinitx1; B[2][2][2] = -dBC * p.vinitx1;
+  B[3][1][1] = p.vinity1; B[3][2][1] = dBC * p.vinity1; B[3][1][2] = p.vinity1; B[3][2][2] = dBC * p.vinity1;
+  B[4][1][1] = p.vinitz1; B[4][2][1] = dBC * p.vinitz1; B[4][1][2] = p.vinitz1; B[4][2][2] = dBC * p.vinitz1;
+  double e5 = pinit1 / gammi1 + 0.5 * (dBC * (p.vinitx1 * p.vinitx1) + p.binitx1 * p.binitx1 +
+                                       p.binity1 * p.binity1 + p.binitz1 * p.binitz1);
+  B[5][1][1] = pinit1; B[5][2][1] = e5; B[5][1][2] = pinit1; B[5][2][2] = e5;
+  B[6][1][1] = B[6][2][1] = p.binitx1; B[6][1][2] = B[6][2][2] = p.binitx2;
+  B[7][1][1] = B[7][2][1] = p.binity1; B[7][1][2] = B[7][2][2] = p.binity2;
+  B[8][1][1] = B[8][2][1] = p.binitz1; B[8][1][2] = B[8][2][2] = p.binitz2;
+  x->dtf = (double)(float)dinit1; // DTF is REAL*4 (M3:22), filled with dinit1
+  int np1[3] = {p.np1x, p.np1y, p.np1z};
+  double ql1[3] = {p.ql1x, p.ql1y, p.ql1z};
+  for (int d = 0; d < 3; ++d) {
+    int n = x->g.n[d];
+    x->hx[d].assign(n + 4, 0.0);
+    x->hdx[d].assign(n + 4, 0.0);
+    for (int i = -1; i <= n + 2; ++i) {
+      double v;
+      if (d == 0) v = (i <= p.np1x) ? p.ql1x / (double)p.np1x : p.ql2x / (double)p.np2x;
+      else v = ql1[d] / (double)np1[d];
+      x->hdx[d][i + 1] = v;
+    }
+    x->hx[d][0] = -x->hdx[d][1];
+    for (int i = 0; i <= n + 2; ++i) x->hx[d][i + 1] = x->hx[d][i] + x->hdx[d][i + 1];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// launch helpers
+// ----------------------------------------------------------------------------
+static const int TPB = 256;
+
+static Box mkbox(int x0, int x1, int y0, int y1, int z0, int z1) {
+  Box b;
+  b.lo[0] = x0; b.cnt[0] = x1 - x0 + 1;
+  b.lo[1] = y0; b.cnt[1] = y1 - y0 + 1;
+  b.lo[2] = z0; b.cnt[2] = z1 - z0 + 1;
+  return b;
+}
+// box with axis d over [a0,a1] and the other axes over [1,n]
+static Box sweepbox(const mcf_ctx *x, int d, int a0, int a1) {
+  int lo[3], hi[3];
+  for (int e = 0; e < 3; ++e) { lo[e] = 1; hi[e] = x->g.n[e]; }
+  lo[d] = a0; hi[d] = a1;
+  return mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+}
+static unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+static long boxsize(const Box &b) { return (long)b.cnt[0] * b.cnt[1] * b.cnt[2]; }
+
+// Launch with nthreads threads; narr = arrays the kernel reads or writes
+// per thread, for the algorithmic byte count of mcf_profile.
+#define KLAUNCH(kern, nthreads, narr, ...)                                                          \
+  do {                                                                                             \
+    long nt_ = (nthreads);                                                                         \
+    if (nt_ > 0) {                                                                                 \
+      hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                     \
+      if (x->prof) { ea_ = prof_event(x); eb_ = prof_event(x); HIPCHK(hipEventRecord(ea_, x->stream)); } \
+      hipLaunchKernelGGL(kern, dim3(nblocks(nt_)), dim3(TPB), 0, x->stream, __VA_ARGS__);          \
+      HIPCHK(hipGetLastError());                                                                   \
+      if (x->prof) {                                                                               \
+        HIPCHK(hipEventRecord(eb_, x->stream));                                                    \
+        x->recs.push_back({#kern, ea_, eb_, 8.0 * (double)nt_ * (double)(narr)});                  \
+      }                                                                                            \
+    }                                                                                              \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// ghost fills
+// ----------------------------------------------------------------------------
+static int fill_periodic(mcf_ctx *x, double *const *a, int nf, int d, int nol) {
+  FillList L{};
+  L.nf = nf;
+  for (int f = 0; f < nf; ++f) L.a[f] = a[f];
+  int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
+  long nt = (long)(x->g.n[da] + 4) * (x->g.n[db] + 4) * 2 * nol;
+  KLAUNCH(k_fill_periodic, nt, 2 * nf, L, x->g, d, nol);
+  return MCF_OK;
+}
+
+static bool x_inflow(const mcf_ctx *x) { return x->cfg.bcx1 == 4 || x->cfg.bcx2 == 4; }
+
+static int launch_xfill(mcf_ctx *x, XFill &X, int nol) {
+  X.rho = x->U[F_RHO];
+  X.bcc1 = x->bcc[0];
+  X.blg1 = x->blg[0];
+  X.blg2 = x->blg[1];
+  long nt = (long)(x->g.n[1] + 4) * (x->g.n[2] + 4) * 2 * nol;
+  KLAUNCH(k_fill_xrule, nt, 2 * X.nf, X, x->g, nol);
+  return MCF_OK;
+}
+
+// BC_MPI(nol, mode) for the 1-based U components in list (M3:800-926)
+static int fillU(mcf_ctx *x, int dmask, int nol, int mode, const int *list, int nk) {
+  double *a[16];
+  for (int t = 0; t < nk; ++t) a[t] = x->U[list[t] - 1];
+  if (dmask & 1) {
+    if (!x_inflow(x)) {
+      CHK(fill_periodic(x, a, nk, 0, nol));
+    } else {
+      XFill X{};
+      X.nf = nk;
+      int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
+      for (int t = 0; t < nk; ++t) {
+        int K = list[t];
+        X.a[t] = a[t];
+        for (int s = 0; s < 2; ++s) {
+          int side = s + 1;
+          if (bc[s] != 4) { X.rule[t][s] = XR_PERIODIC; continue; }
+          if (K == 1) { X.rule[t][s] = XR_CONST; X.val[t][s] = x->dtf; }
+          else if (K == 6) X.rule[t][s] = XR_FACE;
+          else if (K == 5 && mode == 2) {
+            X.rule[t][s] = XR_ENERGY; X.val[t][s] = x->bbrv[5][1][side]; X.val2[s] = x->bbrv[2][1][side];
+          } else if (K == 2 && mode == 2) { X.rule[t][s] = XR_MOM; X.val[t][s] = x->bbrv[2][1][side]; }
+          else { X.rule[t][s] = XR_CONST; X.val[t][s] = x->bbrv[K][mode][side]; }
+        }
+      }
+      CHK(launch_xfill(x, X, nol));
+    }
+  }
+  for (int d = 1; d <= 2; ++d)
+    if (dmask & (1 << d)) CHK(fill_periodic(x, a, nk, d, nol));
+  return MCF_OK;
+}
+
+enum { OT_DXLAG = 1, OT_BCC, OT_BLG, OT_VFC, OT_SPECIES };
+
+// BC_MPI_OT(nol, .) pieces (M3:930-1254)
+static int fillOT(mcf_ctx *x, int dmask, int nol, int what) {
+  int nx = x->g.n[0];
+  int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
+  for (int d = 0; d < 3; ++d) {
+    if (!(dmask & (1 << d))) continue;
+    double *a[16];
+    int nf = 0;
+    if (what == OT_DXLAG) a[nf++] = x->dxlagM;
+    if (what == OT_BCC) for (int c = 0; c < 3; ++c) a[nf++] = x->bcc[c];
+    if (what == OT_BLG) { a[nf++] = x->blg[0]; a[nf++] = x->blg[1]; }
+    if (what == OT_VFC) {
+      int c1 = d == 0 ? 1 : 0, c2 = d == 2 ? 1 : 2; // x: Vfc 2,3; y: 1,3; z: 1,2
+      a[nf++] = x->vfc[c1]; a[nf++] = x->vfc[c2];
+    }
+    if (what == OT_SPECIES) for (int s = 0; s < 8; ++s) a[nf++] = x->sp[s];
+    if (d != 0 || !x_inflow(x)) { CHK(fill_periodic(x, a, nf, d, nol)); continue; }
+    XFill X{};
+    X.nf = nf;
+    for (int t = 0; t < nf; ++t) {
+      X.a[t] = a[t];
+      for (int s = 0; s < 2; ++s) {
+        int side = s + 1;
+        if (bc[s] != 4) { X.rule[t][s] = XR_PERIODIC; continue; }
+        switch (what) {
+        case OT_DXLAG: X.rule[t][s] = XR_CONST; X.val[t][s] = x->hdx[0][s == 0 ? 1 : nx + 2]; break;
+        case OT_BCC:
+          if (t == 0) { X.rule[t][s] = XR_COPY; X.src[t][s] = s == 0 ? 1 : nx; }
+          else { X.rule[t][s] = XR_CONST; X.val[t][s] = x->bbrv[6 + t][1][side]; }
+          break;
+        case OT_BLG: X.rule[t][s] = XR_CONST; X.val[t][s] = x->bbrv[7 + t][1][side]; break;
+        case OT_VFC: X.rule[t][s] = XR_COPY; X.src[t][s] = s == 0 ? 1 : nx; break;
+        case OT_SPECIES: X.rule[t][s] = XR_SPECIES; X.val[t][s] = x->bbrv_cm[t]; break;
+        }
+      }
+    }
+    CHK(launch_xfill(x, X, nol));
+  }
+  return MCF_OK;
+}
+
+// ----------------------------------------------------------------------------
+// stages
+// ----------------------------------------------------------------------------
+static SweepArgs sweep_args(mcf_ctx *x, int d) {
+  SweepArgs A{};
+  for (int f = 0; f < 8; ++f) { A.U[f] = x->U[f]; A.sp[f] = x->sp[f]; A.spmin[f] = x->spmin[f]; }
+  for (int c = 0; c < 3; ++c) A.bcc[c] = x->bcc[c];
+  A.u5e = x->u5e; A.blg[0] = x->blg[0]; A.blg[1] = x->blg[1];
+  A.xlag = x->xlag; A.dxlagM = x->dxlagM; A.pa = x->pa; A.va = x->va;
+  for (int t = 0; t < 8; ++t) A.mf[t] = x->mf[t];
+  for (int t = 0; t < 15; ++t) A.fl[t] = x->fl[t];
+  A.xe = x->gx[d]; A.dxe = x->gdx[d];
+  A.d = d;
+  static const int VN[3] = {1, 2, 3}, VT1[3] = {2, 3, 1}, VT2[3] = {3, 1, 2}, BN[3] = {5, 6, 7};
+  static const int BT1[3] = {1, 2, 0}, BT2[3] = {2, 0, 1};
+  A.VN = VN[d]; A.VT1 = VT1[d]; A.VT2 = VT2[d]; A.BN = BN[d]; A.BT1 = BT1[d]; A.BT2 = BT2[d];
+  A.nblk = x->g.n[d] / x->nsp[d];
+  A.facdep = x->cfg.facdep; A.rmin = x->rmin; A.pmin = x->pmin; A.pmax = x->pmax;
+  return A;
+}
+
+// MHD (M3:1264-1324): RHS -> MOC -> REMAP -> primitives
+static int do_sweep(mcf_ctx *x, int d, double dt) {
+  SweepArgs A = sweep_args(x, d);
+  int n = x->g.n[d];
+  // RHS (M3:1346-1357 fills)
+  int lr[3] = {1, A.VN + 1, 5};
+  CHK(fillU(x, 1 << d, 2, 1, lr, 3));
+  CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
+  Box bf = sweepbox(x, d, 0, n), bc = sweepbox(x, d, 1, n);
+  KLAUNCH(k_rhs_face, boxsize(bf), 13, A, x->g, bf, dt);
+  KLAUNCH(k_rhs_cell, boxsize(bc), 33, A, x->g, bc, dt);
+  // MOC (M3:1950-1976 fills)
+  int lm[4] = {1, A.VT1 + 1, A.VT2 + 1, A.BN + 1};
+  CHK(fillU(x, 1 << d, 2, 1, lm, 4));
+  CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
+  CHK(fillOT(x, 1 << d, 2, OT_BLG));
+  KLAUNCH(k_moc_face, boxsize(bf), 18, A, x->g, bf, dt);
+  KLAUNCH(k_moc_cell, boxsize(bc), 26, A, x->g, bc, dt);
+  // REMAP (M3:1571-1594 fills)
+  CHK(fillOT(x, 1 << d, 2, OT_BLG));
+  int l5[5] = {1, 2, 3, 4, 5};
+  CHK(fillU(x, 1 << d, 2, 2, l5, 5));
+  CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
+  KLAUNCH(k_remap_face, boxsize(bf), 32, A, x->g, bf, dt);
+  KLAUNCH(k_remap_cell, boxsize(bc), 45, A, x->g, bc, dt);
+  return MCF_OK;
+}
+
+static CtArgs ct_args(mcf_ctx *x) {
+  CtArgs A{};
+  for (int f = 0; f < 8; ++f) A.U[f] = x->U[f];
+  for (int c = 0; c < 3; ++c) { A.bcc[c] = x->bcc[c]; A.vfc[c] = x->vfc[c]; A.emf[c] = x->emf[c]; A.gdx[c] = x->gdx[c]; }
+  for (int t = 0; t < 4; ++t) A.dq[t] = x->dq[t];
+  A.facdep = x->cfg.facdep;
+  return A;
+}
+
+static int do_cc4(mcf_ctx *x) {
+  CtArgs A = ct_args(x);
+  int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
+  Box b = mkbox(1, nx, 1, ny, 1, nz);
+  KLAUNCH(k_cc4, boxsize(b), 6, A, x->g, b);
+  return fillOT(x, 7, 2, OT_BCC);
+}
+
+// CT (M3:638-649)
+static int do_ct(mcf_ctx *x, double dt) {
+  CtArgs A = ct_args(x);
+  int n[3] = {x->g.n[0], x->g.n[1], x->g.n[2]};
+  int lx[3] = {2, 7, 8}, ly[3] = {3, 6, 8}, lz[3] = {4, 6, 7}, l5[1] = {5}, l1[1] = {1};
+  CHK(fillU(x, 1, 2, 1, lx, 3));
+  CHK(fillU(x, 2, 2, 1, ly, 3));
+  CHK(fillU(x, 4, 2, 1, lz, 3));
+  Box b1 = mkbox(1, n[0] + 1, 1, n[1] + 1, 1, n[2] + 1);
+  KLAUNCH(k_cc1, boxsize(b1), 6, A, x->g, b1);
+  CHK(fillOT(x, 7, 2, OT_VFC));
+  CHK(fillU(x, 7, 2, 1, l5, 1));
+  for (int o = 0; o < 3; ++o) {
+    if (o == 0) CHK(fillU(x, 7, 2, 1, l1, 1));
+    int d1 = o, d2 = (o + 1) % 3, d3 = (o + 2) % 3;
+    int lo[3], hi[3];
+    // slopes along d1 (dQ2, dQ4)
+    lo[d1] = 0; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
+    Box bs0 = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+    KLAUNCH(k_cct_slopes, boxsize(bs0), 6, A, x->g, bs0, o, 0);
+    // slopes along d2 (dQ1, dQ3)
+    lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 0; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
+    Box bs1 = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+    KLAUNCH(k_cct_slopes, boxsize(bs1), 6, A, x->g, bs1, o, 1);
+    lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
+    Box be = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+    KLAUNCH(k_cct_emf, boxsize(be), 10, A, x->g, be, o, dt);
+  }
+  KLAUNCH(k_cc3, boxsize(b1), 9, A, x->g, b1, dt);
+  return do_cc4(x);
+}
+
+// SOURCE (M3:2448-2572)
+static int do_source(mcf_ctx *x, double dt) {
+  if (x->cfg.ifthrm != 2) return MCF_OK;
+  SrcArgs A{};
+  A.U5 = x->U[F_P];
+  A.rho = x->U[F_RHO];
+  for (int s = 0; s < 8; ++s) A.sp[s] = x->sp[s];
+  A.Tn = x->Tn; A.Pn = x->Pn;
+  for (int d = 0; d < 3; ++d) A.gdx[d] = x->gdx[d];
+  A.pmin = x->pmin; A.pmax = x->pmax;
+  int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
+  int l15[2] = {1, 5}, l5[1] = {5};
+  CHK(fillU(x, 7, 1, 1, l15, 2));
+  CHK(fillOT(x, 7, 1, OT_SPECIES));
+  Box bt = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1), bc = mkbox(1, nx, 1, ny, 1, nz);
+  KLAUNCH(k_src_temp, boxsize(bt), 8, A, x->g, bt, 1);
+  KLAUNCH(k_src_cell, boxsize(bc), 10, A, x->g, bc, dt, 0);
+  CHK(fillU(x, 7, 1, 1, l5, 1));
+  KLAUNCH(k_src_temp, boxsize(bt), 7, A, x->g, bt, 0);
+  KLAUNCH(k_src_cell, boxsize(bc), 10, A, x->g, bc, dt, 1);
+  return MCF_OK;
+}
+
+static int do_dissip(mcf_ctx *x) {
+  int l[3] = {2, 3, 4};
+  CHK(fillU(x, 7, 2, 1, l, 3));
+  DisArgs A{};
+  for (int f = 0; f < 8; ++f) A.U[f] = x->U[f];
+  A.divv = x->divv;
+  int e0 = x->nsp[0] == 1 ? 1 : 0; // one x block: only the right end (M3:3156-3158)
+  int ny = x->g.n[1], nz = x->g.n[2];
+  KLAUNCH(k_dissip_div, (2 - e0) * 12L * (ny + 2) * (nz + 2), 4, A, x->g, e0);
+  KLAUNCH(k_dissip_upd, (2 - e0) * 10L * ny * nz, 6, A, x->g, e0);
+  return MCF_OK;
+}
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
+  if (!cfg || !out) { g_err = "null argument"; return MCF_E_ARG; }
+  *out = nullptr;
+  int n[3] = {cfg->np1x + cfg->np2x, cfg->np1y + cfg->np2y, cfg->np1z + cfg->np2z};
+  int nsp[3] = {cfg->nsplitx, cfg->nsplity, cfg->nsplitz};
+  for (int d = 0; d < 3; ++d) {
+    if (n[d] < 4) { g_err = "grid must have at least 4 cells per axis"; return MCF_E_ARG; }
+    if (nsp[d] < 1 || n[d] % nsp[d]) { g_err = "nsplit must divide the cell count"; return MCF_E_ARG; }
+  }
+  if (cfg->np1x < 1 || cfg->np2x < 0 || cfg->np1y < 1 || cfg->np1z < 1) { g_err = "bad cell counts"; return MCF_E_ARG; }
+  if ((cfg->bcx1 != 3 && cfg->bcx1 != 4) || (cfg->bcx2 != 3 && cfg->bcx2 != 4)) {
+    g_err = "x boundaries must be 3 (periodic) or 4 (inflow)"; return MCF_E_ARG;
+  }
+  // DISSIP works on 10-cell layers at the x ends (M3:3152-3170)
+  if (n[0] / nsp[0] < 10) { g_err = "x block must hold at least 10 cells (DISSIP layers)"; return MCF_E_ARG; }
+  if (cfg->precision != MCF_PREC_FP64) { g_err = "precision mode not built"; return MCF_E_ARG; }
+  long long vol = (long long)(n[0] + 4) * (n[1] + 4) * (n[2] + 4);
+  if (vol >= (1LL << 31)) { g_err = "grid too large for 32-bit indexing"; return MCF_E_ARG; }
+  mcf_ctx *x = new mcf_ctx();
+  x->cfg = *cfg;
+  for (int d = 0; d < 3; ++d) { x->g.n[d] = n[d]; x->nsp[d] = nsp[d]; }
+  x->g.st[0] = 1;
+  x->g.st[1] = n[0] + 4;
+  x->g.st[2] = (n[0] + 4) * (n[1] + 4);
+  x->ncell = (size_t)vol;
+  setup_constants(x);
+  hipError_t e = hipGetDevice(&x->dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { g_err = std::string("HIP init: ") + hipGetErrorString(e); delete x; return MCF_E_HIP; }
+  const int nstate = 19, nscratch = 30;
+  size_t nc = x->ncell;
+  size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
+  size_t ngrid = 0;
+  for (int d = 0; d < 3; ++d) ngrid += 2 * (size_t)(n[d] + 4);
+  size_t total = (nstate + nscratch) * nc + ndivv + ngrid + 16;
+  e = hipMalloc(&x->pool, total * sizeof(double));
+  if (e != hipSuccess) { g_err = std::string("hipMalloc: ") + hipGetErrorString(e); (void)hipStreamDestroy(x->stream); delete x; return MCF_E_NOMEM; }
+  double *p = x->pool;
+  for (int f = 0; f < 8; ++f) { x->U[f] = p; p += nc; }
+  for (int f = 0; f < 8; ++f) { x->sp[f] = p; p += nc; }
+  for (int f = 0; f < 3; ++f) { x->bcc[f] = p; p += nc; }
+  double *scr = p;
+  x->u5e = p; p += nc;
+  x->blg[0] = p; p += nc; x->blg[1] = p; p += nc;
+  x->xlag = p; p += nc; x->dxlagM = p; p += nc; x->pa = p; p += nc; x->va = p; p += nc;
+  for (int t = 0; t < 8; ++t) { x->mf[t] = p; p += nc; }
+  for (int t = 0; t < 15; ++t) { x->fl[t] = p; p += nc; }
+  // CT and SOURCE scratch alias the sweep scratch, but CT also reads
+  // nothing of it and the x-rule fills read blg: keep blg out of the alias.
+  double *q = scr + 3 * nc; // after u5e, blg[0], blg[1]
+  for (int c = 0; c < 3; ++c) { x->vfc[c] = q; q += nc; }
+  for (int t = 0; t < 4; ++t) { x->dq[t] = q; q += nc; }
+  for (int c = 0; c < 3; ++c) { x->emf[c] = q; q += nc; }
+  x->Tn = q; q += nc; x->Pn = q; q += nc;
+  x->divv = p; p += ndivv;
+  for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
+  x->red = p; p += 16;
+  HIPCHK(hipMemsetAsync(x->pool, 0, total * sizeof(double), x->stream));
+  for (int d = 0; d < 3; ++d) {
+    HIPCHK(hipMemcpyAsync(x->gx[d], x->hx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(x->gdx[d], x->hdx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  }
+  HIPCHK(hipHostMalloc((void **)&x->hred, 2 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  *out = x;
+  return MCF_OK;
+}
+
+extern "C" int mcf_finalize(mcf_ctx *x) {
+  if (!x) return MCF_OK;
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  for (auto &r : x->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : x->evpool) (void)hipEventDestroy(e);
+  if (x->pool) (void)hipFree(x->pool);
+  if (x->hred) (void)hipHostFree(x->hred);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return MCF_OK;
+}
+
+extern "C" int mcf_upload(mcf_ctx *x, const double *U8, const double *sp8) {
+  if (!x || !U8 || !sp8) { g_err = "null argument"; return MCF_E_ARG; }
+  size_t bytes = x->ncell * sizeof(double);
+  for (int f = 0; f < 8; ++f) {
+    HIPCHK(hipMemcpyAsync(x->U[f], U8 + f * x->ncell, bytes, hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(x->sp[f], sp8 + f * x->ncell, bytes, hipMemcpyHostToDevice, x->stream));
+  }
+  CHK(do_cc4(x)); // CC(4,0) after the restart read (M3:470)
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+extern "C" int mcf_download(mcf_ctx *x, double *U8, double *sp8) {
+  if (!x || !U8 || !sp8) { g_err = "null argument"; return MCF_E_ARG; }
+  size_t bytes = x->ncell * sizeof(double);
+  for (int f = 0; f < 8; ++f) {
+    HIPCHK(hipMemcpyAsync(U8 + f * x->ncell, x->U[f], bytes, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(sp8 + f * x->ncell, x->sp[f], bytes, hipMemcpyDeviceToHost, x->stream));
+  }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+extern "C" int mcf_download_bcc(mcf_ctx *x, double *bcc3) {
+  if (!x || !bcc3) { g_err = "null argument"; return MCF_E_ARG; }
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpyAsync(bcc3 + c * x->ncell, x->bcc[c], x->ncell * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+extern "C" int mcf_set_time(mcf_ctx *x, double time, int ifevo) {
+  if (!x || ifevo < 1 || ifevo > 6) { g_err = "bad argument"; return MCF_E_ARG; }
+  x->time = time;
+  x->ifevo = ifevo;
+  return MCF_OK;
+}
+extern "C" int mcf_get_time(mcf_ctx *x, double *time, int *ifevo) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  if (time) *time = x->time;
+  if (ifevo) *ifevo = x->ifevo;
+  return MCF_OK;
+}
+
+extern "C" int mcf_timestep(mcf_ctx *x, double tsave, double *dt_out, int *tag) {
+  if (!x || !dt_out) { g_err = "null argument"; return MCF_E_ARG; }
+  DtArgs A{};
+  for (int f = 0; f < 8; ++f) { A.U[f] = x->U[f]; A.sp[f] = x->sp[f]; }
+  for (int c = 0; c < 3; ++c) { A.bcc[c] = x->bcc[c]; A.gdx[c] = x->gdx[c]; }
+  A.out = x->red;
+  A.tfinal = x->cfg.tfinal;
+  hipLaunchKernelGGL(k_dtinit, dim3(1), dim3(64), 0, x->stream, x->red, x->cfg.tfinal);
+  HIPCHK(hipGetLastError());
+  Box b = mkbox(1, x->g.n[0], 1, x->g.n[1], 1, x->g.n[2]);
+  KLAUNCH(k_dtmin, boxsize(b), 13, A, x->g, b);
+  HIPCHK(hipMemcpyAsync(x->hred, x->red, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  double tCFL = x->hred[0], tLMT = x->hred[1];
+  double tfinal = x->cfg.tfinal;
+  double dt = tfinal;
+  double t = x->cfg.cfl * tCFL;
+  dt = dt < t ? dt : t;
+  double stt = dt;
+  dt = dt < tLMT ? dt : tLMT;
+  int st = dt < stt ? 2 : 1;
+  if (x->time + dt > tfinal) dt = tfinal - x->time;
+  if (x->time + dt > tsave) dt = tsave - x->time;
+  if (!(std::isfinite(dt)) || tCFL < 0.0 || tLMT < 0.0) {
+    g_err = "non-finite or negative time step (err at Couran/Stblty)";
+    return MCF_E_NONFINITE;
+  }
+  *dt_out = dt;
+  if (tag) *tag = st;
+  return MCF_OK;
+}
+
+extern "C" int mcf_source(mcf_ctx *x, double dt) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  return do_source(x, dt);
+}
+extern "C" int mcf_sweep(mcf_ctx *x, int dir, double dt) {
+  if (!x || dir < 0 || dir > 2) { g_err = "bad argument"; return MCF_E_ARG; }
+  return do_sweep(x, dir, dt);
+}
+extern "C" int mcf_ct(mcf_ctx *x, double dt) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  return do_ct(x, dt);
+}
+extern "C" int mcf_dissip(mcf_ctx *x) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  return do_dissip(x);
+}
+
+// ifEVO sweep orders xyz, yzx, zxy, xzy, yxz, zyx (M3:611-635)
+static const int k_order[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 0}};
+
+extern "C" int mcf_step(mcf_ctx *x, double dt) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  CHK(do_source(x, 0.5 * dt));
+  const int *o = k_order[x->ifevo - 1];
+  for (int t = 0; t < 3; ++t) CHK(do_sweep(x, o[t], dt));
+  x->ifevo = x->ifevo % 6 + 1;
+  CHK(do_ct(x, dt));
+  CHK(do_source(x, 0.5 * dt));
+  CHK(do_dissip(x));
+  x->time = x->time + dt;
+  return MCF_OK;
+}
+
+extern "C" int mcf_run(mcf_ctx *x, int nsteps, double tsave, double *dts, int *ndone) {
+  if (!x || nsteps < 0) { g_err = "bad argument"; return MCF_E_ARG; }
+  int done = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    if (x->time >= x->cfg.tfinal || x->time >= tsave) break;
+    double dt;
+    int tag;
+    CHK(mcf_timestep(x, tsave, &dt, &tag));
+    CHK(mcf_step(x, dt));
+    if (dts) dts[s] = dt;
+    ++done;
+  }
+  if (ndone) *ndone = done;
+  return MCF_OK;
+}
+
+extern "C" int mcf_sync(mcf_ctx *x) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+extern "C" void *mcf_stream(mcf_ctx *x) { return x ? (void *)x->stream : nullptr; }
+
+extern "C" int mcf_profile(mcf_ctx *x, int enable) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  x->prof = enable != 0;
+  return MCF_OK;
+}
+
+extern "C" int mcf_profile_report(mcf_ctx *x, mcf_kstat *out, int maxn, int *n) {
+  if (!x || !n) { g_err = "null argument"; return MCF_E_ARG; }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  std::map<std::string, mcf_kstat> acc;
+  for (auto &r : x->recs) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    mcf_kstat &k = acc[r.name];
+    std::snprintf(k.name, sizeof(k.name), "%s", r.name);
+    k.launches += 1;
+    k.ms += ms;
+    k.bytes += r.bytes;
+    x->evpool.push_back(r.a);
+    x->evpool.push_back(r.b);
+  }
+  x->recs.clear();
+  int i = 0;
+  for (auto &kv : acc) {
+    if (out && i < maxn) out[i] = kv.second;
+    ++i;
+  }
+  *n = i;
+  return MCF_OK;
+}

@@ -1,0 +1,709 @@
+// mcf_kernels.hip -- gfx950 kernels of the per-timestep MHD update
+// (FP64 path).  Layout: one FP64 array per field on the Fortran index box
+// (-1:N+2)^3, x fastest; one thread per cell, face or edge of the stage being
+// computed, consecutive threads along x so every load is coalesced.
+//
+// Stage structure follows EVOLVE (3DMHDCfCA/MCF3DSGincfca3.f90, "M3"):
+// ghost fills (BC_MPI / BC_MPI_OT, M3:800-1254) are device kernels with the
+// reference's x -> y -> z pass order and x-inflow rules; every stage kernel
+// reads the arrays its reference routine reads, so the fills between stages
+// reproduce the reference's halo contents exactly.
+#include "mcf_kernels.h"
+
+namespace mcf {
+
+// ---------------------------------------------------------------------------
+// index helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ix3(const Geo &g, int i, int j, int k) {
+  return (i + 1) + (j + 1) * g.st[1] + (k + 1) * g.st[2];
+}
+
+// thread -> (i,j,k) of a box, x fastest; false when out of range
+__device__ __forceinline__ bool box_ijk(const Box &b, int &i, int &j, int &k) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long n01 = (long)b.cnt[0] * b.cnt[1];
+  if (t >= n01 * b.cnt[2]) return false;
+  k = b.lo[2] + (int)(t / n01);
+  long r = t - (long)(k - b.lo[2]) * n01;
+  j = b.lo[1] + (int)(r / b.cnt[0]);
+  i = b.lo[0] + (int)(r - (long)(j - b.lo[1]) * b.cnt[0]);
+  return true;
+}
+
+__device__ __forceinline__ double gam_of(const double *const *sp, int c) {
+  double s4 = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c];
+  double g = 3.0 * s4 + 5.0 * sp[2][c];
+  return (5.0 * s4 + 7.0 * sp[2][c]) / g;
+}
+__device__ __forceinline__ double gammi1_of(const double *const *sp, int c) {
+  double s4 = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c];
+  double g = 3.0 * s4 + 5.0 * sp[2][c];
+  return (2.0 * s4 + 2.0 * sp[2][c]) / g;
+}
+
+// ---------------------------------------------------------------------------
+// ghost fills
+// ---------------------------------------------------------------------------
+// Periodic copy of the nol ghost layers on both sides of axis d, over the
+// full transverse index range, for up to 16 arrays.
+__global__ void k_fill_periodic(FillList L, Geo g, int d, int nol) {
+  int n = g.n[d];
+  int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
+  int na = g.n[da] + 4, nb = g.n[db] + 4;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)na * nb * (2 * nol);
+  if (t >= tot) return;
+  int a = (int)(t % na);
+  long r = t / na;
+  int b = (int)(r % nb);
+  int layer = (int)(r / nb); // 0..2*nol-1
+  int sd = g.st[d];
+  int base = a * g.st[da] + b * g.st[db] + sd; // position 0 along d
+  int gpos, spos;
+  if (layer < nol) { gpos = 1 - nol + layer; spos = gpos + n; }
+  else { gpos = n + 1 + (layer - nol); spos = gpos - n; }
+  for (int f = 0; f < L.nf; ++f) L.a[f][base + gpos * sd] = L.a[f][base + spos * sd];
+}
+
+// x-ghost rules of BC_MPI / BC_MPI_OT at an inflow (BC=4) end
+// (M3:818-884, 948-1092).  One thread per ghost point and side; the fields
+// are processed in list order, as the reference's K loop does.
+__global__ void k_fill_xrule(XFill X, Geo g, int nol) {
+  int ny4 = g.n[1] + 4, nz4 = g.n[2] + 4, nx = g.n[0];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)ny4 * nz4 * (2 * nol);
+  if (t >= tot) return;
+  int j = (int)(t % ny4) - 1;
+  long r = t / ny4;
+  int k = (int)(r % nz4) - 1;
+  int layer = (int)(r / nz4);
+  int side = layer < nol ? 0 : 1;
+  int i = side == 0 ? 1 - nol + layer : nx + 1 + (layer - nol);
+  int c = ix3(g, i, j, k);
+  bool inner = (j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2]);
+  for (int f = 0; f < X.nf; ++f) {
+    double *a = X.a[f];
+    switch (X.rule[f][side]) {
+    case XR_PERIODIC: a[c] = a[ix3(g, side == 0 ? i + nx : i - nx, j, k)]; break;
+    case XR_CONST: a[c] = X.val[f][side]; break;
+    case XR_COPY: a[c] = a[ix3(g, X.src[f][side], j, k)]; break;
+    case XR_FACE: // U(:,:,:,6): left copies face 1, right only N+2 copies face N+1
+      if (side == 0) a[c] = a[ix3(g, 1, j, k)];
+      else if (i >= nx + 2) a[c] = a[ix3(g, nx + 1, j, k)];
+      break;
+    case XR_MOM: a[c] = X.rho[c] * X.val[f][side]; break;
+    case XR_ENERGY: {
+      double v = X.val2[side];
+      a[c] = X.val[f][side] * 1.5 + 0.5 * (X.rho[c] * (v * v) + X.bcc1[c] * X.bcc1[c] +
+                                          X.blg1[c] * X.blg1[c] + X.blg2[c] * X.blg2[c]);
+      break;
+    }
+    case XR_SPECIES:
+      if (inner) a[c] = X.val[f][side] * X.rho[c];
+      break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RHS (M3:1330-1467)
+// ---------------------------------------------------------------------------
+// One thread per interface i = 0..N (between cells i and i+1 along d).
+__global__ void k_rhs_face(SweepArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d = A.d;
+  const int s = g.st[d];
+  const int q = d == 0 ? i : d == 1 ? j : k;
+  const int c = ix3(g, i, j, k), cp = c + s;
+  const double *xe = A.dxe; // dxelr, index -1..N+2 at [q+1]
+  const double *U1 = A.U[0], *UVN = A.U[A.VN], *U5 = A.U[4];
+  const double *B1 = A.bcc[A.BT1], *B2 = A.bcc[A.BT2];
+  double dxm = xe[q], dx0 = xe[q + 1], dxp = xe[q + 2], dxpp = xe[q + 3];
+  // slopes at q and q+1 (VLIMIT k = 1, VN, 5, BT1+8, BT2+8)
+  double T0 = tswitch(U5[c], U1[c]), T1 = tswitch(U5[cp], U1[cp]);
+  double g1a = vlim_pick(vlim_parts(U1[c - s], U1[c], U1[cp], dxm, dx0, dxp), T0);
+  double g1b = vlim_pick(vlim_parts(U1[c], U1[cp], U1[cp + s], dx0, dxp, dxpp), T1);
+  double gva = vlim_pick(vlim_parts(UVN[c - s], UVN[c], UVN[cp], dxm, dx0, dxp), T0);
+  double gvb = vlim_pick(vlim_parts(UVN[c], UVN[cp], UVN[cp + s], dx0, dxp, dxpp), T1);
+  double g5a = vlim_pick(vlim_parts(U5[c - s], U5[c], U5[cp], dxm, dx0, dxp), T0);
+  double g5b = vlim_pick(vlim_parts(U5[c], U5[cp], U5[cp + s], dx0, dxp, dxpp), T1);
+  double gAa = vlim_pick(vlim_parts(B1[c - s], B1[c], B1[cp], dxm, dx0, dxp), T0);
+  double gAb = vlim_pick(vlim_parts(B1[c], B1[cp], B1[cp + s], dx0, dxp, dxpp), T1);
+  double gBa = vlim_pick(vlim_parts(B2[c - s], B2[c], B2[cp], dxm, dx0, dxp), T0);
+  double gBb = vlim_pick(vlim_parts(B2[c], B2[cp], B2[cp + s], dx0, dxp, dxpp), T1);
+  const double *const *sp = A.sp;
+  double gamma = gam_of(sp, c);
+  double cm = sqrt((gamma * U5[c] + B1[c] * B1[c] + B2[c] * B2[c]) / U1[c]);
+  double depend1 = 0.5 * (dx0 - cm * dt) * A.facdep;
+  depend1 = dmax(0.0, depend1);
+  gamma = gam_of(sp, cp);
+  cm = sqrt((gamma * U5[cp] + B1[cp] * B1[cp] + B2[cp] * B2[cp]) / U1[cp]);
+  double depend2 = 0.5 * (dxp - cm * dt) * A.facdep;
+  depend2 = dmax(0.0, depend2);
+  double QL1 = posfix(U1[c] + depend1 * g1a, U1[c]);
+  double QR1 = posfix(U1[cp] - depend2 * g1b, U1[cp]);
+  double QL2 = UVN[c] + depend1 * gva;
+  double QR2 = UVN[cp] - depend2 * gvb;
+  double QL3 = posfix(U5[c] + depend1 * g5a, U5[c]);
+  double QR3 = posfix(U5[cp] - depend2 * g5b, U5[cp]);
+  double a1 = B1[c] + depend1 * gAa, a2 = B2[c] + depend1 * gBa;
+  double QL4 = a1 * a1 + a2 * a2;
+  double b1 = B1[cp] - depend2 * gAb, b2 = B2[cp] - depend2 * gBb;
+  double QR4 = b1 * b1 + b2 * b2;
+  double ndHm = 0.5 * (sp[0][c] + sp[0][cp]);
+  double ndpm = 0.5 * (sp[1][c] + sp[1][cp]);
+  double ndHem = 0.5 * (sp[3][c] + sp[3][cp]);
+  double ndHepm = 0.5 * (sp[4][c] + sp[4][cp]);
+  double ndH2m = 0.5 * (sp[2][c] + sp[2][cp]);
+  double gm = (5.0 * (ndHm + ndpm + ndHem + ndHepm) + 7.0 * ndH2m) / (3.0 * (ndHm + ndpm + ndHem + ndHepm) + 5.0 * ndH2m);
+  double Pa, Va;
+  rieman(QL1, QL2, QL3, QL4, QR1, QR2, QR3, QR4, gm, Pa, Va);
+  A.pa[c] = Pa;
+  A.va[c] = Va;
+  A.xlag[c] = A.xe[q + 1] + Va * dt;
+}
+
+// One thread per cell i = 1..N: Lagrangian conservation update.
+__global__ void k_rhs_cell(SweepArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d = A.d;
+  const int s = g.st[d];
+  const int q = d == 0 ? i : d == 1 ? j : k;
+  const int c = ix3(g, i, j, k), cm = c - s;
+  double *U1 = A.U[0];
+  double rho = U1[c];
+  double gi = gammi1_of(A.sp, c);
+  double vx = A.U[1][c], vy = A.U[2][c], vz = A.U[3][c];
+  double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
+  double e = A.U[4][c] / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
+  double dxe = A.dxe[q + 1];
+  double dm = dxe * rho;
+  double xl = A.xlag[c], xlm = A.xlag[cm];
+  double pa = A.pa[c], pam = A.pa[cm], va = A.va[c], vam = A.va[cm];
+  U1[c] = dm / (xl - xlm);
+  A.U[A.VN][c] = A.U[A.VN][c] - dt / dm * (pa - pam);
+  A.u5e[c] = e - dt / dm * (pa * va - pam * vam);
+  A.blg[0][c] = A.bcc[A.BT1][c] * dxe / (xl - xlm);
+  A.blg[1][c] = A.bcc[A.BT2][c] * dxe / (xl - xlm);
+  for (int f = 0; f < 8; ++f) A.sp[f][c] = dxe * A.sp[f][c] / (xl - xlm);
+  A.dxlagM[c] = xl - xlm;
+}
+
+// ---------------------------------------------------------------------------
+// MOC (M3:1939-2098)
+// ---------------------------------------------------------------------------
+struct MocOut {
+  double Bay, Vay, Baz, Vaz;
+};
+
+__device__ __forceinline__ MocOut moc_iface(const SweepArgs &A, int c, int cp, int q, const double *dxl,
+                                            const double *gi, const double *gp, double dt) {
+  const double *U1 = A.U[0], *UT1 = A.U[A.VT1], *UT2 = A.U[A.VT2], *UBn = A.U[A.BN];
+  const double *Bl1 = A.blg[0], *Bl2 = A.blg[1];
+  const double fac = A.facdep;
+  const double *xe = A.xe; // x(q) at [q+1]
+  int cpp = cp + (cp - c);
+  double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
+  double deni = sqrt((dxl[1] + dxl[2]) / (dxl[1] * U1[cp] + dxl[2] * U1[c]));
+  double depend1, depend2, ca;
+  double QL11, QL12, QL13, QR11, QR12, QR13, QL21, QL22, QR21, QR22;
+  if (Bnap > 0.0) {
+    ca = fabs(Bnap * deni) * 0.5 * dt;
+    ca = dmin(dxl[1], ca);
+    depend1 = (0.5 * dxl[1] - ca) * fac;
+    ca = fabs(Bnap * deni) * 0.5 * dt;
+    ca = dmin(dxl[2], ca);
+    depend2 = (0.5 * dxl[2] - ca) * fac;
+    QL11 = UT1[c] + depend1 * gi[1];
+    QR11 = UT1[cp] - depend2 * gp[1];
+    QL12 = Bl1[c] + depend1 * gi[3];
+    QR12 = Bl1[cp] - depend2 * gp[3];
+    QL13 = posfix(U1[c] + depend1 * gi[0], U1[c]);
+    QR13 = posfix(U1[cp] - depend2 * gp[0], U1[cp]);
+    QL21 = UT2[c] + depend1 * gi[2];
+    QR21 = UT2[cp] - depend2 * gp[2];
+    QL22 = Bl2[c] + depend1 * gi[4];
+    QR22 = Bl2[cp] - depend2 * gp[4];
+  } else {
+    ca = fabs(Bnap * deni) * 0.5 * dt;
+    ca = dmin(dxl[2], ca);
+    depend1 = (0.5 * dxl[2] - ca) * fac;
+    ca = fabs(Bnap * deni) * 0.5 * dt;
+    ca = dmin(dxl[1], ca);
+    depend2 = (0.5 * dxl[1] - ca) * fac;
+    QL11 = UT1[cp] - depend1 * gp[1];
+    QR11 = UT1[c] + depend2 * gi[1];
+    QL12 = Bl1[cp] - depend1 * gp[3];
+    QR12 = Bl1[c] + depend2 * gi[3];
+    QL13 = posfix(U1[cp] - depend1 * gp[0], U1[cp]);
+    QR13 = posfix(U1[c] + depend2 * gi[0], U1[c]);
+    QL21 = UT2[cp] - depend1 * gp[2];
+    QR21 = UT2[c] + depend2 * gi[2];
+    QL22 = Bl2[cp] - depend1 * gp[4];
+    QR22 = Bl2[c] + depend2 * gi[4];
+  }
+  MocOut o;
+  chareq(QL11, QL12, QL13, QR11, QR12, QR13, o.Bay, o.Vay);
+  chareq(QL21, QL22, QL13, QR21, QR22, QR13, o.Baz, o.Vaz);
+  return o;
+}
+
+// One thread per interface q = 0..N.  At an MPI block edge of the reference
+// run the interface is evaluated twice: once as the left block sees it (the
+// right cell is a ghost whose U(:,:,:,5) still holds the pre-sweep pressure)
+// and once as the right block sees it.
+__global__ void k_moc_face(SweepArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d = A.d;
+  const int s = g.st[d];
+  const int q = d == 0 ? i : d == 1 ? j : k;
+  const int n = g.n[d];
+  const int c = ix3(g, i, j, k), cp = c + s;
+  double dxl[4] = {A.dxlagM[c - s], A.dxlagM[c], A.dxlagM[cp], A.dxlagM[cp + s]};
+  const double *U1 = A.U[0];
+  const double *Q[5] = {A.U[0], A.U[A.VT1], A.U[A.VT2], A.blg[0], A.blg[1]};
+  Slope2 sa[5], sb[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    sa[v] = vlim_parts(Q[v][c - s], Q[v][c], Q[v][cp], dxl[0], dxl[1], dxl[2]);
+    sb[v] = vlim_parts(Q[v][c], Q[v][cp], Q[v][cp + s], dxl[1], dxl[2], dxl[3]);
+  }
+  // limiter switch: interior cells hold e in u5e, ghost copies hold pressure
+  double Tp0 = tswitch(A.U[4][c], U1[c]), Tp1 = tswitch(A.U[4][cp], U1[cp]);
+  double Te0 = (q >= 1) ? tswitch(A.u5e[c], U1[c]) : Tp0;
+  double Te1 = (q + 1 <= n) ? tswitch(A.u5e[cp], U1[cp]) : Tp1;
+  bool edge = (q % A.nblk) == 0;
+  double gi[5], gp[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    gi[v] = vlim_pick(sa[v], Te0);
+    gp[v] = vlim_pick(sb[v], edge ? Tp1 : Te1);
+  }
+  MocOut L = moc_iface(A, c, cp, q, dxl, gi, gp, dt);
+  MocOut R = L;
+  if (edge) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      gi[v] = vlim_pick(sa[v], Tp0);
+      gp[v] = vlim_pick(sb[v], Te1);
+    }
+    R = moc_iface(A, c, cp, q, dxl, gi, gp, dt);
+  }
+  A.mf[0][c] = L.Bay; A.mf[1][c] = L.Vay; A.mf[2][c] = L.Baz; A.mf[3][c] = L.Vaz;
+  A.mf[4][c] = R.Bay; A.mf[5][c] = R.Vay; A.mf[6][c] = R.Baz; A.mf[7][c] = R.Vaz;
+}
+
+__global__ void k_moc_cell(SweepArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d = A.d;
+  const int s = g.st[d];
+  const int q = d == 0 ? i : d == 1 ? j : k;
+  const int c = ix3(g, i, j, k), cm = c - s, cp = c + s, cpp = cp + s;
+  const double *UBn = A.U[A.BN];
+  const double *xe = A.xe;
+  double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
+  double Bnam = UBn[c] + (A.xlag[cm] - xe[q]) * (UBn[cp] - UBn[cm]) / (xe[q + 1] - xe[q - 1]);
+  double Vnap = A.va[c], Vnam = A.va[cm];
+  double dxl = A.dxlagM[c];
+  double rho = A.U[0][c];
+  double BayL = A.mf[0][c], VayL = A.mf[1][c], BazL = A.mf[2][c], VazL = A.mf[3][c];
+  double BayR = A.mf[4][cm], VayR = A.mf[5][cm], BazR = A.mf[6][cm], VazR = A.mf[7][cm];
+  double *UVN = A.U[A.VN], *UT1 = A.U[A.VT1], *UT2 = A.U[A.VT2];
+  UVN[c] = UVN[c] * rho + dt / dxl * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+  UT1[c] = UT1[c] * rho + dt / dxl * (Bnap * BayL - Bnam * BayR);
+  UT2[c] = UT2[c] * rho + dt / dxl * (Bnap * BazL - Bnam * BazR);
+  A.U[4][c] = A.u5e[c] * rho + dt / dxl *
+                                   ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                    (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+  A.blg[0][c] = A.blg[0][c] + dt / dxl * (Bnap * VayL - Bnam * VayR);
+  A.blg[1][c] = A.blg[1][c] + dt / dxl * (Bnap * VazL - Bnam * VazR);
+}
+
+// ---------------------------------------------------------------------------
+// REMAP (M3:1556-1686) + the conversion back to primitives in MHD
+// (M3:1291-1321)
+// ---------------------------------------------------------------------------
+// One thread per interface q = 0..N: upwind linear fluxes of 15 quantities.
+__global__ void k_remap_face(SweepArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d = A.d;
+  const int s = g.st[d];
+  const int c = ix3(g, i, j, k);
+  double vadt = A.va[c] * dt;
+  int u;
+  double depend, sgn;
+  if (vadt <= 0.0) {
+    u = c + s;
+    depend = 0.5 * (A.dxlagM[u] + vadt) * A.facdep;
+    sgn = -1.0;
+  } else {
+    u = c;
+    depend = 0.5 * (A.dxlagM[u] - vadt) * A.facdep;
+    sgn = 1.0;
+  }
+  double dxm = A.dxlagM[u - s], dx0 = A.dxlagM[u], dxp = A.dxlagM[u + s];
+  double T = tswitch(A.U[4][u], A.U[0][u]);
+#pragma unroll
+  for (int v = 0; v < 15; ++v) {
+    const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
+    double gr = vlim_pick(vlim_parts(Qv[u - s], Qv[u], Qv[u + s], dxm, dx0, dxp), T);
+    double face = sgn > 0.0 ? Qv[u] + depend * gr : Qv[u] - depend * gr;
+    A.fl[v][c] = -(vadt * face);
+  }
+}
+
+__global__ void k_remap_cell(SweepArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d = A.d;
+  const int s = g.st[d];
+  const int q = d == 0 ? i : d == 1 ? j : k;
+  const int c = ix3(g, i, j, k), cm = c - s;
+  double dxl = A.dxlagM[c];
+  double dxE = A.xe[q + 1] - A.xe[q];
+  double Qn[15];
+#pragma unroll
+  for (int v = 0; v < 15; ++v) {
+    const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
+    Qn[v] = (Qv[c] * dxl + A.fl[v][c] - A.fl[v][cm]) / dxE;
+  }
+  // MHD: back to primitive variables with floors
+  double rho = dmax(1.27 * A.rmin, Qn[0]);
+  double invd = 1.0 / rho;
+  double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
+  double s4 = Qn[7] + Qn[8] + Qn[10] + Qn[11];
+  double gg = 3.0 * s4 + 5.0 * Qn[9];
+  double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
+  double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rho;
+  double bn = A.bcc[d][c];
+  double emag = 0.5 * (bn * bn + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
+  double p = (Qn[4] - ekin - emag) * gi;
+  p = dmax(A.pmin, p);
+  p = dmin(A.pmax, p);
+  A.U[0][c] = rho;
+  A.U[1][c] = vx;
+  A.U[2][c] = vy;
+  A.U[3][c] = vz;
+  A.U[4][c] = p;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) A.sp[f][c] = dmax(A.spmin[f], Qn[7 + f]);
+}
+
+// ---------------------------------------------------------------------------
+// Constrained transport (M3:2115-2445)
+// ---------------------------------------------------------------------------
+// CC(1): face velocities (dx-weighted) on the interior faces
+__global__ void k_cc1(CtArgs A, Geo g, Box b) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  int c = ix3(g, i, j, k);
+  int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  if (j <= ny && k <= nz) {
+    double a = A.gdx[0][i], bb = A.gdx[0][i + 1]; // dx(i-1), dx(i)
+    A.vfc[0][c] = (a * A.U[1][c] + bb * A.U[1][c - 1]) / (a + bb);
+  }
+  if (i <= nx && k <= nz) {
+    double a = A.gdx[1][j], bb = A.gdx[1][j + 1];
+    A.vfc[1][c] = (a * A.U[2][c] + bb * A.U[2][c - g.st[1]]) / (a + bb);
+  }
+  if (i <= nx && j <= ny) {
+    double a = A.gdx[2][k], bb = A.gdx[2][k + 1];
+    A.vfc[2][c] = (a * A.U[3][c] + bb * A.U[3][c - g.st[2]]) / (a + bb);
+  }
+}
+
+// CCT slopes: mode 0 -> dQ2/dQ4 (along d1 of Vfc_d2, B_d2), mode 1 ->
+// dQ1/dQ3 (along d2 of Vfc_d1, B_d1) (M3:2203-2268).
+__global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  int d1 = orient, d2 = (orient + 1) % 3;
+  int ax = mode == 0 ? d1 : d2;
+  int fv = mode == 0 ? d2 : d1;
+  int c = ix3(g, i, j, k);
+  int s = g.st[ax];
+  int p = ax == 0 ? i : ax == 1 ? j : k;
+  const double *dx = A.gdx[ax];
+  double dxm = dx[p], dx0 = dx[p + 1], dxp = dx[p + 2];
+  double T = tswitch(A.U[4][c], A.U[0][c]);
+  const double *V = A.vfc[fv], *B = A.U[5 + fv];
+  double sv = vlim_pick(vlim_parts(V[c - s], V[c], V[c + s], dxm, dx0, dxp), T);
+  double sbv = vlim_pick(vlim_parts(B[c - s], B[c], B[c + s], dxm, dx0, dxp), T);
+  if (mode == 0) { A.dq[1][c] = sv; A.dq[3][c] = sbv; }
+  else { A.dq[0][c] = sv; A.dq[2][c] = sbv; }
+}
+
+// CCT edge EMF (M3:2271-2441): one thread per edge.
+__global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  const int d1 = orient, d2 = (orient + 1) % 3;
+  const int c0 = ix3(g, i, j, k);
+  const int s1 = g.st[d1], s2 = g.st[d2];
+  const int p1 = d1 == 0 ? i : d1 == 1 ? j : k;
+  const int p2 = d2 == 0 ? i : d2 == 1 ? j : k;
+  const double *r = A.U[0];
+  double dnc;
+  if (orient == 2) dnc = 0.25 * (r[c0] + r[c0 - s2] + r[c0 - s1] + r[c0 - s1 - s2]);
+  else dnc = 0.25 * (r[c0] + r[c0 - s1] + r[c0 - s2] + r[c0 - s1 - s2]);
+  const double dsqinv = 1.0 / sqrt(dnc);
+  const double *Vx = A.vfc[d1], *Vy = A.vfc[d2], *Bx = A.U[5 + d1], *By = A.U[5 + d2];
+  const double facm = A.facdep * 0.5;
+  const double *dxe = A.gdx[d1], *dye = A.gdx[d2];
+  double C1p[4], C1m[4], C2p[4], C2m[4];
+#pragma unroll
+  for (int cs = 0; cs < 4; ++cs) {
+    // 0: LL (S1=+1,S2=+1)  1: RL (-1,+1)  2: LR (+1,-1)  3: RR (-1,-1)
+    double S1 = (cs == 0 || cs == 2) ? 1.0 : -1.0;
+    double S2 = (cs == 0 || cs == 1) ? 1.0 : -1.0;
+    int q1 = (cs == 0 || cs == 2) ? c0 - s2 : c0;
+    int q2 = (cs == 0 || cs == 1) ? c0 - s1 : c0;
+    double dyv = (cs == 0 || cs == 2) ? dye[p2] : dye[p2 + 1];
+    double dxv = (cs == 0 || cs == 1) ? dxe[p1] : dxe[p1 + 1];
+    double a1p = Vx[q1] + Bx[q1] * dsqinv;
+    double da1p = (A.dq[0][q1] + A.dq[2][q1] * dsqinv) * facm;
+    double a1m = Vx[q1] - Bx[q1] * dsqinv;
+    double da1m = (A.dq[0][q1] - A.dq[2][q1] * dsqinv) * facm;
+    double a2m = Vy[q2] - By[q2] * dsqinv;
+    double da2m = (A.dq[1][q2] - A.dq[3][q2] * dsqinv) * facm;
+    double a2p = Vy[q2] + By[q2] * dsqinv;
+    double da2p = (A.dq[1][q2] + A.dq[3][q2] * dsqinv) * facm;
+    double t;
+    t = a1p + da1p * dyv * S1 - da1p * (a2m + da2m * dxv * S2) * dt;
+    C1p[cs] = t / (1.0 - da1p * da2m * dt * dt);
+    t = a2m + da2m * dxv * S2 - da2m * (a1p + da1p * dyv * S1) * dt;
+    C2m[cs] = t / (1.0 - da1p * da2m * dt * dt);
+    t = a1m + da1m * dyv * S1 - da1m * (a2p + da2p * dxv * S2) * dt;
+    C1m[cs] = t / (1.0 - da1m * da2p * dt * dt);
+    t = a2p + da2p * dxv * S2 - da2p * (a1m + da1m * dyv * S1) * dt;
+    C2p[cs] = t / (1.0 - da1m * da2p * dt * dt);
+  }
+  int f0[4], f1[4];
+  f0[0] = (C1p[0] > 0.0) && (C2m[0] > 0.0);
+  f1[0] = (C1m[0] > 0.0) && (C2p[0] > 0.0);
+  f0[1] = (C1p[1] > 0.0) && (C2m[1] < 0.0);
+  f1[1] = (C1m[1] > 0.0) && (C2p[1] < 0.0);
+  f0[2] = (C1p[2] < 0.0) && (C2m[2] > 0.0);
+  f1[2] = (C1m[2] < 0.0) && (C2p[2] > 0.0);
+  f0[3] = (C1p[3] < 0.0) && (C2m[3] < 0.0);
+  f1[3] = (C1m[3] < 0.0) && (C2p[3] < 0.0);
+  double W0[4], W1[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    int both = (int)(0.51 * (double)f0[m] + 0.51 * (double)f1[m]);
+    W0[m] = (double)f0[m] + (1.0e13 - 1.0) * (double)both;
+    W1[m] = (double)f1[m] + (1.0e13 - 1.0) * (double)both;
+  }
+  double c1p = 0.0, c2m = 0.0, c1m = 0.0, c2p = 0.0;
+  if (f0[0] + f0[1] + f0[2] + f0[3] != 0) {
+    c1p = W0[0] * fabs(C2m[0]) * C1p[0] + W0[1] * fabs(C2m[1]) * C1p[1] + W0[2] * fabs(C2m[2]) * C1p[2] +
+          W0[3] * fabs(C2m[3]) * C1p[3];
+    c1p = c1p / (W0[0] * fabs(C2m[0]) + W0[1] * fabs(C2m[1]) + W0[2] * fabs(C2m[2]) + W0[3] * fabs(C2m[3]));
+    c2m = W0[0] * fabs(C1p[0]) * C2m[0] + W0[1] * fabs(C1p[1]) * C2m[1] + W0[2] * fabs(C1p[2]) * C2m[2] +
+          W0[3] * fabs(C1p[3]) * C2m[3];
+    c2m = c2m / (W0[0] * fabs(C1p[0]) + W0[1] * fabs(C1p[1]) + W0[2] * fabs(C1p[2]) + W0[3] * fabs(C1p[3]));
+  }
+  if (f1[0] + f1[1] + f1[2] + f1[3] != 0) {
+    c1m = W1[0] * fabs(C2p[0]) * C1m[0] + W1[1] * fabs(C2p[1]) * C1m[1] + W1[2] * fabs(C2p[2]) * C1m[2] +
+          W1[3] * fabs(C2p[3]) * C1m[3];
+    c1m = c1m / (W1[0] * fabs(C2p[0]) + W1[1] * fabs(C2p[1]) + W1[2] * fabs(C2p[2]) + W1[3] * fabs(C2p[3]));
+    c2p = W1[0] * fabs(C1m[0]) * C2p[0] + W1[1] * fabs(C1m[1]) * C2p[1] + W1[2] * fabs(C1m[2]) * C2p[2] +
+          W1[3] * fabs(C1m[3]) * C2p[3];
+    c2p = c2p / (W1[0] * fabs(C1m[0]) + W1[1] * fabs(C1m[1]) + W1[2] * fabs(C1m[2]) + W1[3] * fabs(C1m[3]));
+  }
+  double v1a = c1p + c1m, v2a = c2p + c2m, B1a = c1p - c1m, B2a = c2p - c2m;
+  A.emf[(orient + 2) % 3][c0] = 0.25 * (v1a * B2a - v2a * B1a) * sqrt(dnc);
+}
+
+// CC(3): face-B induction (M3:2159-2169)
+__global__ void k_cc3(CtArgs A, Geo g, Box b, double dt) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  int c = ix3(g, i, j, k);
+  int sx = 1, sy = g.st[1], sz = g.st[2];
+  const double *Ex = A.emf[0], *Ey = A.emf[1], *Ez = A.emf[2];
+  double dxi = A.gdx[0][i + 1], dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
+  if (j <= ny && k <= nz)
+    A.U[5][c] = A.U[5][c] + dt / dyj * (Ez[c + sy] - Ez[c]) - dt / dzk * (Ey[c + sz] - Ey[c]);
+  if (i <= nx && k <= nz)
+    A.U[6][c] = A.U[6][c] + dt / dzk * (Ex[c + sz] - Ex[c]) - dt / dxi * (Ez[c + sx] - Ez[c]);
+  if (i <= nx && j <= ny)
+    A.U[7][c] = A.U[7][c] + dt / dxi * (Ey[c + sx] - Ey[c]) - dt / dyj * (Ex[c + sy] - Ex[c]);
+}
+
+// CC(4): cell-centred field (M3:2171-2181)
+__global__ void k_cc4(CtArgs A, Geo g, Box b) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  int c = ix3(g, i, j, k);
+  A.bcc[0][c] = 0.5 * (A.U[5][c] + A.U[5][c + 1]);
+  A.bcc[1][c] = 0.5 * (A.U[6][c] + A.U[6][c + g.st[1]]);
+  A.bcc[2][c] = 0.5 * (A.U[7][c] + A.U[7][c + g.st[2]]);
+}
+
+// ---------------------------------------------------------------------------
+// SOURCE (M3:2448-2572), KI cooling + saturated conduction
+// ---------------------------------------------------------------------------
+__global__ void k_src_temp(SrcArgs A, Geo g, Box b, int savep) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  int c = ix3(g, i, j, k);
+  const double *const *sp = A.sp;
+  A.Tn[c] = A.U5[c] / (KB * (sp[1][c] + sp[0][c] + sp[2][c] + sp[3][c] + sp[4][c]));
+  if (savep) A.Pn[c] = A.U5[c];
+}
+
+__device__ __forceinline__ double qflux(const double *Tn, int c, int s, double dxi, double dxp) {
+  double rt = sqrt((dxi * Tn[c + s] + dxp * Tn[c]) / (dxi + dxp));
+  return KCOND * dmin(rt, 3.873) * (Tn[c + s] - Tn[c]) * 2.0 / (dxi + dxp);
+}
+
+__global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  int c = ix3(g, i, j, k);
+  const double *const *sp = A.sp;
+  const double *Tn = A.Tn;
+  double rho = A.rho[c];
+  double T = Tn[c];
+  double CooL = fcool_ki(rho, T);
+  double gi = gammi1_of(sp, c);
+  double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
+  double Pn = A.Pn[c];
+  double u;
+  if (stage == 0) {
+    u = A.U5[c];
+    if (dt <= 0.2 * Pn / (gi * fabs(CooL))) u = u - gi * CooL * dt * 0.5;
+    else u = imc(u, nsum, dt * 0.5, rho, gi);
+  } else {
+    if (dt <= 0.2 * Pn / (gi * fabs(CooL))) u = Pn - gi * CooL * dt;
+    else {
+      Pn = imc(Pn, nsum, dt, rho, gi);
+      A.Pn[c] = Pn;
+      u = Pn;
+    }
+  }
+  double dxi = A.gdx[0][i + 1], dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
+  double tcd = gi * KCOND * sqrt(T);
+  tcd = 0.49 * nsum * KB / tcd * (dxi * dxi);
+  tcd = copysign(0.5, tcd - dt);
+  int sy = g.st[1], sz = g.st[2];
+  double qxp = qflux(Tn, c, 1, dxi, A.gdx[0][i + 2]);
+  double qxm = qflux(Tn, c - 1, 1, A.gdx[0][i], dxi);
+  double qyp = qflux(Tn, c, sy, dyj, A.gdx[1][j + 2]);
+  double qym = qflux(Tn, c - sy, sy, A.gdx[1][j], dyj);
+  double qzp = qflux(Tn, c, sz, dzk, A.gdx[2][k + 2]);
+  double qzm = qflux(Tn, c - sz, sz, A.gdx[2][k], dzk);
+  double div = (qxp - qxm) / dxi + (qyp - qym) / dyj + (qzp - qzm) / dzk;
+  if (stage == 0) u = u + (tcd + 0.5) * gi * dt * 0.5 * div;
+  else u = u + (tcd + 0.5) * gi * dt * div;
+  u = dmax(u, A.pmin);
+  u = dmin(u, A.pmax);
+  A.U5[c] = u;
+}
+
+// ---------------------------------------------------------------------------
+// Couran + Stblty: min over interior cells
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ void k_dtmin(DtArgs A, Geo g, Box b) {
+  __shared__ double s0[16], s1[16];
+  int i, j, k;
+  double tc = A.tfinal, tl = A.tfinal;
+  if (box_ijk(b, i, j, k)) {
+    int c = ix3(g, i, j, k);
+    const double *const *sp = A.sp;
+    double rho = A.U[0][c], p = A.U[4][c];
+    double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
+    double gamma = gam_of(sp, c);
+    double c2 = (gamma * p + bx * bx + by * by + bz * bz) / rho;
+    double dxi = A.gdx[0][i + 1], dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
+    double t = dxi / (sqrt(c2) + fabs(A.U[1][c]));
+    tc = dmin(tc, t);
+    t = dyj / (sqrt(c2) + fabs(A.U[2][c]));
+    tc = dmin(tc, t);
+    t = dzk / (sqrt(c2) + fabs(A.U[3][c]));
+    tc = dmin(tc, t);
+    double Nn = sp[1][c] + sp[0][c] + sp[2][c] + sp[3][c] + sp[4][c];
+    double Tn = p / (KB * Nn);
+    double dl = dmin(dxi, dyj);
+    dl = dmin(dl, dzk);
+    double gi = gammi1_of(sp, c);
+    double alpha = gi * KCOND * sqrt(dmin(Tn, 15.0));
+    alpha = 0.4 * Nn * KB / alpha * (dl * dl);
+    double CooL = fcool_ki(rho, Tn);
+    double tauC = p / gi / fabs(CooL);
+    tauC = dmax(0.2 * tauC, 2.5e-4);
+    tl = dmin(tl, tauC);
+    tl = dmin(tl, alpha);
+  }
+  tc = wave_min(tc);
+  tl = wave_min(tl);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { s0[wid] = tc; s1[wid] = tl; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) { tc = fmin(tc, s0[w]); tl = fmin(tl, s1[w]); }
+    // positive doubles order like their bit patterns
+    atomicMin((unsigned long long *)&A.out[0], (unsigned long long)__double_as_longlong(tc));
+    atomicMin((unsigned long long *)&A.out[1], (unsigned long long)__double_as_longlong(tl));
+  }
+}
+
+__global__ void k_dtinit(double *out, double v) {
+  if (threadIdx.x < 2) out[threadIdx.x] = v;
+}
+
+// ---------------------------------------------------------------------------
+// DISSIP (M3:3143-3173)
+// ---------------------------------------------------------------------------
+__global__ void k_dissip_div(DisArgs A, Geo g, int e0) {
+  int ny = g.n[1], nz = g.n[2];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long plane = 12L * (ny + 2) * (nz + 2);
+  long tot = (2 - e0) * plane;
+  if (t >= tot) return;
+  int e = e0 + (int)(t / plane);
+  long r = t % plane;
+  int i = (int)(r % 12);
+  int j = (int)((r / 12) % (ny + 2));
+  int k = (int)(r / (12L * (ny + 2)));
+  int x = i + (e == 0 ? 0 : g.n[0] - 10);
+  int c = ix3(g, x, j, k);
+  int sy = g.st[1], sz = g.st[2];
+  A.divv[(long)e * plane + r] =
+      (A.U[1][c + 1] - A.U[1][c - 1] + A.U[2][c + sy] - A.U[2][c - sy] + A.U[3][c + sz] - A.U[3][c - sz]) * 0.25;
+}
+
+__global__ void k_dissip_upd(DisArgs A, Geo g, int e0) {
+  int ny = g.n[1], nz = g.n[2];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long cnt = 10L * ny * nz;
+  if (t >= (2 - e0) * cnt) return;
+  int e = e0 + (int)(t / cnt);
+  long r = t % cnt;
+  int i = 1 + (int)(r % 10);
+  int j = 1 + (int)((r / 10) % ny);
+  int k = 1 + (int)(r / (10L * ny));
+  long plane = 12L * (ny + 2) * (nz + 2);
+  const double *dv = A.divv + e * plane;
+  auto D = [&](int a, int bb, int cc) { return dv[a + 12L * (bb + (long)(ny + 2) * cc)]; };
+  int c = ix3(g, i + (e == 0 ? 0 : g.n[0] - 10), j, k);
+  A.U[1][c] = A.U[1][c] + 0.1 * (D(i + 1, j, k) - D(i - 1, j, k));
+  A.U[2][c] = A.U[2][c] + 0.1 * (D(i, j + 1, k) - D(i, j - 1, k));
+  A.U[3][c] = A.U[3][c] + 0.1 * (D(i, j, k + 1) - D(i, j, k - 1));
+}
+
+} // namespace mcf

@@ -171,3 +171,149 @@ def face_divergence(st: np.ndarray, p: Input3D) -> np.ndarray:
     d += (by[K, 3:ny + 3, I] - by[K, J, I]) / dye[None, J, None]
     d += (bz[3:nz + 3, J, I] - bz[K, J, I]) / dze[K, None, None]
     return d
+
+
+def brio_wu_input(n: int = 64, ny: int | None = None, periodic_x: bool = True) -> Input3D:
+    """Config 1 box: n cells on [0,1] per axis, CFL 0.4, no cooling.
+
+    The reference's floors (pmin ~ 15.8, 1.27*rmin ~ 0.248; MCF3DSGincfca3.f90:
+    173-176) and its species-derived gamma = 5/3 rule out the textbook
+    Brio-Wu values, so the states are rescaled by the MHD similarity
+    rho -> 2 rho, p -> 200 p, B -> sqrt(200) B (v = 0): rho 2 | 0.25,
+    p 200 | 20, Bx = 10.607, By = +-14.142.  T = p/(kb n) >= 9 everywhere, so
+    the limiter's minmod branch is taken uniformly, and the CFL limit binds
+    dt rather than the cooling-time limit of Stblty."""
+    ny = n if ny is None else ny
+    h = 1.0 / n
+    p = Input3D(np1x=n // 2, np2x=n - n // 2, np1y=ny // 2, np2y=ny - ny // 2, np1z=ny // 2, np2z=ny - ny // 2,
+                ql1x=0.5, ql2x=0.5, ql1y=h * (ny // 2), ql2y=h * (ny - ny // 2),
+                ql1z=h * (ny // 2), ql2z=h * (ny - ny // 2),
+                vinitx1=0.0, vinitx2=0.0, binitx1=0.0, binitx2=0.0, binity1=0.0, binity2=0.0,
+                cfl=0.4, facdep=1.0, tfinal=100.0, ifthrm=0)
+    if periodic_x:
+        p.bcx1 = p.bcx2 = 3
+    return p
+
+
+BRIO_WU = dict(rhoL=2.0, rhoR=0.25, pL=200.0, pR=20.0, bx=0.75 * np.sqrt(200.0), byL=np.sqrt(200.0),
+               byR=-np.sqrt(200.0))
+
+
+def brio_wu_state(p: Input3D) -> np.ndarray:
+    """Planar shock tube along x: left state for x < 0.5, right state beyond
+    (a double shock tube when x is periodic).  Only neutral H is present, so
+    gamma = 5/3; the other species start at the reference's floors."""
+    nx, ny, nz = p.n
+    (xe, dxe), _, _ = grid(p)
+    xc = xe - 0.5 * dxe
+    left = xc < 0.5
+    st = empty_state(p.n)
+    bw = BRIO_WU
+    rho = np.where(left, bw["rhoL"], bw["rhoR"])
+    pres = np.where(left, bw["pL"], bw["pR"])
+    by = np.where(left, bw["byL"], bw["byR"])
+    floors = (0.0, 1e-20, 1e-20, 0.1949628 * 0.09, 1e-20, 1e-20, 1e-20, 1e-20)
+    st[0] = rho[None, None, :]
+    st[4] = pres[None, None, :]
+    st[5] = bw["bx"]
+    st[6] = by[None, None, :]
+    st[8] = rho[None, None, :]
+    for s in range(1, 8):
+        st[8 + s] = floors[s]
+    return st
+
+
+# --------------------------------------------------------------------------
+# Config 2: thermally bistable converging flow
+# --------------------------------------------------------------------------
+KMS = 1.0 / 0.97779        # 1 km/s in code velocity units
+MICROGAUSS = 1.0 / 0.44828  # 1 muG in code field units
+
+
+def park_miller(n: int, idum: int = 1) -> np.ndarray:
+    """ran0 of GenDen/IGEN1.f90 (end of file): Park-Miller minimal standard,
+    Schrage's factorisation, AM = 1./IM evaluated in single precision as the
+    PARAMETER statement does."""
+    IA, IM, IQ, IR = 16807, 2147483647, 127773, 2836
+    am = float(np.float32(1.0) / np.float32(IM))
+    out = np.empty(n)
+    for t in range(n):
+        k = idum // IQ
+        idum = IA * (idum - k * IQ) - IR * k
+        if idum < 0:
+            idum += IM
+        out[t] = am * idum
+    return out
+
+
+def kolmogorov_logfield(n: int, kmax: int | None = None, idum: int = 1) -> np.ndarray:
+    """sum over 0 < |k| <= kmax of |k|^(-11/6) cos(2 pi k.x + phi_k) at the
+    cell centres of an n^3 box (GenDen/IGEN1.f90:102-113), phases phi_k =
+    2 pi ran0 drawn in the reference's loop order (l, ll, lll from -kmax to
+    kmax).  Evaluated as one inverse FFT instead of a direct sum."""
+    kmax = min(64, n // 4) if kmax is None else kmax
+    ks = np.arange(-kmax, kmax + 1)
+    L, LL, LLL = np.meshgrid(ks, ks, ks, indexing="ij")
+    kw = np.sqrt((L ** 2 + LL ** 2 + LLL ** 2).astype(float))
+    sel = (kw > 0) & (kw <= kmax)
+    nmode = int(sel.sum())
+    phase = 2.0 * np.pi * park_miller(nmode)
+    amp = np.sqrt(kw[sel] ** (-11.0 / 3.0))
+    l, ll, lll = L[sel], LL[sel], LLL[sel]
+    # array index m = i-1 sits at the cell centre (m + 1/2)/n: fold the
+    # half-cell shift into the coefficient
+    coef = amp * np.exp(1j * (phase + np.pi * (l + ll + lll) / n))
+    C = np.zeros((n, n, n), dtype=complex)
+    np.add.at(C, (l % n, ll % n, lll % n), coef)
+    u = np.real(np.fft.ifftn(C)) * n ** 3        # u[i_x, i_y, i_z], i = 0..n-1 <-> cells 1..n
+    return np.ascontiguousarray(u.transpose(2, 1, 0))  # -> [k, j, i]
+
+
+def converging_flow_input(n: int = 256, nsplit=(4, 4, 4)) -> Input3D:
+    """Config 2 box: 20 pc cube, x inflow at +-10 km/s, periodic y/z, B = 5 muG
+    at 45 deg in the x-y plane, CFL 0.49, KI cooling + conduction.  The WNM
+    composition of MCF3DSGincfca3.f90:156-157 is scaled to n_tot = 1 cm^-3 and
+    the inflow pressure is set to T = 6000 K."""
+    h = n // 2
+    v = 10.0 * KMS
+    b = 5.0 * MICROGAUSS / np.sqrt(2.0)
+    base = Input3D()
+    ntot0 = base.Hini + base.pini + 2 * base.H2ini + base.Heini + base.Hepini
+    s = 1.0 / ntot0
+    p = Input3D(np1x=h, np2x=n - h, np1y=h, np2y=n - h, np1z=h, np2z=n - h,
+                ql1x=10.0, ql2x=10.0, ql1y=10.0, ql2y=10.0, ql1z=10.0, ql2z=10.0,
+                vinitx1=v, vinitx2=-v, binitx1=b, binitx2=b, binity1=b, binity2=b,
+                cfl=0.49, facdep=1.0, tfinal=100.0, bcx1=4, bcx2=4, ifthrm=2,
+                Hini=base.Hini * s, pini=base.pini * s, H2ini=base.H2ini * s, Heini=base.Heini * s,
+                Hepini=base.Hepini * s, Cini=base.Cini * s, COini=base.COini * s, Cpini=base.Cpini * s)
+    nsum = p.pini + p.Hini + p.H2ini + p.Heini + p.Hepini
+    p.pinit_over_kb = nsum * 6000.0
+    return p
+
+
+def converging_flow_state(p: Input3D, drho: float = 0.1, idum: int = 1) -> np.ndarray:
+    """Config 2 initial state, following the reference's DTF initialisation
+    path (MCF3DSGincfca3.f90:434-447): log-normal Kolmogorov density
+    rho = rho0 exp(a u) with a normalising u to rms drho (GenDen/IGEN1.f90:
+    115-125), species proportional to rho, uniform pressure,
+    vx = -v0 tanh((x(i) - ql1x)/2), uniform face field."""
+    nx, ny, nz = p.n
+    assert nx == ny == nz, "cubic grids only"
+    u = kolmogorov_logfield(nx, idum=idum)
+    a = drho / np.sqrt(np.mean(u ** 2))
+    rho0 = p.Hini + p.pini + 2.0 * p.H2ini + 4.0 * p.Heini + 4.0 * p.Hepini
+    st = empty_state(p.n)
+    rho = np.full((nz + 4, ny + 4, nx + 4), rho0)
+    rho[2:nz + 2, 2:ny + 2, 2:nx + 2] = rho0 * np.exp(a * u)
+    comp = np.array([p.Hini, p.pini, p.H2ini, p.Heini, p.Hepini, p.Cini, p.COini, p.Cpini]) / rho0
+    for s_ in range(8):
+        st[8 + s_] = comp[s_] * rho
+    st[0] = rho
+    st[4] = p.pinit_over_kb * KB * 1.0e-3
+    (xe, _), _, _ = grid(p)
+    st[1] = (-p.vinitx1 * np.tanh(0.5 * (xe - p.ql1x)))[None, None, :]
+    st[5] = p.binitx1
+    st[6] = p.binity1
+    st[7] = p.binitz1
+    _wrap_periodic(st, p.n, (False, True, True))
+    return st

@@ -1,0 +1,127 @@
+/*
+ * mcf_abi.h -- C ABI of the MI355X (gfx950) per-timestep MHD update.
+ *
+ * The reference (3DMHD-CfCA/MCF3DSGincfca3.f90, "M3") has no plugin layer:
+ * EVOLVE (M3:507-679) calls Fortran subroutines that share module-global
+ * arrays.  Each entry point below replaces one of those calls and is bound
+ * from Fortran with ISO_C_BINDING (code_tohoku_amd/fdriver/mcf_abi_mod.f90)
+ * and from Python with ctypes (code_tohoku_amd/mcf.py).
+ *
+ * Conventions: every function returns 0 on success or a negative MCF_E*
+ * code; the last error text is available from mcf_last_error().  Arrays
+ * are FP64 in the Fortran layout A(-1:N+2,-1:N+2,-1:N+2), x fastest.  One
+ * context owns all device memory of one domain on the current HIP device;
+ * calls on one context must not overlap.  Only mcf_timestep, the download
+ * calls and mcf_sync wait for the device.
+ */
+#ifndef MCF_ABI_H
+#define MCF_ABI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MCF_OK = 0,
+  MCF_E_ARG = -1,      /* invalid argument                       */
+  MCF_E_HIP = -2,      /* HIP runtime error                      */
+  MCF_E_NOMEM = -3,    /* device allocation failed                */
+  MCF_E_NONFINITE = -4,/* dt not finite / negative (err at Couran)*/
+  MCF_E_STATE = -5     /* call out of order                      */
+};
+
+enum { MCF_PREC_FP64 = 0, MCF_PREC_MIXED = 1 };
+enum { MCF_COOL_KI = 1 };
+
+/* INPUT3D.DAT (M3:134-151), the WNM state hard-wired by INITIA
+ * (M3:155-157) and the options of this implementation. */
+typedef struct mcf_config {
+  int np1x, np2x, np1y, np2y, np1z, np2z;
+  double ql1x, ql2x, ql1y, ql2y, ql1z, ql2z;
+  double vinitx1, vinitx2, vinity1, vinity2, vinitz1, vinitz2;
+  double binitx1, binitx2, binity1, binity2, binitz1, binitz2;
+  double cfl, facdep;
+  int maxstp, nitera;
+  double tfinal;
+  int bcx1, bcx2, bcy1, bcy2, bcz1, bcz2;   /* 3 periodic, 4 inflow (x only) */
+  int ifchem, ifthrm, ifrad, ifgrv;          /* ifthrm = 2: cooling+conduction */
+  double pinit_over_kb;                      /* 8.810807d3 */
+  double Hini, pini, H2ini, Heini, Hepini, Cini, COini, Cpini;
+  /* MPI block split NSPLTx/y/z of the reference run to reproduce (M3:64-72):
+   * the reference's MOC limiter switch reads rank-local ghost cells, so its
+   * results depend on the block edges.  1 = one block on that axis. */
+  int nsplitx, nsplity, nsplitz;
+  int precision;                             /* MCF_PREC_*  */
+  int cooling;                               /* MCF_COOL_KI */
+} mcf_config;
+
+typedef struct mcf_ctx mcf_ctx;
+
+/* Fill *cfg with the defaults of 3DMHD-CfCA/INPUT3D.DAT and M3:155-157. */
+void mcf_default_config(mcf_config *cfg);
+
+/* INITIA (M3:114-486) minus the file I/O: validates cfg, builds the grid,
+ * floors and inflow states (BBRV, BBRV_cm, DTF), allocates device memory. */
+int mcf_init(const mcf_config *cfg, mcf_ctx **ctx);
+/* Frees the context (DEALLOCATE at M3:101-104). */
+int mcf_finalize(mcf_ctx *ctx);
+
+/* Restart read (M3:454-465) followed by CC(4) (M3:470): copies the 8 U
+ * components (U(:,:,:,1:8), contiguous) and the 8 species arrays
+ * (ndH, ndp, ndH2, ndHe, ndHep, ndC, ndCO, ndCp; contiguous, restart order)
+ * to the device and rebuilds the cell-centred field Bcc. */
+int mcf_upload(mcf_ctx *ctx, const double *U8, const double *species8);
+/* SAVEU (M3:684-757) without the file: copies the state back. */
+int mcf_download(mcf_ctx *ctx, double *U8, double *species8);
+/* Cell-centred field Bcc(:,:,:,1:3) (contiguous). */
+int mcf_download_bcc(mcf_ctx *ctx, double *bcc3);
+
+/* Simulation clock and sweep-order counter (time, ifEVO of EVOLVE). */
+int mcf_set_time(mcf_ctx *ctx, double time, int ifevo);
+int mcf_get_time(mcf_ctx *ctx, double *time, int *ifevo);
+
+/* Couran (M3:1912-1930) + Stblty (M3:2784-2822) + the dt reduction and
+ * clips of EVOLVE (M3:567-604): dt = min(tfinal, CFL*tCFL, tLMT), clipped
+ * to tfinal and tsave.  *tag = 1 (CFL bound, 'FastSpd') or 2 ('Conduct'). */
+int mcf_timestep(mcf_ctx *ctx, double tsave, double *dt, int *tag);
+/* SOURCE(dt) (M3:2448-2572): two-stage KI cooling + saturated conduction. */
+int mcf_source(mcf_ctx *ctx, double dt);
+/* MHD(x|y|z) (M3:1264-1324): RHS + MOC + REMAP sweep, dir 0/1/2. */
+int mcf_sweep(mcf_ctx *ctx, int dir, double dt);
+/* CC(1), CC(2)+CCT x3, CC(3), CC(4) (M3:638-649). */
+int mcf_ct(mcf_ctx *ctx, double dt);
+/* DISSIP (M3:3143-3173). */
+int mcf_dissip(mcf_ctx *ctx);
+/* One EVOLVE inner-loop body (M3:607-656): SOURCE(dt/2), three sweeps in
+ * the ifEVO order, CT, SOURCE(dt/2), DISSIP, time += dt. */
+int mcf_step(mcf_ctx *ctx, double dt);
+/* mcf_timestep + mcf_step repeated nsteps times (the EVOLVE inner loop,
+ * M3:562-657, with the tsave/tfinal stops).  dts may be NULL. */
+int mcf_run(mcf_ctx *ctx, int nsteps, double tsave, double *dts, int *ndone);
+
+/* Per-kernel timing: while enabled, every kernel launch of the context is
+ * bracketed by HIP events on the context's stream.  mcf_profile_report
+ * waits for the device, then returns one entry per kernel name with the
+ * number of launches, the summed event time and the summed algorithmic
+ * bytes (each array the kernel reads or writes counted once per element of
+ * its launch).  No reference counterpart (the reference only keeps the
+ * wall clock of M3:556,663-666). */
+typedef struct mcf_kstat {
+  char name[48];
+  int launches;
+  double ms;
+  double bytes;
+} mcf_kstat;
+int mcf_profile(mcf_ctx *ctx, int enable);
+int mcf_profile_report(mcf_ctx *ctx, mcf_kstat *out, int maxn, int *n);
+
+/* Waits for all work of the context. */
+int mcf_sync(mcf_ctx *ctx);
+/* HIP stream the context launches on (hipStream_t), for event timing. */
+void *mcf_stream(mcf_ctx *ctx);
+const char *mcf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
