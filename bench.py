@@ -108,6 +108,10 @@ def main():
            "step_roofline": {"alg_bytes_per_zone": ALG_BYTES_PER_ZONE, "achieved": round(step_gbs, 1),
                              "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
            "dt_mean": float(sum(dts) / len(dts))}
+    nw = 1 << 28
+    cms = g.copy_probe(nw, 10)
+    out["copy_probe"] = {"bytes": 16 * nw, "ms": cms, "GBs": round(16 * nw / (cms * 1e-3) / 1e9, 1)}
+    log(f"[bench] copy kernel: {16 * nw / 2**30:.1f} GiB moved in {cms:.3f} ms = {out['copy_probe']['GBs']} GB/s")
     g.close()
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle {args.cpu_steps} steps at {args.cpu_n}^3")

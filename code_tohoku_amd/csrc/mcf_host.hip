@@ -634,6 +634,29 @@ extern "C" int mcf_sync(mcf_ctx *x) {
 
 extern "C" void *mcf_stream(mcf_ctx *x) { return x ? (void *)x->stream : nullptr; }
 
+extern "C" int mcf_copy_probe(mcf_ctx *x, long nwords, int reps, double *ms) {
+  if (!x || nwords <= 0 || reps <= 0 || !ms) { g_err = "bad argument"; return MCF_E_ARG; }
+  double *buf = nullptr;
+  HIPCHK(hipMalloc(&buf, 2 * (size_t)nwords * sizeof(double)));
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipMemsetAsync(buf, 0, 2 * (size_t)nwords * sizeof(double), x->stream));
+  unsigned nb = 256u * 8u * 32u; // 32 blocks of 256 per CU
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(TPB), 0, x->stream, buf, buf + nwords, nwords); // warm
+  HIPCHK(hipEventRecord(a, x->stream));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(TPB), 0, x->stream, buf, buf + nwords, nwords);
+  HIPCHK(hipEventRecord(b, x->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  *ms = (double)t / reps;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  HIPCHK(hipFree(buf));
+  return MCF_OK;
+}
+
 extern "C" int mcf_profile(mcf_ctx *x, int enable) {
   if (!x) { g_err = "null argument"; return MCF_E_ARG; }
   x->prof = enable != 0;

@@ -109,5 +109,6 @@ __global__ void k_dtmin(DtArgs A, Geo g, Box b);
 __global__ void k_dtinit(double *out, double v);
 __global__ void k_dissip_div(DisArgs A, Geo g, int e0);
 __global__ void k_dissip_upd(DisArgs A, Geo g, int e0);
+__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n);
 
 } // namespace mcf

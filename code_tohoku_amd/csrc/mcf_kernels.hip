@@ -706,4 +706,12 @@ __global__ void k_dissip_upd(DisArgs A, Geo g, int e0) {
   A.U[3][c] = A.U[3][c] + 0.1 * (D(i, j, k + 1) - D(i, j, k - 1));
 }
 
+// Plain streaming copy (8-byte words, grid-stride): the bandwidth and
+// hardware-counter reference for a kernel that moves a known byte count.
+__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
 } // namespace mcf

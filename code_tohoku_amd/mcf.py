@@ -77,11 +77,12 @@ def lib():
         L.mcf_stream.restype = P
         L.mcf_last_error.restype = C.c_char_p
         L.mcf_profile.argtypes = [P, I]
+        L.mcf_copy_probe.argtypes = [P, C.c_long, I, pd]
         L.mcf_profile_report.argtypes = [P, C.POINTER(KStat), I, pi]
         for nm in ("mcf_init", "mcf_finalize", "mcf_upload", "mcf_download", "mcf_download_bcc",
                    "mcf_set_time", "mcf_get_time", "mcf_timestep", "mcf_source", "mcf_sweep",
                    "mcf_ct", "mcf_dissip", "mcf_step", "mcf_run", "mcf_sync", "mcf_profile",
-                   "mcf_profile_report"):
+                   "mcf_profile_report", "mcf_copy_probe"):
             getattr(L, nm).restype = C.c_int
         _lib = L
     return _lib
@@ -192,6 +193,12 @@ class Domain:
     @property
     def stream(self) -> int:
         return lib().mcf_stream(self.h) or 0
+
+    def copy_probe(self, nwords: int = 1 << 28, reps: int = 10) -> float:
+        """Mean ms of one plain device-to-device copy of nwords doubles."""
+        ms = C.c_double()
+        _chk(lib().mcf_copy_probe(self.h, nwords, reps, C.byref(ms)))
+        return ms.value
 
     def profile(self, enable: bool = True):
         """Bracket every kernel launch with HIP events (see mcf_profile)."""

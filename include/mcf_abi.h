@@ -115,6 +115,12 @@ typedef struct mcf_kstat {
 int mcf_profile(mcf_ctx *ctx, int enable);
 int mcf_profile_report(mcf_ctx *ctx, mcf_kstat *out, int maxn, int *n);
 
+/* Streams nwords doubles from one device buffer to another reps times with
+ * a plain copy kernel (k_copy) and returns the mean time per copy: the
+ * achievable-bandwidth and hardware-counter reference for the roofline.
+ * No reference counterpart. */
+int mcf_copy_probe(mcf_ctx *ctx, long nwords, int reps, double *ms);
+
 /* Waits for all work of the context. */
 int mcf_sync(mcf_ctx *ctx);
 /* HIP stream the context launches on (hipStream_t), for event timing. */
