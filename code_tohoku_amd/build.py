@@ -18,6 +18,9 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libmcf.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("MCF_FC", "/opt/rocm/bin/amdflang")
+FDRIVER = os.path.join(HERE, "fdriver")
+EVOLVE = os.path.join(LIBDIR, "mcf_evolve")
 ARCH = os.environ.get("MCF_ARCH", "gfx950")
 
 SOURCES = ["mcf_kernels.hip", "mcf_host.hip"]
@@ -47,7 +50,21 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING module and the EVOLVE driver, linked to libmcf.so."""
+    mod = os.path.join(FDRIVER, "mcf_abi_mod.f90")
+    drv = os.path.join(FDRIVER, "mcf_evolve.f90")
+    modo = os.path.join(OBJDIR, "mcf_abi_mod.o")
+    _run([FC, "-O2", "-module-dir", OBJDIR, "-c", mod, "-o", modo], verbose)
+    tmp = EVOLVE + ".tmp"
+    _run([FC, "-O2", "-module-dir", OBJDIR, "-I", OBJDIR, drv, modo, "-o", tmp, "-L", LIBDIR, "-lmcf",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"], verbose)
+    os.replace(tmp, EVOLVE)
+    return EVOLVE
 
 
 if __name__ == "__main__":
