@@ -41,9 +41,10 @@ struct mcf_ctx {
   size_t ncell = 0;
   double *pool = nullptr;
   double *U[8]{}, *sp[8]{}, *bcc[3]{};
+  // second buffers of the fields the sweep stages write (swapped per stage)
+  double *Ub[5]{}, *spb[8]{};
   // sweep scratch
-  double *u5e = nullptr, *blg[2]{}, *xlag = nullptr, *dxlagM = nullptr, *pa = nullptr, *va = nullptr;
-  double *mf[8]{}, *fl[15]{};
+  double *u5e = nullptr, *blg[2]{}, *blgb[2]{}, *xlag = nullptr, *dxlagM = nullptr, *va = nullptr;
   // CT / source scratch (aliases the sweep scratch: the stages never overlap)
   double *vfc[3]{}, *dq[4]{}, *emf[3]{}, *Tn = nullptr, *Pn = nullptr;
   double *divv = nullptr, *red = nullptr;
@@ -287,14 +288,14 @@ static int fillOT(mcf_ctx *x, int dmask, int nol, int what) {
 // ----------------------------------------------------------------------------
 // stages
 // ----------------------------------------------------------------------------
-static SweepArgs sweep_args(mcf_ctx *x, int d) {
-  SweepArgs A{};
-  for (int f = 0; f < 8; ++f) { A.U[f] = x->U[f]; A.sp[f] = x->sp[f]; A.spmin[f] = x->spmin[f]; }
+static SweepIO sweep_io(mcf_ctx *x, int d) {
+  SweepIO A{};
+  for (int f = 0; f < 8; ++f) { A.U[f] = x->U[f]; A.sp[f] = x->sp[f]; A.spo[f] = x->spb[f]; A.spmin[f] = x->spmin[f]; }
+  for (int f = 0; f < 5; ++f) A.Uo[f] = x->Ub[f];
   for (int c = 0; c < 3; ++c) A.bcc[c] = x->bcc[c];
-  A.u5e = x->u5e; A.blg[0] = x->blg[0]; A.blg[1] = x->blg[1];
-  A.xlag = x->xlag; A.dxlagM = x->dxlagM; A.pa = x->pa; A.va = x->va;
-  for (int t = 0; t < 8; ++t) A.mf[t] = x->mf[t];
-  for (int t = 0; t < 15; ++t) A.fl[t] = x->fl[t];
+  A.blg[0] = x->blg[0]; A.blg[1] = x->blg[1]; A.blgo[0] = x->blgb[0]; A.blgo[1] = x->blgb[1];
+  A.u5e_in = x->u5e; A.u5e = x->u5e;
+  A.xlag = x->xlag; A.dxlagM = x->dxlagM; A.va = x->va;
   A.xe = x->gx[d]; A.dxe = x->gdx[d];
   A.d = d;
   static const int VN[3] = {1, 2, 3}, VT1[3] = {2, 3, 1}, VT2[3] = {3, 1, 2}, BN[3] = {5, 6, 7};
@@ -305,31 +306,63 @@ static SweepArgs sweep_args(mcf_ctx *x, int d) {
   return A;
 }
 
+static void swapU(mcf_ctx *x, int f) { double *t = x->U[f]; x->U[f] = x->Ub[f]; x->Ub[f] = t; }
+static void swapSp(mcf_ctx *x) { for (int f = 0; f < 8; ++f) { double *t = x->sp[f]; x->sp[f] = x->spb[f]; x->spb[f] = t; } }
+static void swapBlg(mcf_ctx *x) { for (int c = 0; c < 2; ++c) { double *t = x->blg[c]; x->blg[c] = x->blgb[c]; x->blgb[c] = t; } }
+
+static dim3 sweep_grid(const mcf_ctx *x, int d) {
+  int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
+  unsigned gx = (unsigned)((nx + 63) / 64);
+  if (d == 0) return dim3(gx, (unsigned)((ny + 7) / 8), (unsigned)nz);
+  if (d == 1) return dim3(gx, (unsigned)((ny + 7) / 8), (unsigned)nz);
+  return dim3(gx, (unsigned)((nz + 7) / 8), (unsigned)ny);
+}
+
+#define SWEEP_LAUNCH(kern, d, narr, ...)                                                             \
+  do {                                                                                             \
+    dim3 gr_ = sweep_grid(x, d);                                                                   \
+    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                       \
+    if (x->prof) { ea_ = prof_event(x); eb_ = prof_event(x); HIPCHK(hipEventRecord(ea_, x->stream)); } \
+    if (d == 0) hipLaunchKernelGGL(kern<0>, gr_, dim3(576), 0, x->stream, __VA_ARGS__);           \
+    else if (d == 1) hipLaunchKernelGGL(kern<1>, gr_, dim3(576), 0, x->stream, __VA_ARGS__);      \
+    else hipLaunchKernelGGL(kern<2>, gr_, dim3(576), 0, x->stream, __VA_ARGS__);                  \
+    HIPCHK(hipGetLastError());                                                                     \
+    if (x->prof) {                                                                                 \
+      HIPCHK(hipEventRecord(eb_, x->stream));                                                      \
+      x->recs.push_back({#kern, ea_, eb_, 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * (narr)}); \
+    }                                                                                              \
+  } while (0)
+
 // MHD (M3:1264-1324): RHS -> MOC -> REMAP -> primitives
 static int do_sweep(mcf_ctx *x, int d, double dt) {
-  SweepArgs A = sweep_args(x, d);
-  int n = x->g.n[d];
-  // RHS (M3:1346-1357 fills)
+  SweepIO A = sweep_io(x, d);
+  // RHS (fills M3:1346-1357): reads rho, v(3), p, Bcc(3), species; writes
+  // rho, v_n, e, Blg(2), species, interface velocity / position, widths
   int lr[3] = {1, A.VN + 1, 5};
   CHK(fillU(x, 1 << d, 2, 1, lr, 3));
   CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
-  Box bf = sweepbox(x, d, 0, n), bc = sweepbox(x, d, 1, n);
-  KLAUNCH(k_rhs_face, boxsize(bf), 13, A, x->g, bf, dt);
-  KLAUNCH(k_rhs_cell, boxsize(bc), 33, A, x->g, bc, dt);
-  // MOC (M3:1950-1976 fills)
+  A = sweep_io(x, d);
+  SWEEP_LAUNCH(k_rhs, d, 33, A, x->g, dt);
+  swapU(x, 0); swapU(x, A.VN); swapSp(x); swapBlg(x);
+  // MOC (fills M3:1950-1976): reads rho, v_t(2), Blg(2), p, e, widths, Bn;
+  // writes momenta(3), E, Blg(2)
   int lm[4] = {1, A.VT1 + 1, A.VT2 + 1, A.BN + 1};
   CHK(fillU(x, 1 << d, 2, 1, lm, 4));
   CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
   CHK(fillOT(x, 1 << d, 2, OT_BLG));
-  KLAUNCH(k_moc_face, boxsize(bf), 18, A, x->g, bf, dt);
-  KLAUNCH(k_moc_cell, boxsize(bc), 26, A, x->g, bc, dt);
-  // REMAP (M3:1571-1594 fills)
+  A = sweep_io(x, d);
+  SWEEP_LAUNCH(k_moc, d, 20, A, x->g, dt);
+  swapU(x, A.VN); swapU(x, A.VT1); swapU(x, A.VT2); swapU(x, 4); swapBlg(x);
+  // REMAP (fills M3:1571-1594): reads rho, m(3), E, Blg(2), species,
+  // widths, interface velocity, Bcc_n; writes rho, v(3), p, species
   CHK(fillOT(x, 1 << d, 2, OT_BLG));
   int l5[5] = {1, 2, 3, 4, 5};
   CHK(fillU(x, 1 << d, 2, 2, l5, 5));
   CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
-  KLAUNCH(k_remap_face, boxsize(bf), 32, A, x->g, bf, dt);
-  KLAUNCH(k_remap_cell, boxsize(bc), 45, A, x->g, bc, dt);
+  A = sweep_io(x, d);
+  SWEEP_LAUNCH(k_remap, d, 31, A, x->g, dt);
+  for (int f = 0; f < 5; ++f) swapU(x, f);
+  swapSp(x);
   return MCF_OK;
 }
 
@@ -450,31 +483,27 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   hipError_t e = hipGetDevice(&x->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { g_err = std::string("HIP init: ") + hipGetErrorString(e); delete x; return MCF_E_HIP; }
-  const int nstate = 19, nscratch = 30;
+  const int nstate = 19, nbuf = 13, nsweep = 8, nscratch = 12;
   size_t nc = x->ncell;
   size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
   size_t ngrid = 0;
   for (int d = 0; d < 3; ++d) ngrid += 2 * (size_t)(n[d] + 4);
-  size_t total = (nstate + nscratch) * nc + ndivv + ngrid + 16;
+  size_t total = (nstate + nbuf + nsweep + nscratch) * nc + ndivv + ngrid + 16;
   e = hipMalloc(&x->pool, total * sizeof(double));
   if (e != hipSuccess) { g_err = std::string("hipMalloc: ") + hipGetErrorString(e); (void)hipStreamDestroy(x->stream); delete x; return MCF_E_NOMEM; }
   double *p = x->pool;
   for (int f = 0; f < 8; ++f) { x->U[f] = p; p += nc; }
   for (int f = 0; f < 8; ++f) { x->sp[f] = p; p += nc; }
   for (int f = 0; f < 3; ++f) { x->bcc[f] = p; p += nc; }
-  double *scr = p;
+  for (int f = 0; f < 5; ++f) { x->Ub[f] = p; p += nc; }
+  for (int f = 0; f < 8; ++f) { x->spb[f] = p; p += nc; }
   x->u5e = p; p += nc;
-  x->blg[0] = p; p += nc; x->blg[1] = p; p += nc;
-  x->xlag = p; p += nc; x->dxlagM = p; p += nc; x->pa = p; p += nc; x->va = p; p += nc;
-  for (int t = 0; t < 8; ++t) { x->mf[t] = p; p += nc; }
-  for (int t = 0; t < 15; ++t) { x->fl[t] = p; p += nc; }
-  // CT and SOURCE scratch alias the sweep scratch, but CT also reads
-  // nothing of it and the x-rule fills read blg: keep blg out of the alias.
-  double *q = scr + 3 * nc; // after u5e, blg[0], blg[1]
-  for (int c = 0; c < 3; ++c) { x->vfc[c] = q; q += nc; }
-  for (int t = 0; t < 4; ++t) { x->dq[t] = q; q += nc; }
-  for (int c = 0; c < 3; ++c) { x->emf[c] = q; q += nc; }
-  x->Tn = q; q += nc; x->Pn = q; q += nc;
+  x->blg[0] = p; p += nc; x->blg[1] = p; p += nc; x->blgb[0] = p; p += nc; x->blgb[1] = p; p += nc;
+  x->xlag = p; p += nc; x->dxlagM = p; p += nc; x->va = p; p += nc;
+  for (int c = 0; c < 3; ++c) { x->vfc[c] = p; p += nc; }
+  for (int t = 0; t < 4; ++t) { x->dq[t] = p; p += nc; }
+  for (int c = 0; c < 3; ++c) { x->emf[c] = p; p += nc; }
+  x->Tn = p; p += nc; x->Pn = p; p += nc;
   x->divv = p; p += ndivv;
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
   x->red = p; p += 16;

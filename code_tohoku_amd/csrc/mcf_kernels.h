@@ -36,26 +36,29 @@ struct XFill {
   double val2[2];
 };
 
-// One MHD sweep along axis d (M3:1264-1324).  Field indices into U:
-// 0 rho, 1..3 v (or momenta), 4 p (or E), 5..7 face B.
-struct SweepArgs {
-  double *U[8];
-  double *sp[8];
+// Fused sweep stages (mcf_sweep.hip): inputs are the current buffers,
+// outputs the other buffer of each field the stage writes.
+struct SweepIO {
+  const double *U[8];
+  const double *sp[8];
   const double *bcc[3];
-  double *u5e;       // specific total energy of interior cells (RHS -> MOC)
-  double *blg[2];    // transverse B on the Lagrangian grid
-  double *xlag;      // Lagrangian interface positions
-  double *dxlagM;    // Lagrangian cell widths
-  double *pa, *va;   // interface pressure / velocity (Va = Vfc of the sweep)
-  double *mf[8];     // MOC interface values, left view then right view
-  double *fl[15];    // REMAP fluxes
-  const double *xe;  // x(i), index -1..N+2 at [i+1]
-  const double *dxe; // dx(i)
+  double *Uo[8];
+  double *spo[8];
+  const double *blg[2];
+  double *blgo[2];
+  const double *u5e_in; // specific total energy from RHS (read by MOC)
+  double *u5e;          // written by RHS
+  double *xlag, *va, *dxlagM;
+  const double *xe, *dxe;
   int d, VN, VT1, VT2, BN, BT1, BT2;
-  int nblk;          // cells per MPI block of the reproduced run along d
+  int nblk;
   double facdep, rmin, pmin, pmax;
   double spmin[8];
 };
+
+template <int D> __global__ void k_rhs(SweepIO A, Geo g, double dt);
+template <int D> __global__ void k_moc(SweepIO A, Geo g, double dt);
+template <int D> __global__ void k_remap(SweepIO A, Geo g, double dt);
 
 struct CtArgs {
   double *U[8];
@@ -92,12 +95,6 @@ struct DisArgs {
 
 __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
-__global__ void k_rhs_face(SweepArgs A, Geo g, Box b, double dt);
-__global__ void k_rhs_cell(SweepArgs A, Geo g, Box b, double dt);
-__global__ void k_moc_face(SweepArgs A, Geo g, Box b, double dt);
-__global__ void k_moc_cell(SweepArgs A, Geo g, Box b, double dt);
-__global__ void k_remap_face(SweepArgs A, Geo g, Box b, double dt);
-__global__ void k_remap_cell(SweepArgs A, Geo g, Box b, double dt);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
 __global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode);
 __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt);

@@ -107,295 +107,6 @@ __global__ void k_fill_xrule(XFill X, Geo g, int nol) {
 }
 
 // ---------------------------------------------------------------------------
-// RHS (M3:1330-1467)
-// ---------------------------------------------------------------------------
-// One thread per interface i = 0..N (between cells i and i+1 along d).
-__global__ void k_rhs_face(SweepArgs A, Geo g, Box b, double dt) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  const int d = A.d;
-  const int s = g.st[d];
-  const int q = d == 0 ? i : d == 1 ? j : k;
-  const int c = ix3(g, i, j, k), cp = c + s;
-  const double *xe = A.dxe; // dxelr, index -1..N+2 at [q+1]
-  const double *U1 = A.U[0], *UVN = A.U[A.VN], *U5 = A.U[4];
-  const double *B1 = A.bcc[A.BT1], *B2 = A.bcc[A.BT2];
-  double dxm = xe[q], dx0 = xe[q + 1], dxp = xe[q + 2], dxpp = xe[q + 3];
-  // slopes at q and q+1 (VLIMIT k = 1, VN, 5, BT1+8, BT2+8)
-  double T0 = tswitch(U5[c], U1[c]), T1 = tswitch(U5[cp], U1[cp]);
-  double g1a = vlim_pick(vlim_parts(U1[c - s], U1[c], U1[cp], dxm, dx0, dxp), T0);
-  double g1b = vlim_pick(vlim_parts(U1[c], U1[cp], U1[cp + s], dx0, dxp, dxpp), T1);
-  double gva = vlim_pick(vlim_parts(UVN[c - s], UVN[c], UVN[cp], dxm, dx0, dxp), T0);
-  double gvb = vlim_pick(vlim_parts(UVN[c], UVN[cp], UVN[cp + s], dx0, dxp, dxpp), T1);
-  double g5a = vlim_pick(vlim_parts(U5[c - s], U5[c], U5[cp], dxm, dx0, dxp), T0);
-  double g5b = vlim_pick(vlim_parts(U5[c], U5[cp], U5[cp + s], dx0, dxp, dxpp), T1);
-  double gAa = vlim_pick(vlim_parts(B1[c - s], B1[c], B1[cp], dxm, dx0, dxp), T0);
-  double gAb = vlim_pick(vlim_parts(B1[c], B1[cp], B1[cp + s], dx0, dxp, dxpp), T1);
-  double gBa = vlim_pick(vlim_parts(B2[c - s], B2[c], B2[cp], dxm, dx0, dxp), T0);
-  double gBb = vlim_pick(vlim_parts(B2[c], B2[cp], B2[cp + s], dx0, dxp, dxpp), T1);
-  const double *const *sp = A.sp;
-  double gamma = gam_of(sp, c);
-  double cm = sqrt((gamma * U5[c] + B1[c] * B1[c] + B2[c] * B2[c]) / U1[c]);
-  double depend1 = 0.5 * (dx0 - cm * dt) * A.facdep;
-  depend1 = dmax(0.0, depend1);
-  gamma = gam_of(sp, cp);
-  cm = sqrt((gamma * U5[cp] + B1[cp] * B1[cp] + B2[cp] * B2[cp]) / U1[cp]);
-  double depend2 = 0.5 * (dxp - cm * dt) * A.facdep;
-  depend2 = dmax(0.0, depend2);
-  double QL1 = posfix(U1[c] + depend1 * g1a, U1[c]);
-  double QR1 = posfix(U1[cp] - depend2 * g1b, U1[cp]);
-  double QL2 = UVN[c] + depend1 * gva;
-  double QR2 = UVN[cp] - depend2 * gvb;
-  double QL3 = posfix(U5[c] + depend1 * g5a, U5[c]);
-  double QR3 = posfix(U5[cp] - depend2 * g5b, U5[cp]);
-  double a1 = B1[c] + depend1 * gAa, a2 = B2[c] + depend1 * gBa;
-  double QL4 = a1 * a1 + a2 * a2;
-  double b1 = B1[cp] - depend2 * gAb, b2 = B2[cp] - depend2 * gBb;
-  double QR4 = b1 * b1 + b2 * b2;
-  double ndHm = 0.5 * (sp[0][c] + sp[0][cp]);
-  double ndpm = 0.5 * (sp[1][c] + sp[1][cp]);
-  double ndHem = 0.5 * (sp[3][c] + sp[3][cp]);
-  double ndHepm = 0.5 * (sp[4][c] + sp[4][cp]);
-  double ndH2m = 0.5 * (sp[2][c] + sp[2][cp]);
-  double gm = (5.0 * (ndHm + ndpm + ndHem + ndHepm) + 7.0 * ndH2m) / (3.0 * (ndHm + ndpm + ndHem + ndHepm) + 5.0 * ndH2m);
-  double Pa, Va;
-  rieman(QL1, QL2, QL3, QL4, QR1, QR2, QR3, QR4, gm, Pa, Va);
-  A.pa[c] = Pa;
-  A.va[c] = Va;
-  A.xlag[c] = A.xe[q + 1] + Va * dt;
-}
-
-// One thread per cell i = 1..N: Lagrangian conservation update.
-__global__ void k_rhs_cell(SweepArgs A, Geo g, Box b, double dt) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  const int d = A.d;
-  const int s = g.st[d];
-  const int q = d == 0 ? i : d == 1 ? j : k;
-  const int c = ix3(g, i, j, k), cm = c - s;
-  double *U1 = A.U[0];
-  double rho = U1[c];
-  double gi = gammi1_of(A.sp, c);
-  double vx = A.U[1][c], vy = A.U[2][c], vz = A.U[3][c];
-  double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
-  double e = A.U[4][c] / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
-  double dxe = A.dxe[q + 1];
-  double dm = dxe * rho;
-  double xl = A.xlag[c], xlm = A.xlag[cm];
-  double pa = A.pa[c], pam = A.pa[cm], va = A.va[c], vam = A.va[cm];
-  U1[c] = dm / (xl - xlm);
-  A.U[A.VN][c] = A.U[A.VN][c] - dt / dm * (pa - pam);
-  A.u5e[c] = e - dt / dm * (pa * va - pam * vam);
-  A.blg[0][c] = A.bcc[A.BT1][c] * dxe / (xl - xlm);
-  A.blg[1][c] = A.bcc[A.BT2][c] * dxe / (xl - xlm);
-  for (int f = 0; f < 8; ++f) A.sp[f][c] = dxe * A.sp[f][c] / (xl - xlm);
-  A.dxlagM[c] = xl - xlm;
-}
-
-// ---------------------------------------------------------------------------
-// MOC (M3:1939-2098)
-// ---------------------------------------------------------------------------
-struct MocOut {
-  double Bay, Vay, Baz, Vaz;
-};
-
-__device__ __forceinline__ MocOut moc_iface(const SweepArgs &A, int c, int cp, int q, const double *dxl,
-                                            const double *gi, const double *gp, double dt) {
-  const double *U1 = A.U[0], *UT1 = A.U[A.VT1], *UT2 = A.U[A.VT2], *UBn = A.U[A.BN];
-  const double *Bl1 = A.blg[0], *Bl2 = A.blg[1];
-  const double fac = A.facdep;
-  const double *xe = A.xe; // x(q) at [q+1]
-  int cpp = cp + (cp - c);
-  double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
-  double deni = sqrt((dxl[1] + dxl[2]) / (dxl[1] * U1[cp] + dxl[2] * U1[c]));
-  double depend1, depend2, ca;
-  double QL11, QL12, QL13, QR11, QR12, QR13, QL21, QL22, QR21, QR22;
-  if (Bnap > 0.0) {
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[1], ca);
-    depend1 = (0.5 * dxl[1] - ca) * fac;
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[2], ca);
-    depend2 = (0.5 * dxl[2] - ca) * fac;
-    QL11 = UT1[c] + depend1 * gi[1];
-    QR11 = UT1[cp] - depend2 * gp[1];
-    QL12 = Bl1[c] + depend1 * gi[3];
-    QR12 = Bl1[cp] - depend2 * gp[3];
-    QL13 = posfix(U1[c] + depend1 * gi[0], U1[c]);
-    QR13 = posfix(U1[cp] - depend2 * gp[0], U1[cp]);
-    QL21 = UT2[c] + depend1 * gi[2];
-    QR21 = UT2[cp] - depend2 * gp[2];
-    QL22 = Bl2[c] + depend1 * gi[4];
-    QR22 = Bl2[cp] - depend2 * gp[4];
-  } else {
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[2], ca);
-    depend1 = (0.5 * dxl[2] - ca) * fac;
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[1], ca);
-    depend2 = (0.5 * dxl[1] - ca) * fac;
-    QL11 = UT1[cp] - depend1 * gp[1];
-    QR11 = UT1[c] + depend2 * gi[1];
-    QL12 = Bl1[cp] - depend1 * gp[3];
-    QR12 = Bl1[c] + depend2 * gi[3];
-    QL13 = posfix(U1[cp] - depend1 * gp[0], U1[cp]);
-    QR13 = posfix(U1[c] + depend2 * gi[0], U1[c]);
-    QL21 = UT2[cp] - depend1 * gp[2];
-    QR21 = UT2[c] + depend2 * gi[2];
-    QL22 = Bl2[cp] - depend1 * gp[4];
-    QR22 = Bl2[c] + depend2 * gi[4];
-  }
-  MocOut o;
-  chareq(QL11, QL12, QL13, QR11, QR12, QR13, o.Bay, o.Vay);
-  chareq(QL21, QL22, QL13, QR21, QR22, QR13, o.Baz, o.Vaz);
-  return o;
-}
-
-// One thread per interface q = 0..N.  At an MPI block edge of the reference
-// run the interface is evaluated twice: once as the left block sees it (the
-// right cell is a ghost whose U(:,:,:,5) still holds the pre-sweep pressure)
-// and once as the right block sees it.
-__global__ void k_moc_face(SweepArgs A, Geo g, Box b, double dt) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  const int d = A.d;
-  const int s = g.st[d];
-  const int q = d == 0 ? i : d == 1 ? j : k;
-  const int n = g.n[d];
-  const int c = ix3(g, i, j, k), cp = c + s;
-  double dxl[4] = {A.dxlagM[c - s], A.dxlagM[c], A.dxlagM[cp], A.dxlagM[cp + s]};
-  const double *U1 = A.U[0];
-  const double *Q[5] = {A.U[0], A.U[A.VT1], A.U[A.VT2], A.blg[0], A.blg[1]};
-  Slope2 sa[5], sb[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    sa[v] = vlim_parts(Q[v][c - s], Q[v][c], Q[v][cp], dxl[0], dxl[1], dxl[2]);
-    sb[v] = vlim_parts(Q[v][c], Q[v][cp], Q[v][cp + s], dxl[1], dxl[2], dxl[3]);
-  }
-  // limiter switch: interior cells hold e in u5e, ghost copies hold pressure
-  double Tp0 = tswitch(A.U[4][c], U1[c]), Tp1 = tswitch(A.U[4][cp], U1[cp]);
-  double Te0 = (q >= 1) ? tswitch(A.u5e[c], U1[c]) : Tp0;
-  double Te1 = (q + 1 <= n) ? tswitch(A.u5e[cp], U1[cp]) : Tp1;
-  bool edge = (q % A.nblk) == 0;
-  double gi[5], gp[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    gi[v] = vlim_pick(sa[v], Te0);
-    gp[v] = vlim_pick(sb[v], edge ? Tp1 : Te1);
-  }
-  MocOut L = moc_iface(A, c, cp, q, dxl, gi, gp, dt);
-  MocOut R = L;
-  if (edge) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      gi[v] = vlim_pick(sa[v], Tp0);
-      gp[v] = vlim_pick(sb[v], Te1);
-    }
-    R = moc_iface(A, c, cp, q, dxl, gi, gp, dt);
-  }
-  A.mf[0][c] = L.Bay; A.mf[1][c] = L.Vay; A.mf[2][c] = L.Baz; A.mf[3][c] = L.Vaz;
-  A.mf[4][c] = R.Bay; A.mf[5][c] = R.Vay; A.mf[6][c] = R.Baz; A.mf[7][c] = R.Vaz;
-}
-
-__global__ void k_moc_cell(SweepArgs A, Geo g, Box b, double dt) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  const int d = A.d;
-  const int s = g.st[d];
-  const int q = d == 0 ? i : d == 1 ? j : k;
-  const int c = ix3(g, i, j, k), cm = c - s, cp = c + s, cpp = cp + s;
-  const double *UBn = A.U[A.BN];
-  const double *xe = A.xe;
-  double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
-  double Bnam = UBn[c] + (A.xlag[cm] - xe[q]) * (UBn[cp] - UBn[cm]) / (xe[q + 1] - xe[q - 1]);
-  double Vnap = A.va[c], Vnam = A.va[cm];
-  double dxl = A.dxlagM[c];
-  double rho = A.U[0][c];
-  double BayL = A.mf[0][c], VayL = A.mf[1][c], BazL = A.mf[2][c], VazL = A.mf[3][c];
-  double BayR = A.mf[4][cm], VayR = A.mf[5][cm], BazR = A.mf[6][cm], VazR = A.mf[7][cm];
-  double *UVN = A.U[A.VN], *UT1 = A.U[A.VT1], *UT2 = A.U[A.VT2];
-  UVN[c] = UVN[c] * rho + dt / dxl * 0.5 * (Bnap * Bnap - Bnam * Bnam);
-  UT1[c] = UT1[c] * rho + dt / dxl * (Bnap * BayL - Bnam * BayR);
-  UT2[c] = UT2[c] * rho + dt / dxl * (Bnap * BazL - Bnam * BazR);
-  A.U[4][c] = A.u5e[c] * rho + dt / dxl *
-                                   ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
-                                    (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
-  A.blg[0][c] = A.blg[0][c] + dt / dxl * (Bnap * VayL - Bnam * VayR);
-  A.blg[1][c] = A.blg[1][c] + dt / dxl * (Bnap * VazL - Bnam * VazR);
-}
-
-// ---------------------------------------------------------------------------
-// REMAP (M3:1556-1686) + the conversion back to primitives in MHD
-// (M3:1291-1321)
-// ---------------------------------------------------------------------------
-// One thread per interface q = 0..N: upwind linear fluxes of 15 quantities.
-__global__ void k_remap_face(SweepArgs A, Geo g, Box b, double dt) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  const int d = A.d;
-  const int s = g.st[d];
-  const int c = ix3(g, i, j, k);
-  double vadt = A.va[c] * dt;
-  int u;
-  double depend, sgn;
-  if (vadt <= 0.0) {
-    u = c + s;
-    depend = 0.5 * (A.dxlagM[u] + vadt) * A.facdep;
-    sgn = -1.0;
-  } else {
-    u = c;
-    depend = 0.5 * (A.dxlagM[u] - vadt) * A.facdep;
-    sgn = 1.0;
-  }
-  double dxm = A.dxlagM[u - s], dx0 = A.dxlagM[u], dxp = A.dxlagM[u + s];
-  double T = tswitch(A.U[4][u], A.U[0][u]);
-#pragma unroll
-  for (int v = 0; v < 15; ++v) {
-    const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-    double gr = vlim_pick(vlim_parts(Qv[u - s], Qv[u], Qv[u + s], dxm, dx0, dxp), T);
-    double face = sgn > 0.0 ? Qv[u] + depend * gr : Qv[u] - depend * gr;
-    A.fl[v][c] = -(vadt * face);
-  }
-}
-
-__global__ void k_remap_cell(SweepArgs A, Geo g, Box b, double dt) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  const int d = A.d;
-  const int s = g.st[d];
-  const int q = d == 0 ? i : d == 1 ? j : k;
-  const int c = ix3(g, i, j, k), cm = c - s;
-  double dxl = A.dxlagM[c];
-  double dxE = A.xe[q + 1] - A.xe[q];
-  double Qn[15];
-#pragma unroll
-  for (int v = 0; v < 15; ++v) {
-    const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-    Qn[v] = (Qv[c] * dxl + A.fl[v][c] - A.fl[v][cm]) / dxE;
-  }
-  // MHD: back to primitive variables with floors
-  double rho = dmax(1.27 * A.rmin, Qn[0]);
-  double invd = 1.0 / rho;
-  double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
-  double s4 = Qn[7] + Qn[8] + Qn[10] + Qn[11];
-  double gg = 3.0 * s4 + 5.0 * Qn[9];
-  double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
-  double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rho;
-  double bn = A.bcc[d][c];
-  double emag = 0.5 * (bn * bn + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
-  double p = (Qn[4] - ekin - emag) * gi;
-  p = dmax(A.pmin, p);
-  p = dmin(A.pmax, p);
-  A.U[0][c] = rho;
-  A.U[1][c] = vx;
-  A.U[2][c] = vy;
-  A.U[3][c] = vz;
-  A.U[4][c] = p;
-#pragma unroll
-  for (int f = 0; f < 8; ++f) A.sp[f][c] = dmax(A.spmin[f], Qn[7 + f]);
-}
-
-// ---------------------------------------------------------------------------
 // Constrained transport (M3:2115-2445)
 // ---------------------------------------------------------------------------
 // CC(1): face velocities (dx-weighted) on the interior faces
