@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
@@ -68,7 +69,8 @@ def main():
     log(f"[bench] building config-2 state {n}^3 ...")
     p = ics.converging_flow_input(n)
     st = ics.converging_flow_state(p)
-    g = mcf.Domain(p, nsplit=nsplit)
+    prec = mcf.PREC_MIXED if args.precision == "mixed" else mcf.PREC_FP64
+    g = mcf.Domain(p, nsplit=nsplit, precision=prec)
     g.upload(st)
     del st
     log(f"[bench] warmup {args.warmup} steps")

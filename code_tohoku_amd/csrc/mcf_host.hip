@@ -59,6 +59,7 @@ struct mcf_ctx {
   double *hred = nullptr; // pinned 2 doubles
   // per-kernel event timing (mcf_profile)
   bool prof = false;
+  bool mixed = false; // MCF_PREC_MIXED: FP32 reconstruction / Riemann / EMF
   struct Rec { const char *name; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
@@ -149,6 +150,8 @@ static void setup_constants(mcf_ctx *x) {
 // launch helpers
 // ----------------------------------------------------------------------------
 static const int TPB = 256;
+// blocks of the dt reduction's first pass (one partial min pair each)
+static const int DT_BLOCKS = 2048;
 
 static Box mkbox(int x0, int x1, int y0, int y1, int z0, int z1) {
   Box b;
@@ -167,21 +170,36 @@ static Box sweepbox(const mcf_ctx *x, int d, int a0, int a1) {
 static unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 static long boxsize(const Box &b) { return (long)b.cnt[0] * b.cnt[1] * b.cnt[2]; }
 
+// Launch on an explicit grid; bytes = algorithmic bytes the launch moves,
+// recorded under name for mcf_profile.
+#define KLAUNCH_N(name, kern, grid, block, bytes, ...)                                              \
+  do {                                                                                             \
+    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                       \
+    if (x->prof) { ea_ = prof_event(x); eb_ = prof_event(x); HIPCHK(hipEventRecord(ea_, x->stream)); } \
+    hipLaunchKernelGGL(kern, (grid), (block), 0, x->stream, __VA_ARGS__);                          \
+    HIPCHK(hipGetLastError());                                                                     \
+    if (x->prof) {                                                                                 \
+      HIPCHK(hipEventRecord(eb_, x->stream));                                                      \
+      x->recs.push_back({name, ea_, eb_, (double)(bytes)});                                        \
+    }                                                                                              \
+  } while (0)
+#define KLAUNCH_G(kern, grid, block, bytes, ...) KLAUNCH_N(#kern, kern, grid, block, bytes, __VA_ARGS__)
+
 // Launch with nthreads threads; narr = arrays the kernel reads or writes
 // per thread, for the algorithmic byte count of mcf_profile.
 #define KLAUNCH(kern, nthreads, narr, ...)                                                          \
   do {                                                                                             \
     long nt_ = (nthreads);                                                                         \
-    if (nt_ > 0) {                                                                                 \
-      hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                     \
-      if (x->prof) { ea_ = prof_event(x); eb_ = prof_event(x); HIPCHK(hipEventRecord(ea_, x->stream)); } \
-      hipLaunchKernelGGL(kern, dim3(nblocks(nt_)), dim3(TPB), 0, x->stream, __VA_ARGS__);          \
-      HIPCHK(hipGetLastError());                                                                   \
-      if (x->prof) {                                                                               \
-        HIPCHK(hipEventRecord(eb_, x->stream));                                                    \
-        x->recs.push_back({#kern, ea_, eb_, 8.0 * (double)nt_ * (double)(narr)});                  \
-      }                                                                                            \
-    }                                                                                              \
+    if (nt_ > 0)                                                                                   \
+      KLAUNCH_G(kern, dim3(nblocks(nt_)), dim3(TPB), 8.0 * (double)nt_ * (double)(narr), __VA_ARGS__); \
+  } while (0)
+
+// KLAUNCH of the instance kern<R>, recorded under kern's name
+#define KLAUNCH_R(kern, R, nthreads, narr, ...)                                                     \
+  do {                                                                                             \
+    long nt_ = (nthreads);                                                                         \
+    if (nt_ > 0)                                                                                   \
+      KLAUNCH_N(#kern, (kern<R>), dim3(nblocks(nt_)), dim3(TPB), 8.0 * (double)nt_ * (double)(narr), __VA_ARGS__); \
   } while (0)
 
 // ----------------------------------------------------------------------------
@@ -318,18 +336,22 @@ static dim3 sweep_grid(const mcf_ctx *x, int d) {
   return dim3(gx, (unsigned)((nz + 7) / 8), (unsigned)ny);
 }
 
-#define SWEEP_LAUNCH(kern, d, narr, ...)                                                             \
+// Stage kernel launch: strict FP64 (R = double) or mixed (R = float).
+// RHS and MOC use 9 waves on the x sweep and 10 on y/z (halo positions),
+// REMAP 9 waves everywhere.
+#define SWEEP_LAUNCH(kern, d, wide, narr, ...)                                                      \
   do {                                                                                             \
     dim3 gr_ = sweep_grid(x, d);                                                                   \
-    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                       \
-    if (x->prof) { ea_ = prof_event(x); eb_ = prof_event(x); HIPCHK(hipEventRecord(ea_, x->stream)); } \
-    if (d == 0) hipLaunchKernelGGL(kern<0>, gr_, dim3(576), 0, x->stream, __VA_ARGS__);           \
-    else if (d == 1) hipLaunchKernelGGL(kern<1>, gr_, dim3(576), 0, x->stream, __VA_ARGS__);      \
-    else hipLaunchKernelGGL(kern<2>, gr_, dim3(576), 0, x->stream, __VA_ARGS__);                  \
-    HIPCHK(hipGetLastError());                                                                     \
-    if (x->prof) {                                                                                 \
-      HIPCHK(hipEventRecord(eb_, x->stream));                                                      \
-      x->recs.push_back({#kern, ea_, eb_, 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * (narr)}); \
+    const unsigned bt_ = (d != 0 && (wide)) ? 640u : 576u;                                         \
+    const double by_ = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * (narr);                   \
+    if (x->mixed) {                                                                                \
+      if (d == 0) KLAUNCH_N(#kern, (kern<0, float>), gr_, dim3(bt_), by_, __VA_ARGS__);                   \
+      else if (d == 1) KLAUNCH_N(#kern, (kern<1, float>), gr_, dim3(bt_), by_, __VA_ARGS__);              \
+      else KLAUNCH_N(#kern, (kern<2, float>), gr_, dim3(bt_), by_, __VA_ARGS__);                          \
+    } else {                                                                                       \
+      if (d == 0) KLAUNCH_N(#kern, (kern<0, double>), gr_, dim3(bt_), by_, __VA_ARGS__);                  \
+      else if (d == 1) KLAUNCH_N(#kern, (kern<1, double>), gr_, dim3(bt_), by_, __VA_ARGS__);             \
+      else KLAUNCH_N(#kern, (kern<2, double>), gr_, dim3(bt_), by_, __VA_ARGS__);                         \
     }                                                                                              \
   } while (0)
 
@@ -342,7 +364,7 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
   CHK(fillU(x, 1 << d, 2, 1, lr, 3));
   CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
   A = sweep_io(x, d);
-  SWEEP_LAUNCH(k_rhs, d, 33, A, x->g, dt);
+  SWEEP_LAUNCH(k_rhs, d, true, 33, A, x->g, dt);
   swapU(x, 0); swapU(x, A.VN); swapSp(x); swapBlg(x);
   // MOC (fills M3:1950-1976): reads rho, v_t(2), Blg(2), p, e, widths, Bn;
   // writes momenta(3), E, Blg(2)
@@ -351,7 +373,7 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
   CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
   CHK(fillOT(x, 1 << d, 2, OT_BLG));
   A = sweep_io(x, d);
-  SWEEP_LAUNCH(k_moc, d, 20, A, x->g, dt);
+  SWEEP_LAUNCH(k_moc, d, true, 20, A, x->g, dt);
   swapU(x, A.VN); swapU(x, A.VT1); swapU(x, A.VT2); swapU(x, 4); swapBlg(x);
   // REMAP (fills M3:1571-1594): reads rho, m(3), E, Blg(2), species,
   // widths, interface velocity, Bcc_n; writes rho, v(3), p, species
@@ -360,7 +382,7 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
   CHK(fillU(x, 1 << d, 2, 2, l5, 5));
   CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
   A = sweep_io(x, d);
-  SWEEP_LAUNCH(k_remap, d, 31, A, x->g, dt);
+  SWEEP_LAUNCH(k_remap, d, false, 31, A, x->g, dt);
   for (int f = 0; f < 5; ++f) swapU(x, f);
   swapSp(x);
   return MCF_OK;
@@ -402,14 +424,17 @@ static int do_ct(mcf_ctx *x, double dt) {
     // slopes along d1 (dQ2, dQ4)
     lo[d1] = 0; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
     Box bs0 = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    KLAUNCH(k_cct_slopes, boxsize(bs0), 6, A, x->g, bs0, o, 0);
+    if (x->mixed) KLAUNCH_R(k_cct_slopes, float, boxsize(bs0), 6, A, x->g, bs0, o, 0);
+    else KLAUNCH_R(k_cct_slopes, double, boxsize(bs0), 6, A, x->g, bs0, o, 0);
     // slopes along d2 (dQ1, dQ3)
     lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 0; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
     Box bs1 = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    KLAUNCH(k_cct_slopes, boxsize(bs1), 6, A, x->g, bs1, o, 1);
+    if (x->mixed) KLAUNCH_R(k_cct_slopes, float, boxsize(bs1), 6, A, x->g, bs1, o, 1);
+    else KLAUNCH_R(k_cct_slopes, double, boxsize(bs1), 6, A, x->g, bs1, o, 1);
     lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
     Box be = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    KLAUNCH(k_cct_emf, boxsize(be), 10, A, x->g, be, o, dt);
+    if (x->mixed) KLAUNCH_R(k_cct_emf, float, boxsize(be), 10, A, x->g, be, o, dt);
+    else KLAUNCH_R(k_cct_emf, double, boxsize(be), 10, A, x->g, be, o, dt);
   }
   KLAUNCH(k_cc3, boxsize(b1), 9, A, x->g, b1, dt);
   return do_cc4(x);
@@ -469,11 +494,12 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   }
   // DISSIP works on 10-cell layers at the x ends (M3:3152-3170)
   if (n[0] / nsp[0] < 10) { g_err = "x block must hold at least 10 cells (DISSIP layers)"; return MCF_E_ARG; }
-  if (cfg->precision != MCF_PREC_FP64) { g_err = "precision mode not built"; return MCF_E_ARG; }
+  if (cfg->precision != MCF_PREC_FP64 && cfg->precision != MCF_PREC_MIXED) { g_err = "unknown precision mode"; return MCF_E_ARG; }
   long long vol = (long long)(n[0] + 4) * (n[1] + 4) * (n[2] + 4);
   if (vol >= (1LL << 31)) { g_err = "grid too large for 32-bit indexing"; return MCF_E_ARG; }
   mcf_ctx *x = new mcf_ctx();
   x->cfg = *cfg;
+  x->mixed = cfg->precision == MCF_PREC_MIXED;
   for (int d = 0; d < 3; ++d) { x->g.n[d] = n[d]; x->nsp[d] = nsp[d]; }
   x->g.st[0] = 1;
   x->g.st[1] = n[0] + 4;
@@ -488,7 +514,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
   size_t ngrid = 0;
   for (int d = 0; d < 3; ++d) ngrid += 2 * (size_t)(n[d] + 4);
-  size_t total = (nstate + nbuf + nsweep + nscratch) * nc + ndivv + ngrid + 16;
+  size_t total = (nstate + nbuf + nsweep + nscratch) * nc + ndivv + ngrid + 16 + 2 * DT_BLOCKS;
   e = hipMalloc(&x->pool, total * sizeof(double));
   if (e != hipSuccess) { g_err = std::string("hipMalloc: ") + hipGetErrorString(e); (void)hipStreamDestroy(x->stream); delete x; return MCF_E_NOMEM; }
   double *p = x->pool;
@@ -506,7 +532,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   x->Tn = p; p += nc; x->Pn = p; p += nc;
   x->divv = p; p += ndivv;
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
-  x->red = p; p += 16;
+  x->red = p; p += 16 + 2 * DT_BLOCKS;
   HIPCHK(hipMemsetAsync(x->pool, 0, total * sizeof(double), x->stream));
   for (int d = 0; d < 3; ++d) {
     HIPCHK(hipMemcpyAsync(x->gx[d], x->hx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
@@ -581,10 +607,10 @@ extern "C" int mcf_timestep(mcf_ctx *x, double tsave, double *dt_out, int *tag) 
   for (int c = 0; c < 3; ++c) { A.bcc[c] = x->bcc[c]; A.gdx[c] = x->gdx[c]; }
   A.out = x->red;
   A.tfinal = x->cfg.tfinal;
-  hipLaunchKernelGGL(k_dtinit, dim3(1), dim3(64), 0, x->stream, x->red, x->cfg.tfinal);
-  HIPCHK(hipGetLastError());
-  Box b = mkbox(1, x->g.n[0], 1, x->g.n[1], 1, x->g.n[2]);
-  KLAUNCH(k_dtmin, boxsize(b), 13, A, x->g, b);
+  long rows = (long)x->g.n[1] * x->g.n[2];
+  unsigned nb = (unsigned)(rows < DT_BLOCKS ? rows : DT_BLOCKS);
+  KLAUNCH_G(k_dtmin, dim3(nb), dim3(256), 104.0 * (double)rows * x->g.n[0], A, x->g, x->red + 16);
+  KLAUNCH_G(k_dtfinal, dim3(1), dim3(256), 16.0 * nb, x->red + 16, (int)nb, x->red, x->cfg.tfinal);
   HIPCHK(hipMemcpyAsync(x->hred, x->red, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   double tCFL = x->hred[0], tLMT = x->hred[1];

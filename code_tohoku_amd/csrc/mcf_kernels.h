@@ -56,9 +56,10 @@ struct SweepIO {
   double spmin[8];
 };
 
-template <int D> __global__ void k_rhs(SweepIO A, Geo g, double dt);
-template <int D> __global__ void k_moc(SweepIO A, Geo g, double dt);
-template <int D> __global__ void k_remap(SweepIO A, Geo g, double dt);
+// R = double: strict FP64; R = float: mixed precision
+template <int D, class R> __global__ void k_rhs(SweepIO A, Geo g, double dt);
+template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
+template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);
 
 struct CtArgs {
   double *U[8];
@@ -96,14 +97,14 @@ struct DisArgs {
 __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
-__global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode);
-__global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt);
+template <class R> __global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode);
+template <class R> __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
 __global__ void k_src_temp(SrcArgs A, Geo g, Box b, int savep);
 __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
-__global__ void k_dtmin(DtArgs A, Geo g, Box b);
-__global__ void k_dtinit(double *out, double v);
+__global__ void k_dtmin(DtArgs A, Geo g, double *part);
+__global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal);
 __global__ void k_dissip_div(DisArgs A, Geo g, int e0);
 __global__ void k_dissip_upd(DisArgs A, Geo g, int e0);
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n);

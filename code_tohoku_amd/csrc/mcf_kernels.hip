@@ -9,6 +9,7 @@
 // reads the arrays its reference routine reads, so the fills between stages
 // reproduce the reference's halo contents exactly.
 #include "mcf_kernels.h"
+#include "mcf_prec.h"
 
 namespace mcf {
 
@@ -130,7 +131,9 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b) {
 }
 
 // CCT slopes: mode 0 -> dQ2/dQ4 (along d1 of Vfc_d2, B_d2), mode 1 ->
-// dQ1/dQ3 (along d2 of Vfc_d1, B_d1) (M3:2203-2268).
+// dQ1/dQ3 (along d2 of Vfc_d1, B_d1) (M3:2203-2268).  The dQ arrays hold
+// values of type R (FP32 on the mixed path).
+template <class R>
 __global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode) {
   int i, j, k;
   if (!box_ijk(b, i, j, k)) return;
@@ -141,17 +144,20 @@ __global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode) {
   int s = g.st[ax];
   int p = ax == 0 ? i : ax == 1 ? j : k;
   const double *dx = A.gdx[ax];
-  double dxm = dx[p], dx0 = dx[p + 1], dxp = dx[p + 2];
-  double T = tswitch(A.U[4][c], A.U[0][c]);
+  const R dxm = (R)dx[p], dx0 = (R)dx[p + 1], dxp = (R)dx[p + 2];
+  const bool mm = use_minmod_r<R>(A.U[4][c], A.U[0][c]);
   const double *V = A.vfc[fv], *B = A.U[5 + fv];
-  double sv = vlim_pick(vlim_parts(V[c - s], V[c], V[c + s], dxm, dx0, dxp), T);
-  double sbv = vlim_pick(vlim_parts(B[c - s], B[c], B[c + s], dxm, dx0, dxp), T);
-  if (mode == 0) { A.dq[1][c] = sv; A.dq[3][c] = sbv; }
-  else { A.dq[0][c] = sv; A.dq[2][c] = sbv; }
+  R sv = vslope<R>(V[c - s], V[c], V[c + s], dxm, dx0, dxp, mm);
+  R sbv = vslope<R>(B[c - s], B[c], B[c + s], dxm, dx0, dxp, mm);
+  R *const *dq = reinterpret_cast<R *const *>(A.dq);
+  if (mode == 0) { dq[1][c] = sv; dq[3][c] = sbv; }
+  else { dq[0][c] = sv; dq[2][c] = sbv; }
 }
 
-// CCT edge EMF (M3:2271-2441): one thread per edge.
-__global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt) {
+// CCT edge EMF (M3:2271-2441): one thread per edge, arithmetic in R.
+template <class R>
+__global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt_d) {
+  using Q = Ar<R>;
   int i, j, k;
   if (!box_ijk(b, i, j, k)) return;
   const int d1 = orient, d2 = (orient + 1) % 3;
@@ -160,77 +166,88 @@ __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt) {
   const int p1 = d1 == 0 ? i : d1 == 1 ? j : k;
   const int p2 = d2 == 0 ? i : d2 == 1 ? j : k;
   const double *r = A.U[0];
-  double dnc;
-  if (orient == 2) dnc = 0.25 * (r[c0] + r[c0 - s2] + r[c0 - s1] + r[c0 - s1 - s2]);
-  else dnc = 0.25 * (r[c0] + r[c0 - s1] + r[c0 - s2] + r[c0 - s1 - s2]);
-  const double dsqinv = 1.0 / sqrt(dnc);
+  double dncd;
+  if (orient == 2) dncd = 0.25 * (r[c0] + r[c0 - s2] + r[c0 - s1] + r[c0 - s1 - s2]);
+  else dncd = 0.25 * (r[c0] + r[c0 - s1] + r[c0 - s2] + r[c0 - s1 - s2]);
+  const R dnc = (R)dncd;
+  const R dt = (R)dt_d;
+  const R dsqinv = Q::div((R)1.0, Q::sqrt_(dnc));
   const double *Vx = A.vfc[d1], *Vy = A.vfc[d2], *Bx = A.U[5 + d1], *By = A.U[5 + d2];
-  const double facm = A.facdep * 0.5;
+  const R *const *dq = reinterpret_cast<const R *const *>(A.dq);
+  const R facm = (R)A.facdep * (R)0.5;
   const double *dxe = A.gdx[d1], *dye = A.gdx[d2];
-  double C1p[4], C1m[4], C2p[4], C2m[4];
+  R C1p[4], C1m[4], C2p[4], C2m[4];
 #pragma unroll
   for (int cs = 0; cs < 4; ++cs) {
     // 0: LL (S1=+1,S2=+1)  1: RL (-1,+1)  2: LR (+1,-1)  3: RR (-1,-1)
-    double S1 = (cs == 0 || cs == 2) ? 1.0 : -1.0;
-    double S2 = (cs == 0 || cs == 1) ? 1.0 : -1.0;
-    int q1 = (cs == 0 || cs == 2) ? c0 - s2 : c0;
-    int q2 = (cs == 0 || cs == 1) ? c0 - s1 : c0;
-    double dyv = (cs == 0 || cs == 2) ? dye[p2] : dye[p2 + 1];
-    double dxv = (cs == 0 || cs == 1) ? dxe[p1] : dxe[p1 + 1];
-    double a1p = Vx[q1] + Bx[q1] * dsqinv;
-    double da1p = (A.dq[0][q1] + A.dq[2][q1] * dsqinv) * facm;
-    double a1m = Vx[q1] - Bx[q1] * dsqinv;
-    double da1m = (A.dq[0][q1] - A.dq[2][q1] * dsqinv) * facm;
-    double a2m = Vy[q2] - By[q2] * dsqinv;
-    double da2m = (A.dq[1][q2] - A.dq[3][q2] * dsqinv) * facm;
-    double a2p = Vy[q2] + By[q2] * dsqinv;
-    double da2p = (A.dq[1][q2] + A.dq[3][q2] * dsqinv) * facm;
-    double t;
+    const R S1 = (cs == 0 || cs == 2) ? (R)1.0 : (R)-1.0;
+    const R S2 = (cs == 0 || cs == 1) ? (R)1.0 : (R)-1.0;
+    const int q1 = (cs == 0 || cs == 2) ? c0 - s2 : c0;
+    const int q2 = (cs == 0 || cs == 1) ? c0 - s1 : c0;
+    const R dyv = (R)((cs == 0 || cs == 2) ? dye[p2] : dye[p2 + 1]);
+    const R dxv = (R)((cs == 0 || cs == 1) ? dxe[p1] : dxe[p1 + 1]);
+    const R vx = (R)Vx[q1], bx = (R)Bx[q1], vy = (R)Vy[q2], by = (R)By[q2];
+    R a1p = vx + bx * dsqinv;
+    R da1p = (dq[0][q1] + dq[2][q1] * dsqinv) * facm;
+    R a1m = vx - bx * dsqinv;
+    R da1m = (dq[0][q1] - dq[2][q1] * dsqinv) * facm;
+    R a2m = vy - by * dsqinv;
+    R da2m = (dq[1][q2] - dq[3][q2] * dsqinv) * facm;
+    R a2p = vy + by * dsqinv;
+    R da2p = (dq[1][q2] + dq[3][q2] * dsqinv) * facm;
+    R t;
     t = a1p + da1p * dyv * S1 - da1p * (a2m + da2m * dxv * S2) * dt;
-    C1p[cs] = t / (1.0 - da1p * da2m * dt * dt);
+    C1p[cs] = Q::div(t, (R)1.0 - da1p * da2m * dt * dt);
     t = a2m + da2m * dxv * S2 - da2m * (a1p + da1p * dyv * S1) * dt;
-    C2m[cs] = t / (1.0 - da1p * da2m * dt * dt);
+    C2m[cs] = Q::div(t, (R)1.0 - da1p * da2m * dt * dt);
     t = a1m + da1m * dyv * S1 - da1m * (a2p + da2p * dxv * S2) * dt;
-    C1m[cs] = t / (1.0 - da1m * da2p * dt * dt);
+    C1m[cs] = Q::div(t, (R)1.0 - da1m * da2p * dt * dt);
     t = a2p + da2p * dxv * S2 - da2p * (a1m + da1m * dyv * S1) * dt;
-    C2p[cs] = t / (1.0 - da1m * da2p * dt * dt);
+    C2p[cs] = Q::div(t, (R)1.0 - da1m * da2p * dt * dt);
   }
+  const R z = (R)0.0;
   int f0[4], f1[4];
-  f0[0] = (C1p[0] > 0.0) && (C2m[0] > 0.0);
-  f1[0] = (C1m[0] > 0.0) && (C2p[0] > 0.0);
-  f0[1] = (C1p[1] > 0.0) && (C2m[1] < 0.0);
-  f1[1] = (C1m[1] > 0.0) && (C2p[1] < 0.0);
-  f0[2] = (C1p[2] < 0.0) && (C2m[2] > 0.0);
-  f1[2] = (C1m[2] < 0.0) && (C2p[2] > 0.0);
-  f0[3] = (C1p[3] < 0.0) && (C2m[3] < 0.0);
-  f1[3] = (C1m[3] < 0.0) && (C2p[3] < 0.0);
-  double W0[4], W1[4];
+  f0[0] = (C1p[0] > z) && (C2m[0] > z);
+  f1[0] = (C1m[0] > z) && (C2p[0] > z);
+  f0[1] = (C1p[1] > z) && (C2m[1] < z);
+  f1[1] = (C1m[1] > z) && (C2p[1] < z);
+  f0[2] = (C1p[2] < z) && (C2m[2] > z);
+  f1[2] = (C1m[2] < z) && (C2p[2] > z);
+  f0[3] = (C1p[3] < z) && (C2m[3] < z);
+  f1[3] = (C1m[3] < z) && (C2p[3] < z);
+  R W0[4], W1[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
+    // the weight of a candidate pair whose two flags are both set is 1e13
     int both = (int)(0.51 * (double)f0[m] + 0.51 * (double)f1[m]);
-    W0[m] = (double)f0[m] + (1.0e13 - 1.0) * (double)both;
-    W1[m] = (double)f1[m] + (1.0e13 - 1.0) * (double)both;
+    W0[m] = (R)f0[m] + (R)(1.0e13 - 1.0) * (R)both;
+    W1[m] = (R)f1[m] + (R)(1.0e13 - 1.0) * (R)both;
   }
-  double c1p = 0.0, c2m = 0.0, c1m = 0.0, c2p = 0.0;
+  R c1p = z, c2m = z, c1m = z, c2p = z;
   if (f0[0] + f0[1] + f0[2] + f0[3] != 0) {
     c1p = W0[0] * fabs(C2m[0]) * C1p[0] + W0[1] * fabs(C2m[1]) * C1p[1] + W0[2] * fabs(C2m[2]) * C1p[2] +
           W0[3] * fabs(C2m[3]) * C1p[3];
-    c1p = c1p / (W0[0] * fabs(C2m[0]) + W0[1] * fabs(C2m[1]) + W0[2] * fabs(C2m[2]) + W0[3] * fabs(C2m[3]));
+    c1p = Q::div(c1p, W0[0] * fabs(C2m[0]) + W0[1] * fabs(C2m[1]) + W0[2] * fabs(C2m[2]) + W0[3] * fabs(C2m[3]));
     c2m = W0[0] * fabs(C1p[0]) * C2m[0] + W0[1] * fabs(C1p[1]) * C2m[1] + W0[2] * fabs(C1p[2]) * C2m[2] +
           W0[3] * fabs(C1p[3]) * C2m[3];
-    c2m = c2m / (W0[0] * fabs(C1p[0]) + W0[1] * fabs(C1p[1]) + W0[2] * fabs(C1p[2]) + W0[3] * fabs(C1p[3]));
+    c2m = Q::div(c2m, W0[0] * fabs(C1p[0]) + W0[1] * fabs(C1p[1]) + W0[2] * fabs(C1p[2]) + W0[3] * fabs(C1p[3]));
   }
   if (f1[0] + f1[1] + f1[2] + f1[3] != 0) {
     c1m = W1[0] * fabs(C2p[0]) * C1m[0] + W1[1] * fabs(C2p[1]) * C1m[1] + W1[2] * fabs(C2p[2]) * C1m[2] +
           W1[3] * fabs(C2p[3]) * C1m[3];
-    c1m = c1m / (W1[0] * fabs(C2p[0]) + W1[1] * fabs(C2p[1]) + W1[2] * fabs(C2p[2]) + W1[3] * fabs(C2p[3]));
+    c1m = Q::div(c1m, W1[0] * fabs(C2p[0]) + W1[1] * fabs(C2p[1]) + W1[2] * fabs(C2p[2]) + W1[3] * fabs(C2p[3]));
     c2p = W1[0] * fabs(C1m[0]) * C2p[0] + W1[1] * fabs(C1m[1]) * C2p[1] + W1[2] * fabs(C1m[2]) * C2p[2] +
           W1[3] * fabs(C1m[3]) * C2p[3];
-    c2p = c2p / (W1[0] * fabs(C1m[0]) + W1[1] * fabs(C1m[1]) + W1[2] * fabs(C1m[2]) + W1[3] * fabs(C1m[3]));
+    c2p = Q::div(c2p, W1[0] * fabs(C1m[0]) + W1[1] * fabs(C1m[1]) + W1[2] * fabs(C1m[2]) + W1[3] * fabs(C1m[3]));
   }
-  double v1a = c1p + c1m, v2a = c2p + c2m, B1a = c1p - c1m, B2a = c2p - c2m;
-  A.emf[(orient + 2) % 3][c0] = 0.25 * (v1a * B2a - v2a * B1a) * sqrt(dnc);
+  R v1a = c1p + c1m, v2a = c2p + c2m, B1a = c1p - c1m, B2a = c2p - c2m;
+  A.emf[(orient + 2) % 3][c0] = (double)((R)0.25 * (v1a * B2a - v2a * B1a) * Q::sqrt_(dnc));
 }
+
+template __global__ void k_cct_slopes<double>(CtArgs, Geo, Box, int, int);
+template __global__ void k_cct_slopes<float>(CtArgs, Geo, Box, int, int);
+template __global__ void k_cct_emf<double>(CtArgs, Geo, Box, int, double);
+template __global__ void k_cct_emf<float>(CtArgs, Geo, Box, int, double);
 
 // CC(3): face-B induction (M3:2159-2169)
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt) {
@@ -328,36 +345,45 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-__global__ void k_dtmin(DtArgs A, Geo g, Box b) {
+// Two-pass min: each block strides over (j,k) rows and writes one pair of
+// partial minima; k_dtfinal folds them.  min is exact, so the result does
+// not depend on the order.
+__global__ void k_dtmin(DtArgs A, Geo g, double *part) {
   __shared__ double s0[16], s1[16];
-  int i, j, k;
+  const int nx = g.n[0];
+  const long rows = (long)g.n[1] * g.n[2];
+  const double *const *sp = A.sp;
   double tc = A.tfinal, tl = A.tfinal;
-  if (box_ijk(b, i, j, k)) {
-    int c = ix3(g, i, j, k);
-    const double *const *sp = A.sp;
-    double rho = A.U[0][c], p = A.U[4][c];
-    double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
-    double gamma = gam_of(sp, c);
-    double c2 = (gamma * p + bx * bx + by * by + bz * bz) / rho;
-    double dxi = A.gdx[0][i + 1], dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
-    double t = dxi / (sqrt(c2) + fabs(A.U[1][c]));
-    tc = dmin(tc, t);
-    t = dyj / (sqrt(c2) + fabs(A.U[2][c]));
-    tc = dmin(tc, t);
-    t = dzk / (sqrt(c2) + fabs(A.U[3][c]));
-    tc = dmin(tc, t);
-    double Nn = sp[1][c] + sp[0][c] + sp[2][c] + sp[3][c] + sp[4][c];
-    double Tn = p / (KB * Nn);
-    double dl = dmin(dxi, dyj);
-    dl = dmin(dl, dzk);
-    double gi = gammi1_of(sp, c);
-    double alpha = gi * KCOND * sqrt(dmin(Tn, 15.0));
-    alpha = 0.4 * Nn * KB / alpha * (dl * dl);
-    double CooL = fcool_ki(rho, Tn);
-    double tauC = p / gi / fabs(CooL);
-    tauC = dmax(0.2 * tauC, 2.5e-4);
-    tl = dmin(tl, tauC);
-    tl = dmin(tl, alpha);
+  for (long r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = 1 + (int)(r % g.n[1]), k = 1 + (int)(r / g.n[1]);
+    double dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
+    for (int i = 1 + threadIdx.x; i <= nx; i += blockDim.x) {
+      int c = ix3(g, i, j, k);
+      double rho = A.U[0][c], p = A.U[4][c];
+      double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
+      double gamma = gam_of(sp, c);
+      double c2 = (gamma * p + bx * bx + by * by + bz * bz) / rho;
+      double dxi = A.gdx[0][i + 1];
+      double cf = sqrt(c2);
+      double t = dxi / (cf + fabs(A.U[1][c]));
+      tc = dmin(tc, t);
+      t = dyj / (cf + fabs(A.U[2][c]));
+      tc = dmin(tc, t);
+      t = dzk / (cf + fabs(A.U[3][c]));
+      tc = dmin(tc, t);
+      double Nn = sp[1][c] + sp[0][c] + sp[2][c] + sp[3][c] + sp[4][c];
+      double Tn = p / (KB * Nn);
+      double dl = dmin(dxi, dyj);
+      dl = dmin(dl, dzk);
+      double gi = gammi1_of(sp, c);
+      double alpha = gi * KCOND * sqrt(dmin(Tn, 15.0));
+      alpha = 0.4 * Nn * KB / alpha * (dl * dl);
+      double CooL = fcool_ki(rho, Tn);
+      double tauC = p / gi / fabs(CooL);
+      tauC = dmax(0.2 * tauC, 2.5e-4);
+      tl = dmin(tl, tauC);
+      tl = dmin(tl, alpha);
+    }
   }
   tc = wave_min(tc);
   tl = wave_min(tl);
@@ -367,14 +393,29 @@ __global__ void k_dtmin(DtArgs A, Geo g, Box b) {
   if (threadIdx.x == 0) {
     int nw = (blockDim.x + 63) >> 6;
     for (int w = 1; w < nw; ++w) { tc = fmin(tc, s0[w]); tl = fmin(tl, s1[w]); }
-    // positive doubles order like their bit patterns
-    atomicMin((unsigned long long *)&A.out[0], (unsigned long long)__double_as_longlong(tc));
-    atomicMin((unsigned long long *)&A.out[1], (unsigned long long)__double_as_longlong(tl));
+    part[2 * blockIdx.x] = tc;
+    part[2 * blockIdx.x + 1] = tl;
   }
 }
 
-__global__ void k_dtinit(double *out, double v) {
-  if (threadIdx.x < 2) out[threadIdx.x] = v;
+__global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal) {
+  __shared__ double s0[16], s1[16];
+  double tc = tfinal, tl = tfinal;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    tc = fmin(tc, part[2 * b]);
+    tl = fmin(tl, part[2 * b + 1]);
+  }
+  tc = wave_min(tc);
+  tl = wave_min(tl);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { s0[wid] = tc; s1[wid] = tl; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) { tc = fmin(tc, s0[w]); tl = fmin(tl, s1[w]); }
+    out[0] = tc;
+    out[1] = tl;
+  }
 }
 
 // ---------------------------------------------------------------------------

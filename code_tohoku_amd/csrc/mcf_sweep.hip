@@ -2,18 +2,33 @@
 // kernels: RHS (M3:1330-1467), MOC (M3:1939-2098), REMAP + primitive
 // recovery (M3:1556-1686, 1291-1321).
 //
-// Each kernel computes the interface values of a tile once into LDS and
-// then updates the tile's cells from LDS, so no per-interface array goes to
-// HBM.  A block is 576 threads: waves 0..7 own a tile of 8 lines x 64 cells
-// and the interface to the right of each cell; wave 8 computes the one
-// interface left of each line.
-//   x sweep: a line is 64 consecutive x cells (one wave), 8 lines = 8 y rows.
-//   y/z sweep: a wave is 64 consecutive x cells at one position along the
-//   sweep axis, 8 waves = 8 positions; loads stay coalesced along x.
-// Stage outputs go to the other buffer of each written field (the host
-// swaps the pointers), so a tile never reads cells another tile has
-// already updated.
+// RHS and MOC run in three phases per tile, with LDS between them:
+//   1. every cell of the tile and its two halo cells along the sweep axis
+//      computes its limited slopes once (RHS also its left/right
+//      reconstructed states);
+//   2. every interface of the tile solves its Riemann / characteristic
+//      problem from the two cells' LDS entries;
+//   3. every cell is updated from its two interfaces.
+// REMAP evaluates the upwind slope per interface (one cell per interface)
+// and then updates the cells.  No per-interface array goes to HBM except
+// the interface velocity and Lagrangian position that later stages read.
+//
+// Tiles.  x sweep: 8 lines of 64 consecutive x cells (one wave per line,
+// lanes along x); wave 8 takes the halo cells and the leftmost interface of
+// the 8 lines.  y/z sweeps: 64 consecutive x cells (lanes) at 8 positions
+// along the sweep axis (one wave per position); wave 8 takes the position
+// before the tile (cell and interface), wave 9 the position after it
+// (cell only).  All loads stay coalesced along x.
+//
+// Outputs go to the other buffer of each written field (the host swaps the
+// pointers), so a tile never reads cells another tile has already updated.
+//
+// The kernels are generic over the compute type R (mcf_prec.h):
+// R = double reproduces the reference bit for bit; R = float is the mixed
+// path (FP32 reconstruction, Riemann and characteristic states; state
+// updates, positions and primitive recovery in FP64).
 #include "mcf_kernels.h"
+#include "mcf_prec.h"
 
 namespace mcf {
 
@@ -33,70 +48,121 @@ __device__ __forceinline__ double gammi1_of_s(const double *const *sp, int c) {
   double g = 3.0 * s4 + 5.0 * sp[2][c];
   return (2.0 * s4 + 2.0 * sp[2][c]) / g;
 }
+template <class R> __device__ __forceinline__ R gam_of_r(const double *const *sp, int c) {
+  if constexpr (Ar<R>::strict) {
+    return gam_of_s(sp, c);
+  } else {
+    float s4 = (float)(sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c]);
+    float h2 = (float)sp[2][c];
+    return Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
+  }
+}
 
-constexpr int LW = 65;        // LDS row pitch (x layout: 65 interfaces per line)
-constexpr int NSLOT = 9 * 65; // LDS interface slots per value
+constexpr int XL = 66; // x sweep: cell slots per line (64 + 2 halo)
+constexpr int XF = 65; // x sweep: interface slots per line
 
-// Thread roles of a 576-thread tile.
-struct Tile {
-  int i, j, k;   // cell of a main thread / left cell of the extra interface
-  int q;         // position of that cell along the sweep axis
-  int fslot;     // LDS slot of the interface computed by this thread
-  int lslot, rslot;
-  bool face_ok;  // this thread computes an interface (position 0..N)
-  bool cell_ok;  // this thread updates a cell (main waves, inside 1..N)
-  bool own;      // interface owned by this tile (not the left halo of a later tile)
+// Thread roles in a tile.  The phase-1 cell of a thread is also the left
+// cell of its interface (phase 2) and, for the main waves, the cell it
+// updates (phase 3).
+struct Roles {
+  int i, j, k, q;    // phase-1 cell and its position along the sweep axis
+  int cslot;         // LDS slot of that cell
+  int rslot_c;       // LDS slot of the cell to its right
+  int fslot;         // LDS slot of the interface right of the cell
+  int lslot, rslot;  // interface slots left/right of an updated cell
+  bool c_ok;         // phase 1: cell exists (position 0..N+1)
+  bool f_ok;         // phase 2: interface exists (position 0..N)
+  bool own;          // interface is written to HBM by this tile
+  bool u_ok;         // phase 3: cell 1..N updated by this thread
 };
 
 template <int D>
-__device__ __forceinline__ Tile tile_map(const Geo &g) {
-  Tile t;
+__device__ __forceinline__ Roles roles(const Geo &g) {
+  Roles r;
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  r.lslot = r.rslot = 0;
+  r.u_ok = false;
   if (D == 0) {
     const int s0 = 1 + 64 * (int)blockIdx.x;
     const int jr = 1 + 8 * (int)blockIdx.y;
-    const int kk = 1 + (int)blockIdx.z;
+    r.k = 1 + (int)blockIdx.z;
     if (w < 8) {
-      t.i = s0 + l; t.j = jr + w; t.k = kk;
-      t.fslot = w * LW + l + 1; t.lslot = w * LW + l; t.rslot = w * LW + l + 1;
-      t.face_ok = t.i <= nx && t.j <= ny;
-      t.cell_ok = t.face_ok;
-      t.own = true;
+      r.i = s0 + l; r.j = jr + w;
+      r.cslot = w * XL + l + 1;
+      r.fslot = w * XF + l + 1;
+      r.lslot = w * XF + l; r.rslot = w * XF + l + 1;
+      bool row = r.j <= ny;
+      r.c_ok = row && r.i <= nx + 1;
+      r.f_ok = row && r.i <= nx;
+      r.u_ok = r.f_ok;
+      r.own = true;
+    } else if (l < 8) {
+      r.i = s0 - 1; r.j = jr + l;
+      r.cslot = l * XL;
+      r.fslot = l * XF;
+      r.c_ok = r.f_ok = r.j <= ny;
+      r.own = (s0 == 1);
     } else {
-      t.i = s0 - 1; t.j = jr + l; t.k = kk;
-      t.fslot = l * LW; t.lslot = t.rslot = 0;
-      t.face_ok = l < 8 && t.j <= ny;
-      t.cell_ok = false;
-      t.own = (s0 - 1 == 0);
+      const int ll = l & 7;
+      r.i = s0 + 64; r.j = jr + ll;
+      r.cslot = ll * XL + 65;
+      r.fslot = 0;
+      r.c_ok = l < 16 && r.j <= ny && r.i <= nx + 1;
+      r.f_ok = false;
+      r.own = false;
     }
-    t.q = t.i;
+    r.rslot_c = r.cslot + 1;
+    r.q = r.i;
   } else {
     const int i0 = 1 + 64 * (int)blockIdx.x;
     const int s0 = 1 + 8 * (int)blockIdx.y;
     const int tr = 1 + (int)blockIdx.z;
     const int n = (D == 1) ? ny : nz;
     const int ntr = (D == 1) ? nz : ny;
+    const bool col = (i0 + l <= nx) && (tr <= ntr);
     int p;
     if (w < 8) {
       p = s0 + w;
-      t.fslot = (w + 1) * 64 + l; t.lslot = w * 64 + l; t.rslot = (w + 1) * 64 + l;
-      t.face_ok = p <= n && i0 + l <= nx && tr <= ntr;
-      t.cell_ok = t.face_ok;
-      t.own = true;
-    } else {
+      r.cslot = (w + 1) * 64 + l;
+      r.fslot = (w + 1) * 64 + l;
+      r.lslot = w * 64 + l; r.rslot = (w + 1) * 64 + l;
+      r.c_ok = col && p <= n + 1;
+      r.f_ok = col && p <= n;
+      r.u_ok = r.f_ok;
+      r.own = true;
+    } else if (w == 8) {
       p = s0 - 1;
-      t.fslot = l; t.lslot = t.rslot = 0;
-      t.face_ok = i0 + l <= nx && tr <= ntr;
-      t.cell_ok = false;
-      t.own = (s0 - 1 == 0);
+      r.cslot = l;
+      r.fslot = l;
+      r.c_ok = r.f_ok = col;
+      r.own = (s0 == 1);
+    } else {
+      p = s0 + 8;
+      r.cslot = 9 * 64 + l;
+      r.fslot = 0;
+      r.c_ok = col && p <= n + 1;
+      r.f_ok = false;
+      r.own = false;
     }
-    t.i = i0 + l;
-    if (D == 1) { t.j = p; t.k = tr; } else { t.k = p; t.j = tr; }
-    t.q = p;
+    r.rslot_c = r.cslot + 64;
+    r.i = i0 + l;
+    if (D == 1) { r.j = p; r.k = tr; } else { r.k = p; r.j = tr; }
+    r.q = p;
   }
-  return t;
+  return r;
+}
+
+template <int D> constexpr int ncell_slots() { return D == 0 ? 8 * XL : 10 * 64; }
+template <int D> constexpr int nface_slots() { return D == 0 ? 8 * XF : 9 * 64; }
+
+// Reconstructed states q +- depend*g with the positivity fallback (M3:1389).
+template <class R> __device__ __forceinline__ R left_state_pos(double q, R depend, R gr) {
+  return (R)posfix(q + (double)(depend * gr), q);
+}
+template <class R> __device__ __forceinline__ R right_state_pos(double q, R depend, R gr) {
+  return (R)posfix(q - (double)(depend * gr), q);
 }
 
 } // namespace
@@ -104,255 +170,363 @@ __device__ __forceinline__ Tile tile_map(const Geo &g) {
 // ---------------------------------------------------------------------------
 // RHS
 // ---------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(576) void k_rhs(SweepIO A, Geo g, double dt) {
-  __shared__ double sPa[NSLOT], sVa[NSLOT];
-  const Tile t = tile_map<D>(g);
+template <int D, class R>
+__global__ __launch_bounds__(D == 0 ? 576 : 640) void k_rhs(SweepIO A, Geo g, double dt) {
+  constexpr int NC = ncell_slots<D>(), NF = nface_slots<D>();
+  __shared__ R sQ[8][NC];       // QL(rho, v_n, p, |B_t|^2) and QR(...) per cell
+  __shared__ R sPa[NF], sVa[NF];
+  const Roles t = roles<D>(g);
   const int s = g.st[D];
   const double *dxe = A.dxe, *xe = A.xe;
-  if (t.face_ok) {
+  const double *U1 = A.U[0], *UVN = A.U[A.VN], *U5 = A.U[4];
+  const double *B1 = A.bcc[A.BT1], *B2 = A.bcc[A.BT2];
+  if (t.c_ok) {
+    const int q = t.q;
+    const int c = ix3s(g, t.i, t.j, t.k);
+    const double rho = U1[c], vn = UVN[c], p = U5[c], b1 = B1[c], b2 = B2[c];
+    const R dxm = (R)dxe[q], dx0 = (R)dxe[q + 1], dxp = (R)dxe[q + 2];
+    const bool mm = use_minmod_r<R>(p, rho);
+    const R g1 = vslope<R>(U1[c - s], rho, U1[c + s], dxm, dx0, dxp, mm);
+    const R gv = vslope<R>(UVN[c - s], vn, UVN[c + s], dxm, dx0, dxp, mm);
+    const R g5 = vslope<R>(U5[c - s], p, U5[c + s], dxm, dx0, dxp, mm);
+    const R gA = vslope<R>(B1[c - s], b1, B1[c + s], dxm, dx0, dxp, mm);
+    const R gB = vslope<R>(B2[c - s], b2, B2[c + s], dxm, dx0, dxp, mm);
+    // fast magnetosonic speed and the traced half-width (M3:1380-1388)
+    const R gamma = gam_of_r<R>(A.sp, c);
+    const R cm = Ar<R>::sqrt_(Ar<R>::div(gamma * (R)p + (R)(b1 * b1) + (R)(b2 * b2), (R)rho));
+    R depend = (R)0.5 * (dx0 - cm * (R)dt) * (R)A.facdep;
+    depend = rmax((R)0.0, depend);
+    sQ[0][t.cslot] = left_state_pos<R>(rho, depend, g1);
+    sQ[1][t.cslot] = (R)(vn + (double)(depend * gv));
+    sQ[2][t.cslot] = left_state_pos<R>(p, depend, g5);
+    R a1 = (R)(b1 + (double)(depend * gA)), a2 = (R)(b2 + (double)(depend * gB));
+    sQ[3][t.cslot] = a1 * a1 + a2 * a2;
+    sQ[4][t.cslot] = right_state_pos<R>(rho, depend, g1);
+    sQ[5][t.cslot] = (R)(vn - (double)(depend * gv));
+    sQ[6][t.cslot] = right_state_pos<R>(p, depend, g5);
+    a1 = (R)(b1 - (double)(depend * gA));
+    a2 = (R)(b2 - (double)(depend * gB));
+    sQ[7][t.cslot] = a1 * a1 + a2 * a2;
+  }
+  __syncthreads();
+  if (t.f_ok) {
     const int q = t.q;
     const int c = ix3s(g, t.i, t.j, t.k), cp = c + s;
-    const double *U1 = A.U[0], *UVN = A.U[A.VN], *U5 = A.U[4];
-    const double *B1 = A.bcc[A.BT1], *B2 = A.bcc[A.BT2];
-    double dxm = dxe[q], dx0 = dxe[q + 1], dxp = dxe[q + 2], dxpp = dxe[q + 3];
-    double T0 = tswitch(U5[c], U1[c]), T1 = tswitch(U5[cp], U1[cp]);
-    double g1a = vlim_pick(vlim_parts(U1[c - s], U1[c], U1[cp], dxm, dx0, dxp), T0);
-    double g1b = vlim_pick(vlim_parts(U1[c], U1[cp], U1[cp + s], dx0, dxp, dxpp), T1);
-    double gva = vlim_pick(vlim_parts(UVN[c - s], UVN[c], UVN[cp], dxm, dx0, dxp), T0);
-    double gvb = vlim_pick(vlim_parts(UVN[c], UVN[cp], UVN[cp + s], dx0, dxp, dxpp), T1);
-    double g5a = vlim_pick(vlim_parts(U5[c - s], U5[c], U5[cp], dxm, dx0, dxp), T0);
-    double g5b = vlim_pick(vlim_parts(U5[c], U5[cp], U5[cp + s], dx0, dxp, dxpp), T1);
-    double gAa = vlim_pick(vlim_parts(B1[c - s], B1[c], B1[cp], dxm, dx0, dxp), T0);
-    double gAb = vlim_pick(vlim_parts(B1[c], B1[cp], B1[cp + s], dx0, dxp, dxpp), T1);
-    double gBa = vlim_pick(vlim_parts(B2[c - s], B2[c], B2[cp], dxm, dx0, dxp), T0);
-    double gBb = vlim_pick(vlim_parts(B2[c], B2[cp], B2[cp + s], dx0, dxp, dxpp), T1);
     const double *const *sp = A.sp;
-    double gamma = gam_of_s(sp, c);
-    double cm = sqrt((gamma * U5[c] + B1[c] * B1[c] + B2[c] * B2[c]) / U1[c]);
-    double depend1 = 0.5 * (dx0 - cm * dt) * A.facdep;
-    depend1 = dmax(0.0, depend1);
-    gamma = gam_of_s(sp, cp);
-    cm = sqrt((gamma * U5[cp] + B1[cp] * B1[cp] + B2[cp] * B2[cp]) / U1[cp]);
-    double depend2 = 0.5 * (dxp - cm * dt) * A.facdep;
-    depend2 = dmax(0.0, depend2);
-    double QL1 = posfix(U1[c] + depend1 * g1a, U1[c]);
-    double QR1 = posfix(U1[cp] - depend2 * g1b, U1[cp]);
-    double QL2 = UVN[c] + depend1 * gva;
-    double QR2 = UVN[cp] - depend2 * gvb;
-    double QL3 = posfix(U5[c] + depend1 * g5a, U5[c]);
-    double QR3 = posfix(U5[cp] - depend2 * g5b, U5[cp]);
-    double a1 = B1[c] + depend1 * gAa, a2 = B2[c] + depend1 * gBa;
-    double QL4 = a1 * a1 + a2 * a2;
-    double b1 = B1[cp] - depend2 * gAb, b2 = B2[cp] - depend2 * gBb;
-    double QR4 = b1 * b1 + b2 * b2;
-    double ndHm = 0.5 * (sp[0][c] + sp[0][cp]);
-    double ndpm = 0.5 * (sp[1][c] + sp[1][cp]);
-    double ndHem = 0.5 * (sp[3][c] + sp[3][cp]);
-    double ndHepm = 0.5 * (sp[4][c] + sp[4][cp]);
-    double ndH2m = 0.5 * (sp[2][c] + sp[2][cp]);
-    double gm = (5.0 * (ndHm + ndpm + ndHem + ndHepm) + 7.0 * ndH2m) / (3.0 * (ndHm + ndpm + ndHem + ndHepm) + 5.0 * ndH2m);
-    double Pa, Va;
-    rieman(QL1, QL2, QL3, QL4, QR1, QR2, QR3, QR4, gm, Pa, Va);
+    // interface gamma from the mean species (M3:1440-1446)
+    R gm;
+    if constexpr (Ar<R>::strict) {
+      double ndHm = 0.5 * (sp[0][c] + sp[0][cp]);
+      double ndpm = 0.5 * (sp[1][c] + sp[1][cp]);
+      double ndHem = 0.5 * (sp[3][c] + sp[3][cp]);
+      double ndHepm = 0.5 * (sp[4][c] + sp[4][cp]);
+      double ndH2m = 0.5 * (sp[2][c] + sp[2][cp]);
+      gm = (5.0 * (ndHm + ndpm + ndHem + ndHepm) + 7.0 * ndH2m) / (3.0 * (ndHm + ndpm + ndHem + ndHepm) + 5.0 * ndH2m);
+    } else {
+      float s4 = (float)(sp[0][c] + sp[0][cp] + sp[1][c] + sp[1][cp] + sp[3][c] + sp[3][cp] + sp[4][c] + sp[4][cp]);
+      float h2 = (float)(sp[2][c] + sp[2][cp]);
+      gm = Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
+    }
+    const int a = t.cslot, b = t.rslot_c;
+    R Pa, Va;
+    rieman_r<R>(sQ[0][a], sQ[1][a], sQ[2][a], sQ[3][a], sQ[4][b], sQ[5][b], sQ[6][b], sQ[7][b], gm, Pa, Va);
     sPa[t.fslot] = Pa;
     sVa[t.fslot] = Va;
     if (t.own) {
-      A.va[c] = Va;
-      A.xlag[c] = xe[q + 1] + Va * dt;
+      A.va[c] = (double)Va;
+      A.xlag[c] = xe[q + 1] + (double)Va * dt;
     }
   }
   __syncthreads();
-  if (!t.cell_ok) return;
+  if (!t.u_ok) return;
+  // Lagrangian update of the cell (M3:1448-1466)
   const int q = t.q;
   const int c = ix3s(g, t.i, t.j, t.k);
-  double rho = A.U[0][c];
-  double gi = gammi1_of_s(A.sp, c);
-  double vx = A.U[1][c], vy = A.U[2][c], vz = A.U[3][c];
-  double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
-  double e = A.U[4][c] / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
-  double dxq = dxe[q + 1];
-  double dm = dxq * rho;
-  double pa = sPa[t.rslot], pam = sPa[t.lslot], va = sVa[t.rslot], vam = sVa[t.lslot];
-  double xl = xe[q + 1] + va * dt, xlm = xe[q] + vam * dt;
-  A.Uo[0][c] = dm / (xl - xlm);
-  A.Uo[A.VN][c] = A.U[A.VN][c] - dt / dm * (pa - pam);
-  A.u5e[c] = e - dt / dm * (pa * va - pam * vam);
-  A.blgo[0][c] = A.bcc[A.BT1][c] * dxq / (xl - xlm);
-  A.blgo[1][c] = A.bcc[A.BT2][c] * dxq / (xl - xlm);
+  const double rho = U1[c];
+  const double vx = A.U[1][c], vy = A.U[2][c], vz = A.U[3][c];
+  const double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
+  const double dxq = dxe[q + 1];
+  const double dm = dxq * rho;
+  const double pa = (double)sPa[t.rslot], pam = (double)sPa[t.lslot];
+  const double va = (double)sVa[t.rslot], vam = (double)sVa[t.lslot];
+  const double xl = xe[q + 1] + va * dt, xlm = xe[q] + vam * dt;
+  const double dxl = xl - xlm;
+  if constexpr (Ar<R>::strict) {
+    double gi = gammi1_of_s(A.sp, c);
+    double e = A.U[4][c] / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
+    A.Uo[0][c] = dm / dxl;
+    A.Uo[A.VN][c] = A.U[A.VN][c] - dt / dm * (pa - pam);
+    A.u5e[c] = e - dt / dm * (pa * va - pam * vam);
+    A.blgo[0][c] = B1[c] * dxq / dxl;
+    A.blgo[1][c] = B2[c] * dxq / dxl;
 #pragma unroll
-  for (int f = 0; f < 8; ++f) A.spo[f][c] = dxq * A.sp[f][c] / (xl - xlm);
-  A.dxlagM[c] = xl - xlm;
+    for (int f = 0; f < 8; ++f) A.spo[f][c] = dxq * A.sp[f][c] / dxl;
+  } else {
+    const double *const *sp = A.sp;
+    double s4 = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c];
+    double irho = 1.0 / rho;
+    double e = A.U[4][c] * (3.0 * s4 + 5.0 * sp[2][c]) / (2.0 * s4 + 2.0 * sp[2][c]) * irho +
+               0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) * irho;
+    double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxq * idl;
+    A.Uo[0][c] = dm * idl;
+    A.Uo[A.VN][c] = A.U[A.VN][c] - dtdm * (pa - pam);
+    A.u5e[c] = e - dtdm * (pa * va - pam * vam);
+    A.blgo[0][c] = B1[c] * cmp;
+    A.blgo[1][c] = B2[c] * cmp;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) A.spo[f][c] = sp[f][c] * cmp;
+  }
+  A.dxlagM[c] = dxl;
 }
 
 // ---------------------------------------------------------------------------
 // MOC
 // ---------------------------------------------------------------------------
-struct MocV {
-  double Bay, Vay, Baz, Vaz;
+template <class R> struct MocV {
+  R Bay, Vay, Baz, Vaz;
 };
 
-__device__ __forceinline__ MocV moc_iface2(const SweepIO &A, int c, int cp, int q, const double *dxl,
-                                          const double *gi, const double *gp, double dt) {
+// Slopes of rho, v_t1, v_t2, Blg1, Blg2 at cell c with the limiter choice mm.
+template <class R>
+__device__ __forceinline__ void moc_slopes(const SweepIO &A, int c, int s, bool mm, R *gs) {
+  const double *Q[5] = {A.U[0], A.U[A.VT1], A.U[A.VT2], A.blg[0], A.blg[1]};
+  const R dxm = (R)A.dxlagM[c - s], dx0 = (R)A.dxlagM[c], dxp = (R)A.dxlagM[c + s];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) gs[v] = vslope<R>(Q[v][c - s], Q[v][c], Q[v][c + s], dxm, dx0, dxp, mm);
+}
+
+// One characteristic interface (M3:2000-2051) between c and cp = c + s, from
+// the slopes gi (cell c) and gp (cell cp).
+template <class R>
+__device__ __forceinline__ MocV<R> moc_iface(const SweepIO &A, int c, int cp, int q, const R *gi, const R *gp,
+                                             double dt) {
   const double *U1 = A.U[0], *UT1 = A.U[A.VT1], *UT2 = A.U[A.VT2], *UBn = A.U[A.BN];
   const double *Bl1 = A.blg[0], *Bl2 = A.blg[1];
-  const double fac = A.facdep;
   const double *xe = A.xe;
-  int cpp = cp + (cp - c);
-  double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
-  double deni = sqrt((dxl[1] + dxl[2]) / (dxl[1] * U1[cp] + dxl[2] * U1[c]));
-  double depend1, depend2, ca;
-  double QL11, QL12, QL13, QR11, QR12, QR13, QL21, QL22, QR21, QR22;
-  if (Bnap > 0.0) {
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[1], ca);
-    depend1 = (0.5 * dxl[1] - ca) * fac;
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[2], ca);
-    depend2 = (0.5 * dxl[2] - ca) * fac;
-    QL11 = UT1[c] + depend1 * gi[1];
-    QR11 = UT1[cp] - depend2 * gp[1];
-    QL12 = Bl1[c] + depend1 * gi[3];
-    QR12 = Bl1[cp] - depend2 * gp[3];
-    QL13 = posfix(U1[c] + depend1 * gi[0], U1[c]);
-    QR13 = posfix(U1[cp] - depend2 * gp[0], U1[cp]);
-    QL21 = UT2[c] + depend1 * gi[2];
-    QR21 = UT2[cp] - depend2 * gp[2];
-    QL22 = Bl2[c] + depend1 * gi[4];
-    QR22 = Bl2[cp] - depend2 * gp[4];
+  const int cpp = cp + (cp - c);
+  const double dl1 = A.dxlagM[c], dl2 = A.dxlagM[cp];
+  // B_n at the Lagrangian interface position (M3:2008)
+  const double Bnap_d = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
+  R deni;
+  if constexpr (Ar<R>::strict) {
+    deni = sqrt((dl1 + dl2) / (dl1 * U1[cp] + dl2 * U1[c]));
   } else {
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[2], ca);
-    depend1 = (0.5 * dxl[2] - ca) * fac;
-    ca = fabs(Bnap * deni) * 0.5 * dt;
-    ca = dmin(dxl[1], ca);
-    depend2 = (0.5 * dxl[1] - ca) * fac;
-    QL11 = UT1[cp] - depend1 * gp[1];
-    QR11 = UT1[c] + depend2 * gi[1];
-    QL12 = Bl1[cp] - depend1 * gp[3];
-    QR12 = Bl1[c] + depend2 * gi[3];
-    QL13 = posfix(U1[cp] - depend1 * gp[0], U1[cp]);
-    QR13 = posfix(U1[c] + depend2 * gi[0], U1[c]);
-    QL21 = UT2[cp] - depend1 * gp[2];
-    QR21 = UT2[c] + depend2 * gi[2];
-    QL22 = Bl2[cp] - depend1 * gp[4];
-    QR22 = Bl2[c] + depend2 * gi[4];
+    deni = Ar<float>::sqrt_(Ar<float>::div((float)(dl1 + dl2), (float)(dl1 * U1[cp] + dl2 * U1[c])));
   }
-  MocV o;
-  chareq(QL11, QL12, QL13, QR11, QR12, QR13, o.Bay, o.Vay);
-  chareq(QL21, QL22, QL13, QR21, QR22, QR13, o.Baz, o.Vaz);
+  const R Bnap = (R)Bnap_d;
+  const R fac = (R)A.facdep;
+  const R rdt = (R)dt;
+  R depend1, depend2, ca;
+  R QL11, QL12, QL13, QR11, QR12, QR13, QL21, QL22, QR21, QR22;
+  if (Bnap > (R)0.0) {
+    ca = fabs(Bnap * deni) * (R)0.5 * rdt;
+    ca = rmin((R)dl1, ca);
+    depend1 = ((R)0.5 * (R)dl1 - ca) * fac;
+    ca = fabs(Bnap * deni) * (R)0.5 * rdt;
+    ca = rmin((R)dl2, ca);
+    depend2 = ((R)0.5 * (R)dl2 - ca) * fac;
+    QL11 = (R)(UT1[c] + (double)(depend1 * gi[1]));
+    QR11 = (R)(UT1[cp] - (double)(depend2 * gp[1]));
+    QL12 = (R)(Bl1[c] + (double)(depend1 * gi[3]));
+    QR12 = (R)(Bl1[cp] - (double)(depend2 * gp[3]));
+    QL13 = left_state_pos<R>(U1[c], depend1, gi[0]);
+    QR13 = right_state_pos<R>(U1[cp], depend2, gp[0]);
+    QL21 = (R)(UT2[c] + (double)(depend1 * gi[2]));
+    QR21 = (R)(UT2[cp] - (double)(depend2 * gp[2]));
+    QL22 = (R)(Bl2[c] + (double)(depend1 * gi[4]));
+    QR22 = (R)(Bl2[cp] - (double)(depend2 * gp[4]));
+  } else {
+    ca = fabs(Bnap * deni) * (R)0.5 * rdt;
+    ca = rmin((R)dl2, ca);
+    depend1 = ((R)0.5 * (R)dl2 - ca) * fac;
+    ca = fabs(Bnap * deni) * (R)0.5 * rdt;
+    ca = rmin((R)dl1, ca);
+    depend2 = ((R)0.5 * (R)dl1 - ca) * fac;
+    QL11 = (R)(UT1[cp] - (double)(depend1 * gp[1]));
+    QR11 = (R)(UT1[c] + (double)(depend2 * gi[1]));
+    QL12 = (R)(Bl1[cp] - (double)(depend1 * gp[3]));
+    QR12 = (R)(Bl1[c] + (double)(depend2 * gi[3]));
+    QL13 = right_state_pos<R>(U1[cp], depend1, gp[0]);
+    QR13 = left_state_pos<R>(U1[c], depend2, gi[0]);
+    QL21 = (R)(UT2[cp] - (double)(depend1 * gp[2]));
+    QR21 = (R)(UT2[c] + (double)(depend2 * gi[2]));
+    QL22 = (R)(Bl2[cp] - (double)(depend1 * gp[4]));
+    QR22 = (R)(Bl2[c] + (double)(depend2 * gi[4]));
+  }
+  MocV<R> o;
+  chareq_r<R>(QL11, QL12, QL13, QR11, QR12, QR13, o.Bay, o.Vay);
+  chareq_r<R>(QL21, QL22, QL13, QR21, QR22, QR13, o.Baz, o.Vaz);
   return o;
 }
 
-// At a block edge of the reproduced MPI run the interface is evaluated twice
-// (left-block view: the right cell's limiter switch reads the pre-sweep
-// pressure of its ghost copy; right-block view: the left cell's does).
-template <int D>
-__global__ __launch_bounds__(576) void k_moc(SweepIO A, Geo g, double dt) {
-  __shared__ double sL[4][NSLOT], sR[4][NSLOT];
-  const Tile t = tile_map<D>(g);
+// The limiter switch of a cell in MOC uses the specific total energy e left
+// by RHS for interior cells and the pre-sweep pressure for ghost cells.
+// An interface at a block edge of the reproduced MPI run (position a
+// multiple of the block length) is evaluated twice: in the left block's
+// view the right cell is that block's ghost copy (pressure switch), in the
+// right block's view the left cell is.  The ghost-view slopes differ from
+// the interior ones only when the two switch values fall on opposite sides
+// of 3; they are recomputed for that case.
+template <int D, class R>
+__global__ __launch_bounds__(D == 0 ? 576 : 640) void k_moc(SweepIO A, Geo g, double dt) {
+  constexpr int NC = ncell_slots<D>(), NF = nface_slots<D>();
+  constexpr int NSH = (5 * NC > 8 * NF) ? 5 * NC : 8 * NF;
+  __shared__ R sh[NSH];
+  R(*sG)[NC] = reinterpret_cast<R(*)[NC]>(sh); // phases 1-2: slopes per cell
+  R(*sI)[NF] = reinterpret_cast<R(*)[NF]>(sh); // phases 2-3: left view 0..3, right view 4..7
+  const Roles t = roles<D>(g);
   const int s = g.st[D];
   const int n = g.n[D];
-  if (t.face_ok) {
+  const double *U1 = A.U[0];
+  if (t.c_ok) {
     const int q = t.q;
-    const int c = ix3s(g, t.i, t.j, t.k), cp = c + s;
-    double dxl[4] = {A.dxlagM[c - s], A.dxlagM[c], A.dxlagM[cp], A.dxlagM[cp + s]};
-    const double *U1 = A.U[0];
-    const double *Q[5] = {A.U[0], A.U[A.VT1], A.U[A.VT2], A.blg[0], A.blg[1]};
-    Slope2 sa[5], sb[5];
+    const int c = ix3s(g, t.i, t.j, t.k);
+    const bool inner = q >= 1 && q <= n;
+    const bool mm = use_minmod_r<R>(inner ? A.u5e_in[c] : A.U[4][c], U1[c]);
+    R gs[5];
+    moc_slopes<R>(A, c, s, mm, gs);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      sa[v] = vlim_parts(Q[v][c - s], Q[v][c], Q[v][cp], dxl[0], dxl[1], dxl[2]);
-      sb[v] = vlim_parts(Q[v][c], Q[v][cp], Q[v][cp + s], dxl[1], dxl[2], dxl[3]);
-    }
-    double Tp0 = tswitch(A.U[4][c], U1[c]), Tp1 = tswitch(A.U[4][cp], U1[cp]);
-    double Te0 = (q >= 1) ? tswitch(A.u5e_in[c], U1[c]) : Tp0;
-    double Te1 = (q + 1 <= n) ? tswitch(A.u5e_in[cp], U1[cp]) : Tp1;
-    bool edge = (q % A.nblk) == 0;
-    double gi[5], gp[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      gi[v] = vlim_pick(sa[v], Te0);
-      gp[v] = vlim_pick(sb[v], edge ? Tp1 : Te1);
-    }
-    MocV L = moc_iface2(A, c, cp, q, dxl, gi, gp, dt);
-    MocV R = L;
-    if (edge) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        gi[v] = vlim_pick(sa[v], Tp0);
-        gp[v] = vlim_pick(sb[v], Te1);
-      }
-      R = moc_iface2(A, c, cp, q, dxl, gi, gp, dt);
-    }
-    sL[0][t.fslot] = L.Bay; sL[1][t.fslot] = L.Vay; sL[2][t.fslot] = L.Baz; sL[3][t.fslot] = L.Vaz;
-    sR[0][t.fslot] = R.Bay; sR[1][t.fslot] = R.Vay; sR[2][t.fslot] = R.Baz; sR[3][t.fslot] = R.Vaz;
+    for (int v = 0; v < 5; ++v) sG[v][t.cslot] = gs[v];
   }
   __syncthreads();
-  if (!t.cell_ok) return;
+  MocV<R> L{}, Rv{};
+  if (t.f_ok) {
+    const int q = t.q;
+    const int c = ix3s(g, t.i, t.j, t.k), cp = c + s;
+    const int a = t.cslot, b = t.rslot_c;
+    R gi[5], gp[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { gi[v] = sG[v][a]; gp[v] = sG[v][b]; }
+    if ((q % A.nblk) != 0) {
+      L = moc_iface<R>(A, c, cp, q, gi, gp, dt);
+      Rv = L;
+    } else {
+      // left cell q (interior unless q = 0), right cell q+1 (interior unless q = n)
+      const bool mm0p = use_minmod_r<R>(A.U[4][c], U1[c]);
+      const bool mm0e = q >= 1 ? use_minmod_r<R>(A.u5e_in[c], U1[c]) : mm0p;
+      const bool mm1p = use_minmod_r<R>(A.U[4][cp], U1[cp]);
+      const bool mm1e = q + 1 <= n ? use_minmod_r<R>(A.u5e_in[cp], U1[cp]) : mm1p;
+      R ga[5];
+      // left-block view: cell q interior, cell q+1 its ghost copy
+      if (mm1p != mm1e) {
+        moc_slopes<R>(A, cp, s, mm1p, ga);
+        L = moc_iface<R>(A, c, cp, q, gi, ga, dt);
+      } else {
+        L = moc_iface<R>(A, c, cp, q, gi, gp, dt);
+      }
+      // right-block view: cell q its ghost copy, cell q+1 interior
+      if (mm0p != mm0e) {
+        moc_slopes<R>(A, c, s, mm0p, ga);
+        Rv = moc_iface<R>(A, c, cp, q, ga, gp, dt);
+      } else if (mm1p != mm1e) {
+        Rv = moc_iface<R>(A, c, cp, q, gi, gp, dt);
+      } else {
+        Rv = L;
+      }
+    }
+  }
+  __syncthreads();
+  if (t.f_ok) {
+    sI[0][t.fslot] = L.Bay; sI[1][t.fslot] = L.Vay; sI[2][t.fslot] = L.Baz; sI[3][t.fslot] = L.Vaz;
+    sI[4][t.fslot] = Rv.Bay; sI[5][t.fslot] = Rv.Vay; sI[6][t.fslot] = Rv.Baz; sI[7][t.fslot] = Rv.Vaz;
+  }
+  __syncthreads();
+  if (!t.u_ok) return;
+  // cell update (M3:2053-2095): left view of the right interface, right
+  // view of the left interface
   const int q = t.q;
   const int c = ix3s(g, t.i, t.j, t.k), cm = c - s, cp = c + s, cpp = cp + s;
   const double *UBn = A.U[A.BN];
   const double *xe = A.xe;
-  double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
-  double Bnam = UBn[c] + (A.xlag[cm] - xe[q]) * (UBn[cp] - UBn[cm]) / (xe[q + 1] - xe[q - 1]);
-  double Vnap = A.va[c], Vnam = A.va[cm];
-  double dxl = A.dxlagM[c];
-  double rho = A.U[0][c];
-  double BayL = sL[0][t.rslot], VayL = sL[1][t.rslot], BazL = sL[2][t.rslot], VazL = sL[3][t.rslot];
-  double BayR = sR[0][t.lslot], VayR = sR[1][t.lslot], BazR = sR[2][t.lslot], VazR = sR[3][t.lslot];
-  A.Uo[A.VN][c] = A.U[A.VN][c] * rho + dt / dxl * 0.5 * (Bnap * Bnap - Bnam * Bnam);
-  A.Uo[A.VT1][c] = A.U[A.VT1][c] * rho + dt / dxl * (Bnap * BayL - Bnam * BayR);
-  A.Uo[A.VT2][c] = A.U[A.VT2][c] * rho + dt / dxl * (Bnap * BazL - Bnam * BazR);
-  A.Uo[4][c] = A.u5e_in[c] * rho + dt / dxl *
-                                      ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
-                                       (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
-  A.blgo[0][c] = A.blg[0][c] + dt / dxl * (Bnap * VayL - Bnam * VayR);
-  A.blgo[1][c] = A.blg[1][c] + dt / dxl * (Bnap * VazL - Bnam * VazR);
+  const double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
+  const double Bnam = UBn[c] + (A.xlag[cm] - xe[q]) * (UBn[cp] - UBn[cm]) / (xe[q + 1] - xe[q - 1]);
+  const double Vnap = A.va[c], Vnam = A.va[cm];
+  const double dxl = A.dxlagM[c];
+  const double rho = U1[c];
+  const double BayL = (double)sI[0][t.rslot], VayL = (double)sI[1][t.rslot];
+  const double BazL = (double)sI[2][t.rslot], VazL = (double)sI[3][t.rslot];
+  const double BayR = (double)sI[4][t.lslot], VayR = (double)sI[5][t.lslot];
+  const double BazR = (double)sI[6][t.lslot], VazR = (double)sI[7][t.lslot];
+  if constexpr (Ar<R>::strict) {
+    A.Uo[A.VN][c] = A.U[A.VN][c] * rho + dt / dxl * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+    A.Uo[A.VT1][c] = A.U[A.VT1][c] * rho + dt / dxl * (Bnap * BayL - Bnam * BayR);
+    A.Uo[A.VT2][c] = A.U[A.VT2][c] * rho + dt / dxl * (Bnap * BazL - Bnam * BazR);
+    A.Uo[4][c] = A.u5e_in[c] * rho + dt / dxl *
+                                        ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                         (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+    A.blgo[0][c] = A.blg[0][c] + dt / dxl * (Bnap * VayL - Bnam * VayR);
+    A.blgo[1][c] = A.blg[1][c] + dt / dxl * (Bnap * VazL - Bnam * VazR);
+  } else {
+    const double f = dt / dxl;
+    A.Uo[A.VN][c] = A.U[A.VN][c] * rho + f * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+    A.Uo[A.VT1][c] = A.U[A.VT1][c] * rho + f * (Bnap * BayL - Bnam * BayR);
+    A.Uo[A.VT2][c] = A.U[A.VT2][c] * rho + f * (Bnap * BazL - Bnam * BazR);
+    A.Uo[4][c] = A.u5e_in[c] * rho + f * ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                          (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+    A.blgo[0][c] = A.blg[0][c] + f * (Bnap * VayL - Bnam * VayR);
+    A.blgo[1][c] = A.blg[1][c] + f * (Bnap * VazL - Bnam * VazR);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // REMAP + primitives
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, class R>
 __global__ __launch_bounds__(576) void k_remap(SweepIO A, Geo g, double dt) {
-  __shared__ double sF[15][NSLOT];
-  const Tile t = tile_map<D>(g);
+  constexpr int NF = nface_slots<D>();
+  __shared__ double sF[15][NF];
+  const Roles t = roles<D>(g);
   const int s = g.st[D];
-  if (t.face_ok) {
+  if (t.f_ok) {
+    // upwind linear flux through the face swept by V*dt (M3:1600-1640)
     const int c = ix3s(g, t.i, t.j, t.k);
-    double vadt = A.va[c] * dt;
+    const double vadt = A.va[c] * dt;
     int u;
-    double depend, sgn;
+    double depend;
+    bool up;
     if (vadt <= 0.0) {
       u = c + s;
       depend = 0.5 * (A.dxlagM[u] + vadt) * A.facdep;
-      sgn = -1.0;
+      up = false;
     } else {
       u = c;
       depend = 0.5 * (A.dxlagM[u] - vadt) * A.facdep;
-      sgn = 1.0;
+      up = true;
     }
-    double dxm = A.dxlagM[u - s], dx0 = A.dxlagM[u], dxp = A.dxlagM[u + s];
-    double T = tswitch(A.U[4][u], A.U[0][u]);
+    const R dxm = (R)A.dxlagM[u - s], dx0 = (R)A.dxlagM[u], dxp = (R)A.dxlagM[u + s];
+    const bool mm = use_minmod_r<R>(A.U[4][u], A.U[0][u]);
+    const R rdep = (R)depend;
 #pragma unroll
     for (int v = 0; v < 15; ++v) {
       const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-      double gr = vlim_pick(vlim_parts(Qv[u - s], Qv[u], Qv[u + s], dxm, dx0, dxp), T);
-      double face = sgn > 0.0 ? Qv[u] + depend * gr : Qv[u] - depend * gr;
+      const R gr = vslope<R>(Qv[u - s], Qv[u], Qv[u + s], dxm, dx0, dxp, mm);
+      double face;
+      if constexpr (Ar<R>::strict) face = up ? Qv[u] + depend * gr : Qv[u] - depend * gr;
+      else face = up ? Qv[u] + (double)(rdep * gr) : Qv[u] - (double)(rdep * gr);
       sF[v][t.fslot] = -(vadt * face);
     }
   }
   __syncthreads();
-  if (!t.cell_ok) return;
+  if (!t.u_ok) return;
+  // remapped conserved state (M3:1645-1686) and primitives (M3:1291-1321)
   const int q = t.q;
   const int c = ix3s(g, t.i, t.j, t.k);
-  double dxl = A.dxlagM[c];
-  double dxE = A.xe[q + 1] - A.xe[q];
+  const double dxl = A.dxlagM[c];
+  const double dxE = A.xe[q + 1] - A.xe[q];
   double Qn[15];
+  if constexpr (Ar<R>::strict) {
 #pragma unroll
-  for (int v = 0; v < 15; ++v) {
-    const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-    Qn[v] = (Qv[c] * dxl + sF[v][t.rslot] - sF[v][t.lslot]) / dxE;
+    for (int v = 0; v < 15; ++v) {
+      const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
+      Qn[v] = (Qv[c] * dxl + sF[v][t.rslot] - sF[v][t.lslot]) / dxE;
+    }
+  } else {
+    const double idxE = 1.0 / dxE;
+#pragma unroll
+    for (int v = 0; v < 15; ++v) {
+      const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
+      Qn[v] = (Qv[c] * dxl + sF[v][t.rslot] - sF[v][t.lslot]) * idxE;
+    }
   }
   double rho = dmax(1.27 * A.rmin, Qn[0]);
   double invd = 1.0 / rho;
@@ -375,14 +549,16 @@ __global__ __launch_bounds__(576) void k_remap(SweepIO A, Geo g, double dt) {
   for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
 }
 
-template __global__ void k_rhs<0>(SweepIO, Geo, double);
-template __global__ void k_rhs<1>(SweepIO, Geo, double);
-template __global__ void k_rhs<2>(SweepIO, Geo, double);
-template __global__ void k_moc<0>(SweepIO, Geo, double);
-template __global__ void k_moc<1>(SweepIO, Geo, double);
-template __global__ void k_moc<2>(SweepIO, Geo, double);
-template __global__ void k_remap<0>(SweepIO, Geo, double);
-template __global__ void k_remap<1>(SweepIO, Geo, double);
-template __global__ void k_remap<2>(SweepIO, Geo, double);
+#define MCF_INST(D, R)                                                                                  \
+  template __global__ void k_rhs<D, R>(SweepIO, Geo, double);                                          \
+  template __global__ void k_moc<D, R>(SweepIO, Geo, double);                                          \
+  template __global__ void k_remap<D, R>(SweepIO, Geo, double);
+MCF_INST(0, double)
+MCF_INST(1, double)
+MCF_INST(2, double)
+MCF_INST(0, float)
+MCF_INST(1, float)
+MCF_INST(2, float)
+#undef MCF_INST
 
 } // namespace mcf
