@@ -12,6 +12,7 @@
 
 #include "../../include/mcf_abi.h"
 #include "mcf_kernels.h"
+#include "mcf_mlp.h"
 
 using namespace mcf;
 
@@ -60,6 +61,13 @@ struct mcf_ctx {
   // per-kernel event timing (mcf_profile)
   bool prof = false;
   bool mixed = false; // MCF_PREC_MIXED: FP32 reconstruction / Riemann / EMF
+  // cooling surrogate: FP32 weights, Adam moments, gradient (4 x MLP_NP),
+  // packed BF16 operands, counters {backward-Euler cells, IMC fallbacks}
+  float *mlp = nullptr;
+  __bf16 *mlp_pk = nullptr;
+  unsigned long long *cool_cnt = nullptr;
+  bool mlp_ready = false;
+  long adam_t = 0;
   struct Rec { const char *name; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
@@ -450,16 +458,38 @@ static int do_source(mcf_ctx *x, double dt) {
   A.Tn = x->Tn; A.Pn = x->Pn;
   for (int d = 0; d < 3; ++d) A.gdx[d] = x->gdx[d];
   A.pmin = x->pmin; A.pmax = x->pmax;
+  if (x->cfg.cooling == MCF_COOL_SURROGATE) {
+    if (!x->mlp_ready) { g_err = "cooling surrogate selected but no weights loaded (mcf_mlp_set_weights / mcf_mlp_train)"; return MCF_E_STATE; }
+    A.sur = 1;
+    A.pk = x->mlp_pk;
+    A.mw = x->mlp;
+    A.cnt = x->cool_cnt;
+    A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2);
+    A.queue = reinterpret_cast<int *>(x->dq[1]);
+    A.qdc = x->dq[0];
+  }
   int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
   int l15[2] = {1, 5}, l5[1] = {5};
   CHK(fillU(x, 7, 1, 1, l15, 2));
   CHK(fillOT(x, 7, 1, OT_SPECIES));
   Box bt = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1), bc = mkbox(1, nx, 1, ny, 1, nz);
   KLAUNCH(k_src_temp, boxsize(bt), 8, A, x->g, bt, 1);
-  KLAUNCH(k_src_cell, boxsize(bc), 10, A, x->g, bc, dt, 0);
+  if (A.sur) {
+    KLAUNCH_G(k_zero_u32, dim3(1), dim3(64), 0, A.qn);
+    KLAUNCH_R(k_src_cell, true, boxsize(bc), 10, A, x->g, bc, dt, 0);
+    KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 0);
+  } else {
+    KLAUNCH_R(k_src_cell, false, boxsize(bc), 10, A, x->g, bc, dt, 0);
+  }
   CHK(fillU(x, 7, 1, 1, l5, 1));
   KLAUNCH(k_src_temp, boxsize(bt), 7, A, x->g, bt, 0);
-  KLAUNCH(k_src_cell, boxsize(bc), 10, A, x->g, bc, dt, 1);
+  if (A.sur) {
+    KLAUNCH_G(k_zero_u32, dim3(1), dim3(64), 0, A.qn);
+    KLAUNCH_R(k_src_cell, true, boxsize(bc), 10, A, x->g, bc, dt, 1);
+    KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 1);
+  } else {
+    KLAUNCH_R(k_src_cell, false, boxsize(bc), 10, A, x->g, bc, dt, 1);
+  }
   return MCF_OK;
 }
 
@@ -495,6 +525,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   // DISSIP works on 10-cell layers at the x ends (M3:3152-3170)
   if (n[0] / nsp[0] < 10) { g_err = "x block must hold at least 10 cells (DISSIP layers)"; return MCF_E_ARG; }
   if (cfg->precision != MCF_PREC_FP64 && cfg->precision != MCF_PREC_MIXED) { g_err = "unknown precision mode"; return MCF_E_ARG; }
+  if (cfg->cooling != MCF_COOL_KI && cfg->cooling != MCF_COOL_SURROGATE) { g_err = "unknown cooling mode"; return MCF_E_ARG; }
   long long vol = (long long)(n[0] + 4) * (n[1] + 4) * (n[2] + 4);
   if (vol >= (1LL << 31)) { g_err = "grid too large for 32-bit indexing"; return MCF_E_ARG; }
   mcf_ctx *x = new mcf_ctx();
@@ -539,6 +570,11 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
     HIPCHK(hipMemcpyAsync(x->gdx[d], x->hdx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
   }
   HIPCHK(hipHostMalloc((void **)&x->hred, 2 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipMalloc(&x->mlp, 4 * MLP_NP * sizeof(float)));
+  HIPCHK(hipMalloc(&x->mlp_pk, MLP_NPK * sizeof(__bf16)));
+  HIPCHK(hipMalloc(&x->cool_cnt, 4 * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(x->mlp, 0, 4 * MLP_NP * sizeof(float), x->stream));
+  HIPCHK(hipMemsetAsync(x->cool_cnt, 0, 4 * sizeof(unsigned long long), x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   *out = x;
   return MCF_OK;
@@ -550,6 +586,9 @@ extern "C" int mcf_finalize(mcf_ctx *x) {
   for (auto &r : x->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : x->evpool) (void)hipEventDestroy(e);
   if (x->pool) (void)hipFree(x->pool);
+  if (x->mlp) (void)hipFree(x->mlp);
+  if (x->mlp_pk) (void)hipFree(x->mlp_pk);
+  if (x->cool_cnt) (void)hipFree(x->cool_cnt);
   if (x->hred) (void)hipHostFree(x->hred);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
@@ -740,5 +779,208 @@ extern "C" int mcf_profile_report(mcf_ctx *x, mcf_kstat *out, int maxn, int *n) 
     ++i;
   }
   *n = i;
+  return MCF_OK;
+}
+
+// ----------------------------------------------------------------------------
+// KI-cooling surrogate
+// ----------------------------------------------------------------------------
+static int mlp_pack(mcf_ctx *x) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_NPK + 255) / 256), dim3(256), 0, x->stream, x->mlp, x->mlp_pk);
+  HIPCHK(hipGetLastError());
+  return MCF_OK;
+}
+
+extern "C" int mcf_mlp_set_weights(mcf_ctx *x, const float *w) {
+  if (!x || !w) { g_err = "null argument"; return MCF_E_ARG; }
+  for (int e = 0; e < MLP_NP; ++e)
+    if (!std::isfinite(w[e])) { g_err = "non-finite weight"; return MCF_E_ARG; }
+  HIPCHK(hipMemcpyAsync(x->mlp, w, MLP_NP * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  hipLaunchKernelGGL(k_fill_f32, dim3((2 * MLP_NP + 255) / 256), dim3(256), 0, x->stream, x->mlp + MLP_NP, 2 * MLP_NP, 0.0f);
+  HIPCHK(hipGetLastError());
+  CHK(mlp_pack(x));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->adam_t = 0;
+  x->mlp_ready = true;
+  return MCF_OK;
+}
+
+extern "C" int mcf_mlp_get_weights(mcf_ctx *x, float *w) {
+  if (!x || !w) { g_err = "null argument"; return MCF_E_ARG; }
+  if (!x->mlp_ready) { g_err = "no weights loaded"; return MCF_E_STATE; }
+  HIPCHK(hipMemcpyAsync(w, x->mlp, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+// Gaussian initial weights, std 1/sqrt(fan_in), zero biases.
+extern "C" int mcf_mlp_init(mcf_ctx *x, uint64_t seed) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  std::vector<float> w(MLP_NP, 0.0f);
+  uint64_t s = seed;
+  auto next = [&s]() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  auto gauss = [&]() { return std::sqrt(-2.0 * std::log(next())) * std::cos(6.283185307179586 * next()); };
+  for (int e = 0; e < 192; ++e) w[MLP_OW1 + e] = (float)(gauss() / std::sqrt(3.0));
+  for (int e = 0; e < 4096; ++e) w[MLP_OW2 + e] = (float)(gauss() / 8.0);
+  for (int e = 0; e < 64; ++e) w[MLP_OW3 + e] = (float)(gauss() / 8.0);
+  return mcf_mlp_set_weights(x, w.data());
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+extern "C" int mcf_cool_samples(mcf_ctx *x, int64_t n, uint64_t seed, float *X, float *Y) {
+  if (!x || n <= 0 || !X || !Y) { g_err = "bad argument"; return MCF_E_ARG; }
+  DevBuf b;
+  HIPCHK(hipMalloc(&b.p, 4 * (size_t)n * sizeof(float)));
+  float *dX = (float *)b.p, *dY = dX + 3 * n;
+  hipLaunchKernelGGL(k_cool_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, x->stream, (long)n, (unsigned long long)seed, dX, dY);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(X, dX, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(Y, dY, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+extern "C" int mcf_mlp_forward(mcf_ctx *x, const float *X, int64_t n, float *Y) {
+  if (!x || n <= 0 || !X || !Y) { g_err = "bad argument"; return MCF_E_ARG; }
+  if (!x->mlp_ready) { g_err = "no weights loaded"; return MCF_E_STATE; }
+  DevBuf b;
+  HIPCHK(hipMalloc(&b.p, 4 * (size_t)n * sizeof(float)));
+  float *dX = (float *)b.p, *dY = dX + 3 * n;
+  HIPCHK(hipMemcpyAsync(dX, X, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, x->stream, x->mlp_pk, x->mlp, dX, (long)n, dY);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(Y, dY, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+static int mlp_grad_dev(mcf_ctx *x, const float *dX, const float *dY, long n, float *dg, float *dloss) {
+  long nch = (n + 63) / 64;
+  unsigned nb = (unsigned)(nch < 4096 ? nch : 4096);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, x->stream, x->mlp_pk, x->mlp, dX, dY, n, 1.0f / (float)n, dg, dloss);
+  HIPCHK(hipGetLastError());
+  return MCF_OK;
+}
+
+extern "C" int mcf_mlp_grad(mcf_ctx *x, const float *X, const float *T, int64_t n, float *grad, float *loss) {
+  if (!x || n <= 0 || !X || !T || !grad) { g_err = "bad argument"; return MCF_E_ARG; }
+  if (!x->mlp_ready) { g_err = "no weights loaded"; return MCF_E_STATE; }
+  DevBuf b;
+  HIPCHK(hipMalloc(&b.p, (4 * (size_t)n + MLP_NP + 64) * sizeof(float)));
+  float *dX = (float *)b.p, *dY = dX + 3 * n, *dg = dY + n, *dl = dg + MLP_NP;
+  HIPCHK(hipMemcpyAsync(dX, X, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(dY, T, (size_t)n * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  hipLaunchKernelGGL(k_fill_f32, dim3((MLP_NP + 64 + 255) / 256), dim3(256), 0, x->stream, dg, MLP_NP + 64, 0.0f);
+  CHK(mlp_grad_dev(x, dX, dY, (long)n, dg, dl));
+  HIPCHK(hipMemcpyAsync(grad, dg, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  float hl = 0.0f;
+  HIPCHK(hipMemcpyAsync(&hl, dl, sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (loss) *loss = hl;
+  return MCF_OK;
+}
+
+// Adam on the mean-squared error of IMC samples, cosine learning-rate decay.
+extern "C" int mcf_mlp_train(mcf_ctx *x, const mcf_train_opts *o, float *epoch_loss, float *val_loss) {
+  if (!x || !o || o->nsamples < o->batch || o->batch < 64 || o->batch % 64 || o->epochs <= 0 || o->nval < 64) {
+    g_err = "bad training options (batch a multiple of 64, nsamples >= batch, nval >= 64)";
+    return MCF_E_ARG;
+  }
+  if (!x->mlp_ready) CHK(mcf_mlp_init(x, o->seed ^ 0x2545F4914F6CDD1Dull));
+  const long n = (long)o->nsamples, nv = (long)o->nval, B = o->batch;
+  DevBuf b;
+  HIPCHK(hipMalloc(&b.p, (4 * (size_t)(n + nv) + 2 * (size_t)o->epochs) * sizeof(float)));
+  float *dX = (float *)b.p, *dY = dX + 3 * n, *vX = dY + n, *vY = vX + 3 * nv, *el = vY + nv, *vl = el + o->epochs;
+  hipLaunchKernelGGL(k_cool_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, x->stream, n, (unsigned long long)o->seed, dX, dY);
+  hipLaunchKernelGGL(k_cool_samples, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, x->stream, nv,
+                     (unsigned long long)(o->seed ^ 0x6A09E667F3BCC909ull), vX, vY);
+  hipLaunchKernelGGL(k_fill_f32, dim3((2 * o->epochs + 255) / 256), dim3(256), 0, x->stream, el, 2 * o->epochs, 0.0f);
+  HIPCHK(hipGetLastError());
+  float *w = x->mlp, *m = w + MLP_NP, *v = m + MLP_NP, *g = v + MLP_NP;
+  const long steps = n / B, total = steps * o->epochs;
+  const unsigned nbp = (MLP_NP + 255) / 256;
+  for (int ep = 0; ep < o->epochs; ++ep) {
+    for (long s = 0; s < steps; ++s) {
+      hipLaunchKernelGGL(k_fill_f32, dim3(nbp), dim3(256), 0, x->stream, g, MLP_NP, 0.0f);
+      hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)(B / 64)), dim3(64), 0, x->stream, x->mlp_pk, w, dX + 3 * s * B,
+                         dY + s * B, (long)B, 1.0f / (float)B, g, el + ep);
+      x->adam_t += 1;
+      const double tt = (double)(ep * steps + s + 1) / (double)total;
+      const float lr = (float)(o->lr_min + 0.5 * (o->lr - o->lr_min) * (1.0 + std::cos(3.141592653589793 * tt)));
+      hipLaunchKernelGGL(k_mlp_adam, dim3(nbp), dim3(256), 0, x->stream, w, m, v, g, lr,
+                         (float)std::pow(0.9, (double)x->adam_t), (float)std::pow(0.999, (double)x->adam_t));
+      hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_NPK + 255) / 256), dim3(256), 0, x->stream, w, x->mlp_pk);
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_f32, dim3(nbp), dim3(256), 0, x->stream, g, MLP_NP, 0.0f);
+    CHK(mlp_grad_dev(x, vX, vY, nv, g, vl + ep));
+  }
+  std::vector<float> hl(2 * o->epochs);
+  HIPCHK(hipMemcpyAsync(hl.data(), el, 2 * o->epochs * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int ep = 0; ep < o->epochs; ++ep) {
+    if (epoch_loss) epoch_loss[ep] = hl[ep] / (float)steps;
+    if (val_loss) val_loss[ep] = hl[o->epochs + ep];
+  }
+  return MCF_OK;
+}
+
+extern "C" int mcf_cool_solve(mcf_ctx *x, const double *P0, const double *nn, const double *dt, const double *rho,
+                              const double *gammi1, int64_t n, int mode, double *P, int32_t *flag, double *ms) {
+  if (!x || n <= 0 || !P0 || !nn || !dt || !rho || !gammi1 || !P || (mode != 0 && mode != 1)) {
+    g_err = "bad argument";
+    return MCF_E_ARG;
+  }
+  if (mode == 1 && !x->mlp_ready) { g_err = "no weights loaded"; return MCF_E_STATE; }
+  DevBuf b;
+  const size_t nb = (size_t)n * sizeof(double);
+  HIPCHK(hipMalloc(&b.p, 6 * nb + (size_t)n * sizeof(int)));
+  double *d0 = (double *)b.p, *d1 = d0 + n, *d2 = d1 + n, *d3 = d2 + n, *d4 = d3 + n, *dP = d4 + n;
+  int *df = (int *)(dP + n);
+  HIPCHK(hipMemcpyAsync(d0, P0, nb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(d1, nn, nb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(d2, dt, nb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(d3, rho, nb, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(d4, gammi1, nb, hipMemcpyHostToDevice, x->stream));
+  hipEvent_t ea, eb;
+  HIPCHK(hipEventCreate(&ea));
+  HIPCHK(hipEventCreate(&eb));
+  HIPCHK(hipEventRecord(ea, x->stream));
+  hipLaunchKernelGGL(k_cool_solve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, x->stream, d0, d1, d2, d3, d4,
+                     (long)n, mode, x->mlp_pk, x->mlp, dP, df);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(eb, x->stream));
+  HIPCHK(hipMemcpyAsync(P, dP, nb, hipMemcpyDeviceToHost, x->stream));
+  if (flag) HIPCHK(hipMemcpyAsync(flag, df, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, ea, eb));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  return MCF_OK;
+}
+
+extern "C" int mcf_cool_stats(mcf_ctx *x, int64_t *counts, int reset) {
+  if (!x || !counts) { g_err = "null argument"; return MCF_E_ARG; }
+  unsigned long long h[2];
+  HIPCHK(hipMemcpyAsync(h, x->cool_cnt, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  counts[0] = (int64_t)h[0];
+  counts[1] = (int64_t)h[1];
+  if (reset) {
+    HIPCHK(hipMemsetAsync(x->cool_cnt, 0, sizeof(h), x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+  }
   return MCF_OK;
 }

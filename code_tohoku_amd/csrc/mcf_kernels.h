@@ -78,6 +78,16 @@ struct SrcArgs {
   double *Tn, *Pn;
   const double *gdx[3];
   double pmin, pmax;
+  // cooling surrogate (sur != 0): packed MLP operands, FP32 weights and the
+  // counters {backward-Euler cells, IMC fallbacks}
+  // and the queue of backward-Euler cells with their conduction increments
+  int sur;
+  const __bf16 *pk;
+  const float *mw;
+  unsigned long long *cnt;
+  unsigned *qn;
+  int *queue;
+  double *qdc;
 };
 
 struct DtArgs {
@@ -102,11 +112,25 @@ template <class R> __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient,
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
 __global__ void k_src_temp(SrcArgs A, Geo g, Box b, int savep);
-__global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
+template <bool SUR> __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
+__global__ void k_src_stiff(SrcArgs A, double dt, int stage);
+__global__ void k_zero_u32(unsigned *p);
 __global__ void k_dtmin(DtArgs A, Geo g, double *part);
 __global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal);
 __global__ void k_dissip_div(DisArgs A, Geo g, int e0);
 __global__ void k_dissip_upd(DisArgs A, Geo g, int e0);
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n);
+
+// cooling surrogate (mcf_cool.hip)
+__global__ void k_cool_samples(long n, unsigned long long seed, float *X, float *Y);
+__global__ void k_mlp_pack(const float *w, __bf16 *pk);
+__global__ void k_mlp_infer(const __bf16 *pk, const float *w, const float *X, long n, float *Y);
+__global__ void k_mlp_grad(const __bf16 *pk, const float *w, const float *X, const float *Yt, long n, float inv_b,
+                           float *grad, float *loss);
+__global__ void k_mlp_adam(float *w, float *m, float *v, const float *g, float lr, float b1t, float b2t);
+__global__ void k_fill_f32(float *a, int n, float v);
+__global__ void k_cool_solve(const double *P0, const double *nn, const double *dt, const double *rho,
+                             const double *gi, long n, int mode, const __bf16 *pk, const float *w, double *Pout,
+                             int *flag);
 
 } // namespace mcf

@@ -10,6 +10,7 @@
 // reproduce the reference's halo contents exactly.
 #include "mcf_kernels.h"
 #include "mcf_prec.h"
+#include "mcf_cool.h"
 
 namespace mcf {
 
@@ -293,9 +294,15 @@ __device__ __forceinline__ double qflux(const double *Tn, int c, int s, double d
   return KCOND * dmin(rt, 3.873) * (Tn[c + s] - Tn[c]) * 2.0 / (dxi + dxp);
 }
 
-__global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
+// One stage of SOURCE's thermal update (M3:2461-2572).  With SUR the
+// backward-Euler cells (dt > 0.2*t_cool) are not solved here: they are
+// queued with their conduction increment for k_src_stiff.
+template <bool SUR>
+__global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
   int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
+  const bool valid = box_ijk(b, i, j, k);
+  if (!SUR && !valid) return;
+  if (!valid) { i = j = k = 1; }
   int c = ix3(g, i, j, k);
   const double *const *sp = A.sp;
   const double *Tn = A.Tn;
@@ -305,14 +312,15 @@ __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
   double gi = gammi1_of(sp, c);
   double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
   double Pn = A.Pn[c];
-  double u;
+  const bool expl = dt <= 0.2 * Pn / (gi * fabs(CooL));
+  double u = 0.0;
   if (stage == 0) {
     u = A.U5[c];
-    if (dt <= 0.2 * Pn / (gi * fabs(CooL))) u = u - gi * CooL * dt * 0.5;
-    else u = imc(u, nsum, dt * 0.5, rho, gi);
+    if (expl) u = u - gi * CooL * dt * 0.5;
+    else if (!SUR) u = imc(u, nsum, dt * 0.5, rho, gi);
   } else {
-    if (dt <= 0.2 * Pn / (gi * fabs(CooL))) u = Pn - gi * CooL * dt;
-    else {
+    if (expl) u = Pn - gi * CooL * dt;
+    else if (!SUR) {
       Pn = imc(Pn, nsum, dt, rho, gi);
       A.Pn[c] = Pn;
       u = Pn;
@@ -330,11 +338,75 @@ __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
   double qzp = qflux(Tn, c, sz, dzk, A.gdx[2][k + 2]);
   double qzm = qflux(Tn, c - sz, sz, A.gdx[2][k], dzk);
   double div = (qxp - qxm) / dxi + (qyp - qym) / dyj + (qzp - qzm) / dzk;
-  if (stage == 0) u = u + (tcd + 0.5) * gi * dt * 0.5 * div;
-  else u = u + (tcd + 0.5) * gi * dt * div;
+  double dc;
+  if (stage == 0) dc = (tcd + 0.5) * gi * dt * 0.5 * div;
+  else dc = (tcd + 0.5) * gi * dt * div;
+  if constexpr (SUR) {
+    const bool stiff = valid && !expl;
+    const unsigned long long m = __ballot(stiff);
+    if (m) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)m) - 1;
+      unsigned base = 0;
+      if (lane == leader) base = atomicAdd(A.qn, (unsigned)__popcll(m));
+      base = __shfl(base, leader, 64);
+      if (stiff) {
+        const unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        A.queue[pos] = c;
+        A.qdc[c] = dc;
+      }
+    }
+    if (!valid || stiff) return;
+  }
+  u = u + dc;
   u = dmax(u, A.pmin);
   u = dmin(u, A.pmax);
   A.U5[c] = u;
+}
+template __global__ void k_src_cell<false>(SrcArgs, Geo, Box, double, int);
+template __global__ void k_src_cell<true>(SrcArgs, Geo, Box, double, int);
+
+// The queued backward-Euler cells of one SOURCE stage, 64 per wave: the
+// surrogate predicts, two Newton steps refine, the residual test accepts
+// or IMC solves; then the cell's conduction increment and the pressure
+// clamps are applied as in k_src_cell.  Waves stride over the queue.
+__global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int stage) {
+  const unsigned nq = *A.qn;
+  const unsigned wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned nw = (gridDim.x * blockDim.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  unsigned long long nfb = 0;
+  for (unsigned base = wid * 64u; base < nq; base += nw * 64u) {
+    const unsigned e = base + (unsigned)lane;
+    const bool ok = e < nq;
+    const int c = ok ? A.queue[e] : A.queue[base];
+    const double *const *sp = A.sp;
+    const double rho = A.rho[c];
+    const double gi = gammi1_of(sp, c);
+    const double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
+    const double P0 = stage == 0 ? A.U5[c] : A.Pn[c];
+    const double dts = stage == 0 ? dt * 0.5 : dt;
+    float u0, u1, u2;
+    sur_features(P0, nsum, dts, rho, gi, u0, u1, u2);
+    const float y = mlp_wave_forward(A.pk, A.mw, u0, u1, u2);
+    double Pi;
+    const bool fell = !sur_refine(P0, nsum, dts, rho, gi, y, Pi);
+    if (ok && fell) Pi = imc(P0, nsum, dts, rho, gi);
+    nfb += __popcll(__ballot(ok && fell));
+    if (ok) {
+      if (stage == 1) A.Pn[c] = Pi;
+      double u = Pi + A.qdc[c];
+      u = dmax(u, A.pmin);
+      u = dmin(u, A.pmax);
+      A.U5[c] = u;
+    }
+  }
+  if (lane == 0 && nfb) atomicAdd(&A.cnt[1], nfb);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[0], (unsigned long long)nq);
+}
+
+__global__ void k_zero_u32(unsigned *p) {
+  if (threadIdx.x == 0) *p = 0u;
 }
 
 // ---------------------------------------------------------------------------

@@ -47,33 +47,39 @@ template <class R> __device__ __forceinline__ bool use_minmod_r(double u5, doubl
 // van Albada  g  = max(0,(2d+d- + eps)/(d+^2 + d-^2 + eps)) (q+ - q-)/(dx0 + dx-/2 + dx+/2)
 // minmod      gm = minmod(2d+/(dx0+dx+), 2d-/(dx0+dx-)).
 // The reference blends g*(0.5-s) + gm*(0.5+s) with s = +-0.5, which equals
-// the selected candidate for every finite input.
+// the selected candidate for every finite input.  Both candidates are two
+// divisions; the operands of the two are selected first, so the code has
+// no branch (the loads of many slopes can be issued together) and each
+// candidate keeps its own operation order.
 template <class R>
 __device__ __forceinline__ R vslope(double qm, double q0, double qp, R dxm, R dx0, R dxp, bool mm) {
   if constexpr (Ar<R>::strict) {
-    double delp = qp - q0;
-    double delm = q0 - qm;
-    if (!mm) {
-      double flmt = (2.0 * delp * delm + EPSL) / (delp * delp + delm * delm + EPSL);
-      flmt = dmax(0.0, flmt);
-      return flmt * (qp - qm) / (dx0 + 0.5 * dxm + 0.5 * dxp);
-    }
-    delp = 2.0 * delp / (dx0 + dxp);
-    delm = 2.0 * delm / (dx0 + dxm);
-    return (0.5 + copysign(0.5, delp * delm)) * copysign(1.0, delp) * dmin(fabs(delp), fabs(delm));
+    const double delp = qp - q0;
+    const double delm = q0 - qm;
+    const double tp = 2.0 * delp;
+    const double n1 = mm ? tp : tp * delm + EPSL;
+    const double d1 = mm ? dx0 + dxp : delp * delp + delm * delm + EPSL;
+    const double r1 = n1 / d1;
+    const double n2 = mm ? 2.0 * delm : dmax(0.0, r1) * (qp - qm);
+    const double d2 = mm ? dx0 + dxm : dx0 + 0.5 * dxm + 0.5 * dxp;
+    const double r2 = n2 / d2;
+    const double gm = (0.5 + copysign(0.5, r1 * r2)) * copysign(1.0, r1) * dmin(fabs(r1), fabs(r2));
+    return mm ? gm : r2;
   } else {
-    float delp = (float)(qp - q0);
-    float delm = (float)(q0 - qm);
-    if (!mm) {
-      const float eps = (float)EPSL;
-      float flmt = Ar<float>::div(2.0f * delp * delm + eps, delp * delp + delm * delm + eps);
-      flmt = rmax(0.0f, flmt);
-      return Ar<float>::div(flmt * (float)(qp - qm), dx0 + 0.5f * dxm + 0.5f * dxp);
-    }
-    float a = Ar<float>::div(2.0f * delp, dx0 + dxp);
-    float b = Ar<float>::div(2.0f * delm, dx0 + dxm);
-    float m = rmin(fabsf(a), fabsf(b));
-    return (a * b > 0.0f) ? copysignf(m, a) : 0.0f;
+    using A = Ar<float>;
+    const float delp = (float)(qp - q0);
+    const float delm = (float)(q0 - qm);
+    const float eps = (float)EPSL;
+    const float tp = 2.0f * delp;
+    const float n1 = mm ? tp : tp * delm + eps;
+    const float d1 = mm ? dx0 + dxp : delp * delp + delm * delm + eps;
+    const float r1 = A::div(n1, d1);
+    const float n2 = mm ? 2.0f * delm : rmax(0.0f, r1) * (float)(qp - qm);
+    const float d2 = mm ? dx0 + dxm : dx0 + 0.5f * dxm + 0.5f * dxp;
+    const float r2 = A::div(n2, d2);
+    const float m = rmin(fabsf(r1), fabsf(r2));
+    const float gm = (r1 * r2 > 0.0f) ? copysignf(m, r1) : 0.0f;
+    return mm ? gm : r2;
   }
 }
 

@@ -21,6 +21,9 @@ LIB_PATH = os.environ.get("MCF_LIB", os.path.join(HERE, "lib", "libmcf.so"))
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+COOL_KI = 1
+COOL_SURROGATE = 2
+MLP_NPARAM = 4481
 
 
 class Config(C.Structure):
@@ -39,6 +42,11 @@ class Config(C.Structure):
 
 class KStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("nsamples", C.c_int64), ("nval", C.c_int64), ("seed", C.c_uint64), ("batch", C.c_int),
+                ("epochs", C.c_int), ("lr", C.c_float), ("lr_min", C.c_float)]
 
 
 class MCFError(RuntimeError):
@@ -79,10 +87,22 @@ def lib():
         L.mcf_profile.argtypes = [P, I]
         L.mcf_copy_probe.argtypes = [P, C.c_long, I, pd]
         L.mcf_profile_report.argtypes = [P, C.POINTER(KStat), I, pi]
+        pf, pi32, pi64 = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+        L.mcf_mlp_init.argtypes = [P, C.c_uint64]
+        L.mcf_mlp_set_weights.argtypes = [P, pf]
+        L.mcf_mlp_get_weights.argtypes = [P, pf]
+        L.mcf_mlp_train.argtypes = [P, C.POINTER(TrainOpts), pf, pf]
+        L.mcf_mlp_forward.argtypes = [P, pf, C.c_int64, pf]
+        L.mcf_mlp_grad.argtypes = [P, pf, pf, C.c_int64, pf, pf]
+        L.mcf_cool_samples.argtypes = [P, C.c_int64, C.c_uint64, pf, pf]
+        L.mcf_cool_solve.argtypes = [P, pd, pd, pd, pd, pd, C.c_int64, I, pd, pi32, pd]
+        L.mcf_cool_stats.argtypes = [P, pi64, I]
         for nm in ("mcf_init", "mcf_finalize", "mcf_upload", "mcf_download", "mcf_download_bcc",
                    "mcf_set_time", "mcf_get_time", "mcf_timestep", "mcf_source", "mcf_sweep",
                    "mcf_ct", "mcf_dissip", "mcf_step", "mcf_run", "mcf_sync", "mcf_profile",
-                   "mcf_profile_report", "mcf_copy_probe"):
+                   "mcf_profile_report", "mcf_copy_probe", "mcf_mlp_init", "mcf_mlp_set_weights",
+                   "mcf_mlp_get_weights", "mcf_mlp_train", "mcf_mlp_forward", "mcf_mlp_grad",
+                   "mcf_cool_samples", "mcf_cool_solve", "mcf_cool_stats"):
             getattr(L, nm).restype = C.c_int
         _lib = L
     return _lib
@@ -94,7 +114,7 @@ def _chk(rc: int):
         raise MCFError(f"mcf error {rc}: {msg}")
 
 
-def make_config(p: Input3D, nsplit=(2, 2, 2), precision: int = PREC_FP64) -> Config:
+def make_config(p: Input3D, nsplit=(2, 2, 2), precision: int = PREC_FP64, cooling: int = COOL_KI) -> Config:
     c = Config()
     lib().mcf_default_config(C.byref(c))
     for name, _ in Config._fields_:
@@ -102,7 +122,7 @@ def make_config(p: Input3D, nsplit=(2, 2, 2), precision: int = PREC_FP64) -> Con
             setattr(c, name, getattr(p, name))
     c.nsplitx, c.nsplity, c.nsplitz = nsplit
     c.precision = precision
-    c.cooling = 1
+    c.cooling = cooling
     return c
 
 
@@ -110,13 +130,17 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
 class Domain:
     """One domain resident on the current HIP device."""
 
-    def __init__(self, p: Input3D, nsplit=(2, 2, 2), precision: int = PREC_FP64):
+    def __init__(self, p: Input3D, nsplit=(2, 2, 2), precision: int = PREC_FP64, cooling: int = COOL_KI):
         self.p = p
         self.n = p.n
-        self.cfg = make_config(p, nsplit, precision)
+        self.cfg = make_config(p, nsplit, precision, cooling)
         h = C.c_void_p()
         _chk(lib().mcf_init(C.byref(self.cfg), C.byref(h)))
         self.h = h
@@ -210,3 +234,62 @@ class Domain:
         n = C.c_int()
         _chk(lib().mcf_profile_report(self.h, buf, 128, C.byref(n)))
         return {buf[i].name.decode(): (buf[i].launches, buf[i].ms, buf[i].bytes) for i in range(min(n.value, 128))}
+
+    # --- KI-cooling surrogate -------------------------------------------
+    def mlp_init(self, seed: int = 1):
+        _chk(lib().mcf_mlp_init(self.h, seed))
+
+    def mlp_set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.shape == (MLP_NPARAM,)
+        _chk(lib().mcf_mlp_set_weights(self.h, _fp(w)))
+
+    def mlp_get_weights(self) -> np.ndarray:
+        w = np.empty(MLP_NPARAM, np.float32)
+        _chk(lib().mcf_mlp_get_weights(self.h, _fp(w)))
+        return w
+
+    def mlp_train(self, nsamples=10_000_000, nval=1 << 17, seed=1, batch=8192, epochs=8, lr=3e-3, lr_min=1e-5):
+        """Train on IMC samples generated on the device; (train, held-out) loss per epoch."""
+        o = TrainOpts(nsamples, nval, seed, batch, epochs, lr, lr_min)
+        el = np.zeros(epochs, np.float32)
+        vl = np.zeros(epochs, np.float32)
+        _chk(lib().mcf_mlp_train(self.h, C.byref(o), _fp(el), _fp(vl)))
+        return el, vl
+
+    def mlp_forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty(len(X), np.float32)
+        _chk(lib().mcf_mlp_forward(self.h, _fp(X), len(X), _fp(y)))
+        return y
+
+    def mlp_grad(self, X: np.ndarray, T: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        _chk(lib().mcf_mlp_grad(self.h, _fp(X), _fp(T), len(X), _fp(g), C.byref(loss)))
+        return g, loss.value
+
+    def cool_samples(self, n: int, seed: int = 1):
+        X = np.empty((n, 3), np.float32)
+        Y = np.empty(n, np.float32)
+        _chk(lib().mcf_cool_samples(self.h, n, seed, _fp(X), _fp(Y)))
+        return X, Y
+
+    def cool_solve(self, P0, nn, dt, rho, gammi1, mode: int):
+        """Backward-Euler cooling solves: (P, fallback flags, kernel ms)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (P0, nn, dt, rho, gammi1)]
+        n = len(arrs[0])
+        P = np.empty(n)
+        fl = np.empty(n, np.int32)
+        ms = C.c_double()
+        _chk(lib().mcf_cool_solve(self.h, *[_dp(a) for a in arrs], n, mode, _dp(P),
+                                  fl.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(ms)))
+        return P, fl, ms.value
+
+    def cool_stats(self, reset: bool = False):
+        """(backward-Euler cells, IMC fallbacks) counted in SOURCE."""
+        c = np.zeros(2, np.int64)
+        _chk(lib().mcf_cool_stats(self.h, c.ctypes.data_as(C.POINTER(C.c_int64)), 1 if reset else 0))
+        return int(c[0]), int(c[1])

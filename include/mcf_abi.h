@@ -31,7 +31,10 @@ enum {
 };
 
 enum { MCF_PREC_FP64 = 0, MCF_PREC_MIXED = 1 };
-enum { MCF_COOL_KI = 1 };
+/* MCF_COOL_KI: KI cooling with the reference's explicit/IMC update.
+ * MCF_COOL_SURROGATE: the same, with the backward-Euler (IMC) cells solved
+ * by the learned surrogate (see the surrogate section below). */
+enum { MCF_COOL_KI = 1, MCF_COOL_SURROGATE = 2 };
 
 /* INPUT3D.DAT (M3:134-151), the WNM state hard-wired by INITIA
  * (M3:155-157) and the options of this implementation. */
@@ -52,7 +55,7 @@ typedef struct mcf_config {
    * results depend on the block edges.  1 = one block on that axis. */
   int nsplitx, nsplity, nsplitz;
   int precision;                             /* MCF_PREC_*  */
-  int cooling;                               /* MCF_COOL_KI */
+  int cooling;                               /* MCF_COOL_*  */
 } mcf_config;
 
 typedef struct mcf_ctx mcf_ctx;
@@ -120,6 +123,55 @@ int mcf_profile_report(mcf_ctx *ctx, mcf_kstat *out, int maxn, int *n);
  * achievable-bandwidth and hardware-counter reference for the roofline.
  * No reference counterpart. */
 int mcf_copy_probe(mcf_ctx *ctx, long nwords, int reps, double *ms);
+
+/* ---------------------------------------------------------------------
+ * KI-cooling surrogate.  No counterpart in the reference: it replaces the
+ * IMC bisection (M3:2898-2921) of SOURCE's backward-Euler cells
+ * (dt > 0.2 P/(gammi1 |Lambda|), M3:2480-2500) when cfg.cooling is
+ * MCF_COOL_SURROGATE.  An MLP 3 -> 64 -> 64 -> 1 (SiLU; BF16 MFMA operands,
+ * FP32 accumulation) maps (log10 N, log10 T, log10 dt/t_cool) to
+ * log10 T_end/T; two Newton steps on the backward-Euler equation refine it
+ * and a cell whose relative residual exceeds 1e-4 is solved by IMC.
+ * Parameters: MCF_MLP_NPARAM floats, W1[64][3] b1[64] W2[64][64] b2[64]
+ * W3[64] b3.  Features and target are scaled as in mcf_cool_samples.
+ * --------------------------------------------------------------------- */
+#define MCF_MLP_NPARAM 4481
+typedef struct mcf_train_opts {
+  int64_t nsamples;   /* training samples drawn from IMC */
+  int64_t nval;       /* held-out samples */
+  uint64_t seed;
+  int batch;          /* multiple of 64 */
+  int epochs;
+  float lr, lr_min;   /* Adam, cosine decay from lr to lr_min */
+} mcf_train_opts;
+/* Gaussian initial weights (std 1/sqrt(fan_in)), zero biases. */
+int mcf_mlp_init(mcf_ctx *ctx, uint64_t seed);
+int mcf_mlp_set_weights(mcf_ctx *ctx, const float *w);
+int mcf_mlp_get_weights(mcf_ctx *ctx, float *w);
+/* Generates o->nsamples IMC samples on the device and trains the current
+ * weights (initialised first if none are loaded).  epoch_loss / val_loss
+ * (NULL or o->epochs floats) receive the mean training loss and the
+ * held-out loss after each epoch. */
+int mcf_mlp_train(mcf_ctx *ctx, const mcf_train_opts *o, float *epoch_loss, float *val_loss);
+/* Forward pass: X[n][3] scaled features -> Y[n] (host arrays). */
+int mcf_mlp_forward(mcf_ctx *ctx, const float *X, int64_t n, float *Y);
+/* Mean-squared-error loss over (X, T) and its gradient with respect to the
+ * MCF_MLP_NPARAM parameters (backward pass of the training kernels). */
+int mcf_mlp_grad(mcf_ctx *ctx, const float *X, const float *T, int64_t n, float *grad, float *loss);
+/* n training samples: log-uniform N in [0.1,1e4] cm^-3, T in [10,1e4] K,
+ * dt/t_cool in [1e-3,1e3] with gammi1 = 2/3 and n = N.  X[i][0..2] =
+ * ((log10 N - 1.5)/2.5, (log10 T[K] - 2.5)/1.5, log10(dt/t_cool)/3);
+ * Y[i] = log10(P_IMC/P_old)/3. */
+int mcf_cool_samples(mcf_ctx *ctx, int64_t n, uint64_t seed, float *X, float *Y);
+/* Backward-Euler solves P + gammi1*Lambda(rho, P/(kb n))*dt = P0 for n
+ * cells (host arrays): mode 0 IMC, mode 1 surrogate + Newton + residual
+ * test with IMC fallback (flag[i] = 1 where IMC ran; flag may be NULL).
+ * *ms = kernel time. */
+int mcf_cool_solve(mcf_ctx *ctx, const double *P0, const double *n, const double *dt, const double *rho,
+                   const double *gammi1, int64_t ncell, int mode, double *P, int32_t *flag, double *ms);
+/* counts[0] = backward-Euler cells solved in SOURCE since the last reset,
+ * counts[1] = of those, the IMC fallbacks (surrogate mode only). */
+int mcf_cool_stats(mcf_ctx *ctx, int64_t *counts, int reset);
 
 /* Waits for all work of the context. */
 int mcf_sync(mcf_ctx *ctx);
