@@ -173,33 +173,54 @@ __global__ void k_fill_f32(float *a, int n, float v) {
 }
 
 // Backward-Euler cooling solves P + gammi1*Lambda(P)*dt = P0 for n cells.
-// mode 0: IMC.  mode 1: surrogate + Newton + residual test, IMC where the
-// test fails (flag[i] = 1 there, 0 where the surrogate result stands).
-__global__ __launch_bounds__(256) void k_cool_solve(const double *P0, const double *nn, const double *dt, const double *rho,
-                             const double *gi, long n, int mode, const __bf16 *pk, const float *w, double *Pout,
-                             int *flag) {
+// mode 0: IMC.  mode 1: surrogate + Newton + residual test; rejected cells
+// (flag[i] = 1) are queued in fq for k_cool_fallback, which runs IMC on
+// dense waves.
+__global__ __launch_bounds__(256) void k_cool_solve(const double *P0, const double *nn, const double *dt,
+                                                    const double *rho, const double *gi, long n, int mode,
+                                                    const __bf16 *pk, const float *w, double *Pout, int *flag,
+                                                    unsigned *nfb, int *fq) {
   long base = ((long)blockIdx.x * blockDim.x + threadIdx.x) & ~63L;
-  long i = base + (threadIdx.x & 63);
+  const int lane = threadIdx.x & 63;
+  long i = base + lane;
   bool ok = i < n;
   double p0 = ok ? P0[i] : 1.0, n0 = ok ? nn[i] : 1.0, d0 = ok ? dt[i] : 0.0, r0 = ok ? rho[i] : 1.27;
   double g0 = ok ? gi[i] : 2.0 / 3.0;
-  double P;
-  int fb = 0;
   if (mode == 0) {
-    P = imc(p0, n0, d0, r0, g0);
-  } else {
-    float u0, u1, u2;
-    sur_features(p0, n0, d0, r0, g0, u0, u1, u2);
-    if (!ok) u0 = u1 = u2 = 0.0f;
-    float y = mlp_wave_forward(pk, w, u0, u1, u2);
-    if (!sur_refine(p0, n0, d0, r0, g0, y, P)) {
-      P = imc(p0, n0, d0, r0, g0);
-      fb = 1;
+    if (ok) {
+      Pout[i] = imc(p0, n0, d0, r0, g0);
+      if (flag) flag[i] = 0;
     }
+    return;
+  }
+  float u0, u1, u2;
+  sur_features(p0, n0, d0, r0, g0, u0, u1, u2);
+  if (!ok) u0 = u1 = u2 = 0.0f;
+  float y = mlp_wave_forward(pk, w, u0, u1, u2);
+  double P;
+  const bool fell = !sur_refine(p0, n0, d0, r0, g0, y, P);
+  const unsigned long long mf = __ballot(ok && fell);
+  if (mf) {
+    const int leader = __ffsll((long long)mf) - 1;
+    unsigned b = 0;
+    if (lane == leader) b = atomicAdd(nfb, (unsigned)__popcll(mf));
+    b = __shfl(b, leader, 64);
+    if (ok && fell) fq[b + (unsigned)__popcll(mf & ((1ull << lane) - 1ull))] = (int)i;
   }
   if (ok) {
-    Pout[i] = P;
-    if (flag) flag[i] = fb;
+    if (!fell) Pout[i] = P;
+    if (flag) flag[i] = fell ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cool_fallback(const double *P0, const double *nn, const double *dt,
+                                                       const double *rho, const double *gi, const unsigned *nfb,
+                                                       const int *fq, double *Pout) {
+  const unsigned nf = *nfb;
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nf; e += stride) {
+    const int i = fq[e];
+    Pout[i] = imc(P0[i], nn[i], dt[i], rho[i], gi[i]);
   }
 }
 

@@ -466,6 +466,7 @@ static int do_source(mcf_ctx *x, double dt) {
     A.cnt = x->cool_cnt;
     A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2);
     A.queue = reinterpret_cast<int *>(x->dq[1]);
+    A.fqueue = reinterpret_cast<int *>(x->dq[2]);
     A.qdc = x->dq[0];
   }
   int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
@@ -475,18 +476,20 @@ static int do_source(mcf_ctx *x, double dt) {
   Box bt = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1), bc = mkbox(1, nx, 1, ny, 1, nz);
   KLAUNCH(k_src_temp, boxsize(bt), 8, A, x->g, bt, 1);
   if (A.sur) {
-    KLAUNCH_G(k_zero_u32, dim3(1), dim3(64), 0, A.qn);
+    KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
     KLAUNCH_R(k_src_cell, true, boxsize(bc), 10, A, x->g, bc, dt, 0);
     KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 0);
+    KLAUNCH_G(k_src_fallback, dim3(256), dim3(256), 0, A, dt, 0);
   } else {
     KLAUNCH_R(k_src_cell, false, boxsize(bc), 10, A, x->g, bc, dt, 0);
   }
   CHK(fillU(x, 7, 1, 1, l5, 1));
   KLAUNCH(k_src_temp, boxsize(bt), 7, A, x->g, bt, 0);
   if (A.sur) {
-    KLAUNCH_G(k_zero_u32, dim3(1), dim3(64), 0, A.qn);
+    KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
     KLAUNCH_R(k_src_cell, true, boxsize(bc), 10, A, x->g, bc, dt, 1);
     KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 1);
+    KLAUNCH_G(k_src_fallback, dim3(256), dim3(256), 0, A, dt, 1);
   } else {
     KLAUNCH_R(k_src_cell, false, boxsize(bc), 10, A, x->g, bc, dt, 1);
   }
@@ -944,9 +947,11 @@ extern "C" int mcf_cool_solve(mcf_ctx *x, const double *P0, const double *nn, co
   if (mode == 1 && !x->mlp_ready) { g_err = "no weights loaded"; return MCF_E_STATE; }
   DevBuf b;
   const size_t nb = (size_t)n * sizeof(double);
-  HIPCHK(hipMalloc(&b.p, 6 * nb + (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc(&b.p, 6 * nb + 2 * (size_t)n * sizeof(int) + 64));
   double *d0 = (double *)b.p, *d1 = d0 + n, *d2 = d1 + n, *d3 = d2 + n, *d4 = d3 + n, *dP = d4 + n;
-  int *df = (int *)(dP + n);
+  int *df = (int *)(dP + n), *fq = df + n;
+  unsigned *nfb = (unsigned *)(fq + n);
+  hipLaunchKernelGGL(k_zero_u32x2, dim3(1), dim3(64), 0, x->stream, nfb);
   HIPCHK(hipMemcpyAsync(d0, P0, nb, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipMemcpyAsync(d1, nn, nb, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipMemcpyAsync(d2, dt, nb, hipMemcpyHostToDevice, x->stream));
@@ -957,7 +962,9 @@ extern "C" int mcf_cool_solve(mcf_ctx *x, const double *P0, const double *nn, co
   HIPCHK(hipEventCreate(&eb));
   HIPCHK(hipEventRecord(ea, x->stream));
   hipLaunchKernelGGL(k_cool_solve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, x->stream, d0, d1, d2, d3, d4,
-                     (long)n, mode, x->mlp_pk, x->mlp, dP, df);
+                     (long)n, mode, x->mlp_pk, x->mlp, dP, df, nfb, fq);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_cool_fallback, dim3(256), dim3(256), 0, x->stream, d0, d1, d2, d3, d4, nfb, fq, dP);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(eb, x->stream));
   HIPCHK(hipMemcpyAsync(P, dP, nb, hipMemcpyDeviceToHost, x->stream));

@@ -85,8 +85,8 @@ struct SrcArgs {
   const __bf16 *pk;
   const float *mw;
   unsigned long long *cnt;
-  unsigned *qn;
-  int *queue;
+  unsigned *qn;   // {queued cells, rejected cells}
+  int *queue, *fqueue;
   double *qdc;
 };
 
@@ -114,7 +114,8 @@ __global__ void k_cc4(CtArgs A, Geo g, Box b);
 __global__ void k_src_temp(SrcArgs A, Geo g, Box b, int savep);
 template <bool SUR> __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
-__global__ void k_zero_u32(unsigned *p);
+__global__ void k_src_fallback(SrcArgs A, double dt, int stage);
+__global__ void k_zero_u32x2(unsigned *p);
 __global__ void k_dtmin(DtArgs A, Geo g, double *part);
 __global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal);
 __global__ void k_dissip_div(DisArgs A, Geo g, int e0);
@@ -131,6 +132,8 @@ __global__ void k_mlp_adam(float *w, float *m, float *v, const float *g, float l
 __global__ void k_fill_f32(float *a, int n, float v);
 __global__ void k_cool_solve(const double *P0, const double *nn, const double *dt, const double *rho,
                              const double *gi, long n, int mode, const __bf16 *pk, const float *w, double *Pout,
-                             int *flag);
+                             int *flag, unsigned *nfb, int *fq);
+__global__ void k_cool_fallback(const double *P0, const double *nn, const double *dt, const double *rho,
+                                const double *gi, const unsigned *nfb, const int *fq, double *Pout);
 
 } // namespace mcf

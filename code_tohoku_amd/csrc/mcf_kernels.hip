@@ -367,15 +367,23 @@ template __global__ void k_src_cell<false>(SrcArgs, Geo, Box, double, int);
 template __global__ void k_src_cell<true>(SrcArgs, Geo, Box, double, int);
 
 // The queued backward-Euler cells of one SOURCE stage, 64 per wave: the
-// surrogate predicts, two Newton steps refine, the residual test accepts
-// or IMC solves; then the cell's conduction increment and the pressure
-// clamps are applied as in k_src_cell.  Waves stride over the queue.
+// surrogate predicts, Newton steps refine, the residual test accepts; the
+// cell's conduction increment and the pressure clamps are then applied as
+// in k_src_cell.  Rejected cells go to a second queue for k_src_fallback,
+// so IMC runs on dense waves.  Waves stride over the queue.
+__device__ __forceinline__ void src_finish(const SrcArgs &A, int c, int stage, double Pi) {
+  if (stage == 1) A.Pn[c] = Pi;
+  double u = Pi + A.qdc[c];
+  u = dmax(u, A.pmin);
+  u = dmin(u, A.pmax);
+  A.U5[c] = u;
+}
+
 __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int stage) {
   const unsigned nq = *A.qn;
   const unsigned wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned nw = (gridDim.x * blockDim.x) >> 6;
   const int lane = threadIdx.x & 63;
-  unsigned long long nfb = 0;
   for (unsigned base = wid * 64u; base < nq; base += nw * 64u) {
     const unsigned e = base + (unsigned)lane;
     const bool ok = e < nq;
@@ -391,23 +399,40 @@ __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int sta
     const float y = mlp_wave_forward(A.pk, A.mw, u0, u1, u2);
     double Pi;
     const bool fell = !sur_refine(P0, nsum, dts, rho, gi, y, Pi);
-    if (ok && fell) Pi = imc(P0, nsum, dts, rho, gi);
-    nfb += __popcll(__ballot(ok && fell));
-    if (ok) {
-      if (stage == 1) A.Pn[c] = Pi;
-      double u = Pi + A.qdc[c];
-      u = dmax(u, A.pmin);
-      u = dmin(u, A.pmax);
-      A.U5[c] = u;
+    const unsigned long long mf = __ballot(ok && fell);
+    if (mf) {
+      const int leader = __ffsll((long long)mf) - 1;
+      unsigned fb = 0;
+      if (lane == leader) fb = atomicAdd(A.qn + 1, (unsigned)__popcll(mf));
+      fb = __shfl(fb, leader, 64);
+      if (ok && fell) A.fqueue[fb + (unsigned)__popcll(mf & ((1ull << lane) - 1ull))] = c;
     }
+    if (ok && !fell) src_finish(A, c, stage, Pi);
   }
-  if (lane == 0 && nfb) atomicAdd(&A.cnt[1], nfb);
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[0], (unsigned long long)nq);
 }
 
-__global__ void k_zero_u32(unsigned *p) {
-  if (threadIdx.x == 0) *p = 0u;
+// IMC for the cells the surrogate's residual test rejected.
+__global__ __launch_bounds__(256) void k_src_fallback(SrcArgs A, double dt, int stage) {
+  const unsigned nf = A.qn[1];
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nf; e += stride) {
+    const int c = A.fqueue[e];
+    const double *const *sp = A.sp;
+    const double rho = A.rho[c];
+    const double gi = gammi1_of(sp, c);
+    const double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
+    const double P0 = stage == 0 ? A.U5[c] : A.Pn[c];
+    const double dts = stage == 0 ? dt * 0.5 : dt;
+    src_finish(A, c, stage, imc(P0, nsum, dts, rho, gi));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[1], (unsigned long long)nf);
 }
+
+__global__ void k_zero_u32x2(unsigned *p) {
+  if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+}
+
 
 // ---------------------------------------------------------------------------
 // Couran + Stblty: min over interior cells

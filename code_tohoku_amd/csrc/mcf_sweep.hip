@@ -76,6 +76,22 @@ struct Roles {
   bool u_ok;         // phase 3: cell 1..N updated by this thread
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Renumber them so that every XCD takes one contiguous range of tiles,
+// ordered with the first two grid axes fastest: tiles that share halo rows
+// (neighbours along x for the x sweep, along the sweep axis for y/z) then
+// run on the same XCD at about the same time.  The renumbering is a
+// bijection; grids whose tile count is not a multiple of 8 keep the
+// dispatch order.
+__device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, nt = gx * gy * gridDim.z;
+  unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  if ((nt & 7u) == 0u) t = (t & 7u) * (nt >> 3) + (t >> 3);
+  bx = (int)(t % gx);
+  by = (int)((t / gx) % gy);
+  bz = (int)(t / (gx * gy));
+}
+
 template <int D>
 __device__ __forceinline__ Roles roles(const Geo &g) {
   Roles r;
@@ -84,10 +100,12 @@ __device__ __forceinline__ Roles roles(const Geo &g) {
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   r.lslot = r.rslot = 0;
   r.u_ok = false;
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
   if (D == 0) {
-    const int s0 = 1 + 64 * (int)blockIdx.x;
-    const int jr = 1 + 8 * (int)blockIdx.y;
-    r.k = 1 + (int)blockIdx.z;
+    const int s0 = 1 + 64 * bx;
+    const int jr = 1 + 8 * by;
+    r.k = 1 + bz;
     if (w < 8) {
       r.i = s0 + l; r.j = jr + w;
       r.cslot = w * XL + l + 1;
@@ -116,9 +134,9 @@ __device__ __forceinline__ Roles roles(const Geo &g) {
     r.rslot_c = r.cslot + 1;
     r.q = r.i;
   } else {
-    const int i0 = 1 + 64 * (int)blockIdx.x;
-    const int s0 = 1 + 8 * (int)blockIdx.y;
-    const int tr = 1 + (int)blockIdx.z;
+    const int i0 = 1 + 64 * bx;
+    const int s0 = 1 + 8 * by;
+    const int tr = 1 + bz;
     const int n = (D == 1) ? ny : nz;
     const int ntr = (D == 1) ? nz : ny;
     const bool col = (i0 + l <= nx) && (tr <= ntr);

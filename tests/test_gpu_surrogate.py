@@ -33,7 +33,7 @@ def dom():
 
 @pytest.fixture(scope="module")
 def trained(dom):
-    el, vl = dom.mlp_train(nsamples=1 << 21, nval=1 << 16, seed=3, batch=8192, epochs=4, lr=3e-3, lr_min=1e-5)
+    el, vl = dom.mlp_train(nsamples=1 << 22, nval=1 << 16, seed=3, batch=8192, epochs=6, lr=3e-3, lr_min=1e-5)
     print("train loss", el, "held-out", vl)
     return dom.mlp_get_weights(), el, vl
 
@@ -99,7 +99,7 @@ def test_surrogate_solve_accurate_and_faster(dom, trained):
     Ps, fl, _ = dom.cool_solve(P0, nn, dt, rho, gi, 1)
     acc = fl == 0
     print("fallback fraction", 1 - acc.mean())
-    assert acc.mean() > 0.5
+    assert acc.mean() > 0.8
     assert np.all(Ps[~acc] == Pi[~acc])
     idx = np.flatnonzero(acc)[:400]
     for k in idx:
@@ -121,15 +121,32 @@ def test_surrogate_solve_accurate_and_faster(dom, trained):
     assert ms_sur < ms_imc
 
 
+def test_network_gain_over_newton_alone(dom, trained):
+    """The same refinement started from T_old (a network that outputs 0)
+    falls back to IMC far more often than when started from the trained
+    network's prediction."""
+    w, _, _ = trained
+    P0, nn, dt, rho, gi = _stiff_cells(1 << 16, 4)
+    dom.mlp_set_weights(w)
+    _, fl_net, _ = dom.cool_solve(P0, nn, dt, rho, gi, 1)
+    dom.mlp_set_weights(np.zeros(mcf.MLP_NPARAM, np.float32))
+    _, fl_0, _ = dom.cool_solve(P0, nn, dt, rho, gi, 1)
+    dom.mlp_set_weights(w)
+    print("fallback with network", fl_net.mean(), "without", fl_0.mean())
+    assert fl_0.mean() > 2.0 * fl_net.mean() + 0.05
+
+
 def test_step_with_surrogate_close_to_plain(trained):
     w, _, _ = trained
-    # dense gas in the thermally unstable range: most cells take the
-    # backward-Euler branch of SOURCE
+    # the converging flow at 1e4 times the density and pressure (n = 1e4,
+    # T = 6000 K): the cooling time is shorter than the dt floor of Stblty
+    # (2.5e-4), so most cells take the backward-Euler branch of SOURCE
     p = ics.converging_flow_input(32)
     st = ics.converging_flow_state(p)
-    st[0] *= 30.0
+    st[0] *= 1.0e4
+    st[4] *= 1.0e4
     for s in range(8, 16):
-        st[s] *= 30.0
+        st[s] *= 1.0e4
     a = mcf.Domain(p, nsplit=(1, 1, 1))
     b = mcf.Domain(p, nsplit=(1, 1, 1), cooling=mcf.COOL_SURROGATE)
     a.upload(st)
