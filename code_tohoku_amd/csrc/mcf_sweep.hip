@@ -188,29 +188,77 @@ template <class R> __device__ __forceinline__ R right_state_pos(double q, R depe
 // ---------------------------------------------------------------------------
 // RHS
 // ---------------------------------------------------------------------------
+// Occupancy targets (waves per SIMD) of RHS and MOC: 5 lets two tiles share
+// a CU; the FP64 versions hold more live values and stay at 4, where they do
+// not spill.
+template <class R> constexpr int stage_waves() { return Ar<R>::strict ? 4 : 5; }
+
+// Every global load of the three phases is issued before the first barrier
+// (the cell's stencil, its species and those of the right neighbour, the
+// remaining velocity and field components), so one memory latency covers
+// the whole kernel.
 template <int D, class R>
-__global__ __launch_bounds__(D == 0 ? 576 : 640) void k_rhs(SweepIO A, Geo g, double dt) {
+__global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_rhs(SweepIO A, Geo g, double dt) {
   constexpr int NC = ncell_slots<D>(), NF = nface_slots<D>();
+  constexpr int VN = D + 1, VT1 = (D + 1) % 3 + 1, VT2 = (D + 2) % 3 + 1;
+  constexpr int BT1 = (D + 1) % 3, BT2 = (D + 2) % 3;
   __shared__ R sQ[8][NC];       // QL(rho, v_n, p, |B_t|^2) and QR(...) per cell
   __shared__ R sPa[NF], sVa[NF];
   const Roles t = roles<D>(g);
   const int s = g.st[D];
+  const int q = t.q;
+  const int c = ix3s(g, t.i, t.j, t.k);
   const double *dxe = A.dxe, *xe = A.xe;
-  const double *U1 = A.U[0], *UVN = A.U[A.VN], *U5 = A.U[4];
-  const double *B1 = A.bcc[A.BT1], *B2 = A.bcc[A.BT2];
+  const double *U1 = A.U[0], *UVN = A.U[VN], *U5 = A.U[4];
+  const double *B1 = A.bcc[BT1], *B2 = A.bcc[BT2];
+  const double *const *sp = A.sp;
+  double rho = 0, vn = 0, p = 0, b1 = 0, b2 = 0;
+  double r_m = 0, r_p = 0, v_m = 0, v_p = 0, p_m = 0, p_p = 0, a_m = 0, a_p = 0, c_m = 0, c_p = 0;
+  double dxm_d = 1, dx0_d = 1, dxp_d = 1;
+  double spc[8], spn[5];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) spc[f] = 0.0;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) spn[f] = 0.0;
+  double vt1 = 0, vt2 = 0, bnc = 0, xq0 = 0, xq1 = 0;
   if (t.c_ok) {
-    const int q = t.q;
-    const int c = ix3s(g, t.i, t.j, t.k);
-    const double rho = U1[c], vn = UVN[c], p = U5[c], b1 = B1[c], b2 = B2[c];
-    const R dxm = (R)dxe[q], dx0 = (R)dxe[q + 1], dxp = (R)dxe[q + 2];
+    rho = U1[c]; vn = UVN[c]; p = U5[c]; b1 = B1[c]; b2 = B2[c];
+    r_m = U1[c - s]; r_p = U1[c + s]; v_m = UVN[c - s]; v_p = UVN[c + s];
+    p_m = U5[c - s]; p_p = U5[c + s]; a_m = B1[c - s]; a_p = B1[c + s]; c_m = B2[c - s]; c_p = B2[c + s];
+    dxm_d = dxe[q]; dx0_d = dxe[q + 1]; dxp_d = dxe[q + 2];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) spc[f] = sp[f][c];
+  }
+  if (t.f_ok) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) spn[f] = sp[f][c + s];
+    xq1 = xe[q + 1];
+  }
+  if (t.u_ok) {
+#pragma unroll
+    for (int f = 5; f < 8; ++f) spc[f] = sp[f][c];
+    vt1 = A.U[VT1][c]; vt2 = A.U[VT2][c]; bnc = A.bcc[D][c];
+    xq0 = xe[q];
+  }
+  if (t.c_ok) {
+    const R dxm = (R)dxm_d, dx0 = (R)dx0_d, dxp = (R)dxp_d;
     const bool mm = use_minmod_r<R>(p, rho);
-    const R g1 = vslope<R>(U1[c - s], rho, U1[c + s], dxm, dx0, dxp, mm);
-    const R gv = vslope<R>(UVN[c - s], vn, UVN[c + s], dxm, dx0, dxp, mm);
-    const R g5 = vslope<R>(U5[c - s], p, U5[c + s], dxm, dx0, dxp, mm);
-    const R gA = vslope<R>(B1[c - s], b1, B1[c + s], dxm, dx0, dxp, mm);
-    const R gB = vslope<R>(B2[c - s], b2, B2[c + s], dxm, dx0, dxp, mm);
+    const R g1 = vslope<R>(r_m, rho, r_p, dxm, dx0, dxp, mm);
+    const R gv = vslope<R>(v_m, vn, v_p, dxm, dx0, dxp, mm);
+    const R g5 = vslope<R>(p_m, p, p_p, dxm, dx0, dxp, mm);
+    const R gA = vslope<R>(a_m, b1, a_p, dxm, dx0, dxp, mm);
+    const R gB = vslope<R>(c_m, b2, c_p, dxm, dx0, dxp, mm);
     // fast magnetosonic speed and the traced half-width (M3:1380-1388)
-    const R gamma = gam_of_r<R>(A.sp, c);
+    R gamma;
+    if constexpr (Ar<R>::strict) {
+      double s4 = spc[0] + spc[1] + spc[3] + spc[4];
+      double gg = 3.0 * s4 + 5.0 * spc[2];
+      gamma = (5.0 * s4 + 7.0 * spc[2]) / gg;
+    } else {
+      float s4 = (float)(spc[0] + spc[1] + spc[3] + spc[4]);
+      float h2 = (float)spc[2];
+      gamma = Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
+    }
     const R cm = Ar<R>::sqrt_(Ar<R>::div(gamma * (R)p + (R)(b1 * b1) + (R)(b2 * b2), (R)rho));
     R depend = (R)0.5 * (dx0 - cm * (R)dt) * (R)A.facdep;
     depend = rmax((R)0.0, depend);
@@ -228,21 +276,18 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640) void k_rhs(SweepIO A, Geo g, do
   }
   __syncthreads();
   if (t.f_ok) {
-    const int q = t.q;
-    const int c = ix3s(g, t.i, t.j, t.k), cp = c + s;
-    const double *const *sp = A.sp;
     // interface gamma from the mean species (M3:1440-1446)
     R gm;
     if constexpr (Ar<R>::strict) {
-      double ndHm = 0.5 * (sp[0][c] + sp[0][cp]);
-      double ndpm = 0.5 * (sp[1][c] + sp[1][cp]);
-      double ndHem = 0.5 * (sp[3][c] + sp[3][cp]);
-      double ndHepm = 0.5 * (sp[4][c] + sp[4][cp]);
-      double ndH2m = 0.5 * (sp[2][c] + sp[2][cp]);
+      double ndHm = 0.5 * (spc[0] + spn[0]);
+      double ndpm = 0.5 * (spc[1] + spn[1]);
+      double ndHem = 0.5 * (spc[3] + spn[3]);
+      double ndHepm = 0.5 * (spc[4] + spn[4]);
+      double ndH2m = 0.5 * (spc[2] + spn[2]);
       gm = (5.0 * (ndHm + ndpm + ndHem + ndHepm) + 7.0 * ndH2m) / (3.0 * (ndHm + ndpm + ndHem + ndHepm) + 5.0 * ndH2m);
     } else {
-      float s4 = (float)(sp[0][c] + sp[0][cp] + sp[1][c] + sp[1][cp] + sp[3][c] + sp[3][cp] + sp[4][c] + sp[4][cp]);
-      float h2 = (float)(sp[2][c] + sp[2][cp]);
+      float s4 = (float)(spc[0] + spn[0] + spc[1] + spn[1] + spc[3] + spn[3] + spc[4] + spn[4]);
+      float h2 = (float)(spc[2] + spn[2]);
       gm = Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
     }
     const int a = t.cslot, b = t.rslot_c;
@@ -252,47 +297,48 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640) void k_rhs(SweepIO A, Geo g, do
     sVa[t.fslot] = Va;
     if (t.own) {
       A.va[c] = (double)Va;
-      A.xlag[c] = xe[q + 1] + (double)Va * dt;
+      A.xlag[c] = xq1 + (double)Va * dt;
     }
   }
   __syncthreads();
   if (!t.u_ok) return;
   // Lagrangian update of the cell (M3:1448-1466)
-  const int q = t.q;
-  const int c = ix3s(g, t.i, t.j, t.k);
-  const double rho = U1[c];
-  const double vx = A.U[1][c], vy = A.U[2][c], vz = A.U[3][c];
-  const double bx = A.bcc[0][c], by = A.bcc[1][c], bz = A.bcc[2][c];
-  const double dxq = dxe[q + 1];
+  double vv[3], bb[3];
+  vv[D] = vn; vv[(D + 1) % 3] = vt1; vv[(D + 2) % 3] = vt2;
+  bb[D] = bnc; bb[BT1] = b1; bb[BT2] = b2;
+  const double vx = vv[0], vy = vv[1], vz = vv[2];
+  const double bx = bb[0], by = bb[1], bz = bb[2];
+  const double dxq = dx0_d;
   const double dm = dxq * rho;
   const double pa = (double)sPa[t.rslot], pam = (double)sPa[t.lslot];
   const double va = (double)sVa[t.rslot], vam = (double)sVa[t.lslot];
-  const double xl = xe[q + 1] + va * dt, xlm = xe[q] + vam * dt;
+  const double xl = xq1 + va * dt, xlm = xq0 + vam * dt;
   const double dxl = xl - xlm;
   if constexpr (Ar<R>::strict) {
-    double gi = gammi1_of_s(A.sp, c);
-    double e = A.U[4][c] / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
+    double s4 = spc[0] + spc[1] + spc[3] + spc[4];
+    double gg = 3.0 * s4 + 5.0 * spc[2];
+    double gi = (2.0 * s4 + 2.0 * spc[2]) / gg;
+    double e = p / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
     A.Uo[0][c] = dm / dxl;
-    A.Uo[A.VN][c] = A.U[A.VN][c] - dt / dm * (pa - pam);
+    A.Uo[VN][c] = vn - dt / dm * (pa - pam);
     A.u5e[c] = e - dt / dm * (pa * va - pam * vam);
-    A.blgo[0][c] = B1[c] * dxq / dxl;
-    A.blgo[1][c] = B2[c] * dxq / dxl;
+    A.blgo[0][c] = b1 * dxq / dxl;
+    A.blgo[1][c] = b2 * dxq / dxl;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) A.spo[f][c] = dxq * A.sp[f][c] / dxl;
+    for (int f = 0; f < 8; ++f) A.spo[f][c] = dxq * spc[f] / dxl;
   } else {
-    const double *const *sp = A.sp;
-    double s4 = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c];
+    double s4 = spc[0] + spc[1] + spc[3] + spc[4];
     double irho = 1.0 / rho;
-    double e = A.U[4][c] * (3.0 * s4 + 5.0 * sp[2][c]) / (2.0 * s4 + 2.0 * sp[2][c]) * irho +
+    double e = p * (3.0 * s4 + 5.0 * spc[2]) / (2.0 * s4 + 2.0 * spc[2]) * irho +
                0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) * irho;
     double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxq * idl;
     A.Uo[0][c] = dm * idl;
-    A.Uo[A.VN][c] = A.U[A.VN][c] - dtdm * (pa - pam);
+    A.Uo[VN][c] = vn - dtdm * (pa - pam);
     A.u5e[c] = e - dtdm * (pa * va - pam * vam);
-    A.blgo[0][c] = B1[c] * cmp;
-    A.blgo[1][c] = B2[c] * cmp;
+    A.blgo[0][c] = b1 * cmp;
+    A.blgo[1][c] = b2 * cmp;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) A.spo[f][c] = sp[f][c] * cmp;
+    for (int f = 0; f < 8; ++f) A.spo[f][c] = spc[f] * cmp;
   }
   A.dxlagM[c] = dxl;
 }
@@ -386,7 +432,7 @@ __device__ __forceinline__ MocV<R> moc_iface(const SweepIO &A, int c, int cp, in
 // the interior ones only when the two switch values fall on opposite sides
 // of 3; they are recomputed for that case.
 template <int D, class R>
-__global__ __launch_bounds__(D == 0 ? 576 : 640) void k_moc(SweepIO A, Geo g, double dt) {
+__global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_moc(SweepIO A, Geo g, double dt) {
   constexpr int NC = ncell_slots<D>(), NF = nface_slots<D>();
   constexpr int NSH = (5 * NC > 8 * NF) ? 5 * NC : 8 * NF;
   __shared__ R sh[NSH];
@@ -489,63 +535,76 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640) void k_moc(SweepIO A, Geo g, do
 // ---------------------------------------------------------------------------
 // REMAP + primitives
 // ---------------------------------------------------------------------------
+// The 15 remapped quantities (rho, m(3), E, Blg(2), 8 species) go through LDS
+// RG at a time, which keeps the tile's LDS small enough for several tiles
+// per CU.  Each interface loads the four cells c-1..c+2 around it before it
+// knows which of c, c+1 is upwind, so no load waits for the interface
+// velocity.
+constexpr int RG = 5;
+
+__device__ __forceinline__ const double *remap_field(const SweepIO &A, int v) {
+  return v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
+}
+
 template <int D, class R>
-__global__ __launch_bounds__(576) void k_remap(SweepIO A, Geo g, double dt) {
+__global__ __launch_bounds__(576, 7) void k_remap(SweepIO A, Geo g, double dt) {
   constexpr int NF = nface_slots<D>();
-  __shared__ double sF[15][NF];
+  __shared__ double sF[RG][NF];
   const Roles t = roles<D>(g);
   const int s = g.st[D];
-  if (t.f_ok) {
-    // upwind linear flux through the face swept by V*dt (M3:1600-1640)
-    const int c = ix3s(g, t.i, t.j, t.k);
-    const double vadt = A.va[c] * dt;
-    int u;
-    double depend;
-    bool up;
-    if (vadt <= 0.0) {
-      u = c + s;
-      depend = 0.5 * (A.dxlagM[u] + vadt) * A.facdep;
-      up = false;
-    } else {
-      u = c;
-      depend = 0.5 * (A.dxlagM[u] - vadt) * A.facdep;
-      up = true;
-    }
-    const R dxm = (R)A.dxlagM[u - s], dx0 = (R)A.dxlagM[u], dxp = (R)A.dxlagM[u + s];
-    const bool mm = use_minmod_r<R>(A.U[4][u], A.U[0][u]);
-    const R rdep = (R)depend;
-#pragma unroll
-    for (int v = 0; v < 15; ++v) {
-      const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-      const R gr = vslope<R>(Qv[u - s], Qv[u], Qv[u + s], dxm, dx0, dxp, mm);
-      double face;
-      if constexpr (Ar<R>::strict) face = up ? Qv[u] + depend * gr : Qv[u] - depend * gr;
-      else face = up ? Qv[u] + (double)(rdep * gr) : Qv[u] - (double)(rdep * gr);
-      sF[v][t.fslot] = -(vadt * face);
-    }
-  }
-  __syncthreads();
-  if (!t.u_ok) return;
-  // remapped conserved state (M3:1645-1686) and primitives (M3:1291-1321)
-  const int q = t.q;
   const int c = ix3s(g, t.i, t.j, t.k);
-  const double dxl = A.dxlagM[c];
-  const double dxE = A.xe[q + 1] - A.xe[q];
-  double Qn[15];
-  if constexpr (Ar<R>::strict) {
-#pragma unroll
-    for (int v = 0; v < 15; ++v) {
-      const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-      Qn[v] = (Qv[c] * dxl + sF[v][t.rslot] - sF[v][t.lslot]) / dxE;
-    }
-  } else {
-    const double idxE = 1.0 / dxE;
-#pragma unroll
-    for (int v = 0; v < 15; ++v) {
-      const double *Qv = v < 5 ? A.U[v] : v < 7 ? A.blg[v - 5] : A.sp[v - 7];
-      Qn[v] = (Qv[c] * dxl + sF[v][t.rslot] - sF[v][t.lslot]) * idxE;
-    }
+  // upwind linear flux through the face swept by V*dt (M3:1600-1640)
+  double vadt = 0.0, depend = 0.0, dxl = 0.0;
+  bool up = true, mm = false;
+  R dxm = (R)1, dx0 = (R)1, dxp = (R)1;
+  if (t.f_ok) {
+    vadt = A.va[c] * dt;
+    up = !(vadt <= 0.0);
+    const double w0 = A.dxlagM[c - s], w1 = A.dxlagM[c], w2 = A.dxlagM[c + s], w3 = A.dxlagM[c + 2 * s];
+    const double u4 = up ? A.U[4][c] : A.U[4][c + s], u0 = up ? A.U[0][c] : A.U[0][c + s];
+    dxl = w1;
+    if (up) depend = 0.5 * (w1 - vadt) * A.facdep;
+    else depend = 0.5 * (w2 + vadt) * A.facdep;
+    dxm = (R)(up ? w0 : w1);
+    dx0 = (R)(up ? w1 : w2);
+    dxp = (R)(up ? w2 : w3);
+    mm = use_minmod_r<R>(u4, u0);
   }
+  const R rdep = (R)depend;
+  const int q = t.q;
+  const double dxE = t.u_ok ? A.xe[q + 1] - A.xe[q] : 1.0;
+  const double bn = t.u_ok ? A.bcc[D][c] : 0.0;
+  double Qn[15];
+#pragma unroll
+  for (int g0 = 0; g0 < 15; g0 += RG) {
+    double Qc[RG];
+    if (t.f_ok) {
+#pragma unroll
+      for (int k = 0; k < RG; ++k) {
+        const double *Qv = remap_field(A, g0 + k);
+        const double q0 = Qv[c - s], q1 = Qv[c], q2 = Qv[c + s], q3 = Qv[c + 2 * s];
+        Qc[k] = q1;
+        const R gr = vslope<R>(up ? q0 : q1, up ? q1 : q2, up ? q2 : q3, dxm, dx0, dxp, mm);
+        const double qu = up ? q1 : q2;
+        double face;
+        if constexpr (Ar<R>::strict) face = up ? qu + depend * gr : qu - depend * gr;
+        else face = up ? qu + (double)(rdep * gr) : qu - (double)(rdep * gr);
+        sF[k][t.fslot] = -(vadt * face);
+      }
+    }
+    __syncthreads();
+    if (t.u_ok) {
+      // remapped conserved state (M3:1645-1686)
+#pragma unroll
+      for (int k = 0; k < RG; ++k) {
+        if constexpr (Ar<R>::strict) Qn[g0 + k] = (Qc[k] * dxl + sF[k][t.rslot] - sF[k][t.lslot]) / dxE;
+        else Qn[g0 + k] = (Qc[k] * dxl + sF[k][t.rslot] - sF[k][t.lslot]) * (1.0 / dxE);
+      }
+    }
+    __syncthreads();
+  }
+  if (!t.u_ok) return;
+  // primitives (M3:1291-1321)
   double rho = dmax(1.27 * A.rmin, Qn[0]);
   double invd = 1.0 / rho;
   double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
@@ -553,7 +612,6 @@ __global__ __launch_bounds__(576) void k_remap(SweepIO A, Geo g, double dt) {
   double gg = 3.0 * s4 + 5.0 * Qn[9];
   double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
   double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rho;
-  double bn = A.bcc[D][c];
   double emag = 0.5 * (bn * bn + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
   double p = (Qn[4] - ekin - emag) * gi;
   p = dmax(A.pmin, p);
