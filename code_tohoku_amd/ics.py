@@ -317,3 +317,85 @@ def converging_flow_state(p: Input3D, drho: float = 0.1, idum: int = 1) -> np.nd
     st[7] = p.binitz1
     _wrap_periodic(st, p.n, (False, True, True))
     return st
+
+
+def turbulence_input(n: int = 512) -> Input3D:
+    """Decaying MHD turbulence box: 10 pc periodic cube, n_tot = 10 cm^-3
+    (the WNM composition ratios of MCF3DSGincfca3.f90:156-157), T = 100 K,
+    B = 3 muG along z, CFL 0.49, KI cooling + conduction.  The reference's
+    pressure floor (pmin = 1.83e3 K cm^-3, MCF3DSGincfca3.f90:173) is above
+    n T = 1e3 K cm^-3, so the first update raises the gas to ~183 K."""
+    h = n // 2
+    base = Input3D()
+    ntot0 = base.Hini + base.pini + 2 * base.H2ini + base.Heini + base.Hepini
+    s = 10.0 / ntot0
+    bz = 3.0 * MICROGAUSS
+    p = Input3D(np1x=h, np2x=n - h, np1y=h, np2y=n - h, np1z=h, np2z=n - h,
+                ql1x=5.0, ql2x=5.0, ql1y=5.0, ql2y=5.0, ql1z=5.0, ql2z=5.0,
+                vinitx1=0.0, vinitx2=0.0, binitx1=0.0, binitx2=0.0, binity1=0.0, binity2=0.0,
+                binitz1=bz, binitz2=bz, cfl=0.49, facdep=1.0, tfinal=100.0, bcx1=3, bcx2=3, ifthrm=2,
+                Hini=base.Hini * s, pini=base.pini * s, H2ini=base.H2ini * s, Heini=base.Heini * s,
+                Hepini=base.Hepini * s, Cini=base.Cini * s, COini=base.COini * s, Cpini=base.Cpini * s)
+    nsum = p.pini + p.Hini + p.H2ini + p.Heini + p.Hepini
+    p.pinit_over_kb = nsum * 100.0
+    return p
+
+
+def solenoidal_velocity(n: int, kmin: float = 1.0, kmax: float = 4.0, seed: int = 1) -> np.ndarray:
+    """Divergence-free random field on an n^3 periodic grid, (3, n, n, n) in
+    (z, y, x) index order: Gaussian Fourier amplitudes with flat power on
+    kmin <= |k| <= kmax (k in units of the box wavenumber), the component
+    along k removed, zero mean, unit rms speed."""
+    try:
+        import scipy.fft as sfft
+        irfftn = lambda a: sfft.irfftn(a, s=(n, n, n), workers=-1)
+    except ImportError:  # pragma: no cover
+        irfftn = lambda a: np.fft.irfftn(a, s=(n, n, n))
+    rng = np.random.default_rng(seed)
+    kz = np.fft.fftfreq(n, 1.0 / n)[:, None, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :, None]
+    kx = np.fft.rfftfreq(n, 1.0 / n)[None, None, :]
+    k2 = kx * kx + ky * ky + kz * kz
+    shell = (k2 >= kmin * kmin) & (k2 <= kmax * kmax)
+    idx = np.nonzero(shell)
+    m = len(idx[0])
+    a = rng.normal(size=(3, m)) + 1j * rng.normal(size=(3, m))
+    kv = np.stack([np.broadcast_to(kx, shell.shape)[idx], np.broadcast_to(ky, shell.shape)[idx],
+                   np.broadcast_to(kz, shell.shape)[idx]])
+    a -= kv * (np.sum(kv * a, axis=0) / np.sum(kv * kv, axis=0))
+    v = np.empty((3, n, n, n))
+    for c in range(3):
+        F = np.zeros(shell.shape, dtype=np.complex128)
+        F[idx] = a[c]
+        v[c] = irfftn(F)
+    v -= v.mean(axis=(1, 2, 3), keepdims=True)
+    v /= np.sqrt(np.mean(np.sum(v * v, axis=0)))
+    return v
+
+
+def turbulence_state(p: Input3D, mach: float = 5.0, seed: int = 1) -> np.ndarray:
+    """Uniform density, pressure and B = Bz; solenoidal velocity with k in
+    [1, 4] scaled to rms Mach ``mach`` with respect to the adiabatic sound
+    speed."""
+    nx, ny, nz = p.n
+    assert nx == ny == nz, "cubic grids only"
+    rho0 = p.Hini + p.pini + 2.0 * p.H2ini + 4.0 * p.Heini + 4.0 * p.Hepini
+    s4 = p.Hini + p.pini + p.Heini + p.Hepini
+    gamma = (5.0 * s4 + 7.0 * p.H2ini) / (3.0 * s4 + 5.0 * p.H2ini)
+    p0 = p.pinit_over_kb * KB * 1.0e-3
+    cs = np.sqrt(gamma * p0 / rho0)
+    v = solenoidal_velocity(nx, seed=seed) * (mach * cs)
+    st = empty_state(p.n)
+    st[0] = rho0
+    comp = np.array([p.Hini, p.pini, p.H2ini, p.Heini, p.Hepini, p.Cini, p.COini, p.Cpini])
+    for s_ in range(8):
+        st[8 + s_] = comp[s_]
+    st[4] = p0
+    I = slice(2, nx + 2)
+    for c in range(3):
+        st[1 + c][I, I, I] = v[c]
+    st[5] = p.binitx1
+    st[6] = p.binity1
+    st[7] = p.binitz1
+    _wrap_periodic(st, p.n, (True, True, True))
+    return st

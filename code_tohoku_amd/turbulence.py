@@ -1,0 +1,148 @@
+"""Decaying supersonic MHD turbulence with KI cooling: the mixed-precision
+path with the cooling surrogate against the plain FP64 path.
+
+Both domains start from the same state (ics.turbulence_state: periodic
+10 pc box, n = 10 cm^-3, T = 100 K, solenoidal velocity with k in [1, 4] at
+rms Mach 5, B = 3 muG along z) and advance with the FP64 run's dt sequence.
+Reported: the density PDF of both runs and their distance (L1 and
+Kolmogorov-Smirnov), the shell-averaged kinetic and magnetic energy spectra
+and their relative differences, per-cell relative L1 / L-infinity errors of
+rho, p and B, max|div B|, the surrogate's cell counts, and the time per step
+of each path at this size.
+
+    python -m code_tohoku_amd.turbulence [--n 512] [--steps 500] [--out FILE]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import ics, mcf, surrogate
+from .accuracy import divb, errors
+
+try:
+    import scipy.fft as _sfft
+
+    def _rfftn(a):
+        return _sfft.rfftn(a, workers=-1)
+except ImportError:  # pragma: no cover
+    def _rfftn(a):
+        return np.fft.rfftn(a)
+
+
+def shell_spectrum(fields) -> np.ndarray:
+    """E(k) = 1/2 sum over the shell k-1/2 <= |k'| < k+1/2 of |f_hat(k')|^2 / N^6
+    summed over the given (n, n, n) fields (interior cells, periodic)."""
+    n = fields[0].shape[0]
+    kz = np.fft.fftfreq(n, 1.0 / n)[:, None, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :, None]
+    kx = np.fft.rfftfreq(n, 1.0 / n)[None, None, :]
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    w = np.full(kk.shape, 2.0)  # the rfft half-space counts each mode twice ...
+    w[:, :, 0] = 1.0            # ... except the kx = 0 plane
+    if n % 2 == 0:
+        w[:, :, -1] = 1.0       # and the Nyquist plane
+    ib = np.rint(kk).astype(np.int64).ravel()
+    nb = n // 2 + 1
+    E = np.zeros(nb)
+    for f in fields:
+        F = _rfftn(f - f.mean())
+        pw = (np.abs(F) ** 2 * w).ravel() / float(n) ** 6
+        E += 0.5 * np.bincount(ib, weights=pw, minlength=ib.max() + 1)[:nb]
+    return E
+
+
+def density_pdf(rho: np.ndarray, rho0: float, bins: np.ndarray) -> np.ndarray:
+    h, _ = np.histogram(np.log10(rho / rho0), bins=bins)
+    return h / (h.sum() * np.diff(bins))
+
+
+def run(n: int = 512, steps: int = 500, log=print) -> dict:
+    p = ics.turbulence_input(n)
+    st = ics.turbulence_state(p)
+    rho0 = float(st[0, 2, 2, 2])
+    nsplit = (max(1, n // 64),) * 3
+    w = surrogate.load_weights()
+    a = mcf.Domain(p, nsplit=nsplit, precision=mcf.PREC_FP64)
+    b = mcf.Domain(p, nsplit=nsplit, precision=mcf.PREC_MIXED, cooling=mcf.COOL_SURROGATE)
+    a.upload(st)
+    b.upload(st)
+    b.mlp_set_weights(w)
+    del st
+    t0 = time.perf_counter()
+    for k in range(steps):
+        dt, _tag = a.timestep(1e30)
+        a.step(dt)
+        b.step(dt)
+        if (k + 1) % 50 == 0:
+            b.sync()
+            log(f"[turb] step {k + 1}/{steps} t={a.time[0]:.5f} dt={dt:.3e} {time.perf_counter() - t0:.1f} s")
+    b.sync()
+    res = {"grid": n, "steps": steps, "time": a.time[0], "rho0": rho0}
+    A = a.download()
+    B = b.download()
+    res["per_cell"] = errors(A, B, n)
+    res["max_divB"] = {"fp64": divb(A, p), "mixed+surrogate": divb(B, p)}
+    I = slice(2, n + 2)
+    bins = np.linspace(-3.0, 3.0, 121)
+    pa = density_pdf(A[0][I, I, I], rho0, bins)
+    pb = density_pdf(B[0][I, I, I], rho0, bins)
+    dxb = np.diff(bins)
+    ca, cb = np.cumsum(pa * dxb), np.cumsum(pb * dxb)
+    res["density_pdf"] = {"bins_log10_rho_over_rho0": bins.tolist(), "fp64": pa.tolist(),
+                          "mixed+surrogate": pb.tolist(), "L1": float(np.sum(np.abs(pa - pb) * dxb)),
+                          "KS": float(np.max(np.abs(ca - cb))),
+                          "max_log10_rho": [float(np.log10(A[0][I, I, I].max() / rho0)),
+                                            float(np.log10(B[0][I, I, I].max() / rho0))]}
+    # kinetic (velocity) spectra
+    ka = shell_spectrum([A[c][I, I, I] for c in (1, 2, 3)])
+    kb = shell_spectrum([B[c][I, I, I] for c in (1, 2, 3)])
+    del A, B
+    ma = shell_spectrum([x[I, I, I] for x in a.bcc()])
+    mb = shell_spectrum([x[I, I, I] for x in b.bcc()])
+    kmax = n // 4
+    rel = lambda x, y: [float(abs(y[k] - x[k]) / x[k]) if x[k] > 0 else 0.0 for k in range(1, kmax + 1)]
+    res["spectra"] = {"k": list(range(1, kmax + 1)),
+                      "kinetic_fp64": ka[1:kmax + 1].tolist(), "kinetic_mixed": kb[1:kmax + 1].tolist(),
+                      "magnetic_fp64": ma[1:kmax + 1].tolist(), "magnetic_mixed": mb[1:kmax + 1].tolist(),
+                      "kinetic_max_rel_diff": max(rel(ka, kb)), "magnetic_max_rel_diff": max(rel(ma, mb))}
+    nst, nfb = b.cool_stats()
+    res["surrogate_cells"] = {"backward_euler": nst, "imc_fallbacks": nfb,
+                              "fallback_fraction": (nfb / nst) if nst else None}
+    # throughput of each path at this size (10 more steps, each domain alone)
+    for name, d in (("fp64", a), ("mixed+surrogate", b)):
+        d.run(1, tsave=1e30)
+        d.sync()
+        t1 = time.perf_counter()
+        d.run(10, tsave=1e30)
+        d.sync()
+        el = time.perf_counter() - t1
+        res.setdefault("throughput", {})[name] = {"ms_per_step": 100.0 * el, "zone_updates_per_s": n ** 3 * 10 / el}
+    a.close()
+    b.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--out", default=None)
+    o = ap.parse_args()
+    res = run(o.n, o.steps, log=lambda s: print(s, file=sys.stderr, flush=True))
+    txt = json.dumps(res, indent=1)
+    if o.out:
+        with open(o.out, "w") as f:
+            f.write(txt)
+    summary = {k: v for k, v in res.items() if k not in ("density_pdf", "spectra")}
+    summary["density_pdf"] = {k: res["density_pdf"][k] for k in ("L1", "KS", "max_log10_rho")}
+    summary["spectra"] = {k: res["spectra"][k] for k in ("kinetic_max_rel_diff", "magnetic_max_rel_diff")}
+    print(json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()
