@@ -46,8 +46,8 @@ struct mcf_ctx {
   double *Ub[5]{}, *spb[8]{};
   // sweep scratch
   double *u5e = nullptr, *blg[2]{}, *blgb[2]{}, *xlag = nullptr, *dxlagM = nullptr, *va = nullptr;
-  // CT / source scratch (aliases the sweep scratch: the stages never overlap)
-  double *vfc[3]{}, *dq[4]{}, *emf[3]{}, *Tn = nullptr, *Pn = nullptr;
+  // CT scratch; SOURCE keeps its stiff-cell queues in dq (SOURCE and CT never overlap)
+  double *vfc[3]{}, *dq[4]{}, *emf[3]{};
   double *divv = nullptr, *red = nullptr;
   double *gx[3]{}, *gdx[3]{};
   std::vector<double> hx[3], hdx[3];
@@ -372,7 +372,18 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
   CHK(fillU(x, 1 << d, 2, 1, lr, 3));
   CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
   A = sweep_io(x, d);
-  SWEEP_LAUNCH(k_rhs, d, true, 33, A, x->g, dt);
+  if (d == 0 || x->mixed) {
+    SWEEP_LAUNCH(k_rhs, d, true, 33, A, x->g, dt);
+  } else {
+    // FP64 along y/z: one thread per x column marching along the sweep axis
+    // (measured faster than the LDS tiles there; the mixed kernels are faster
+    // as tiles, where more of them fit per CU)
+    const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
+    dim3 gr((unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4), (unsigned)((n + RHS_COL_SEG - 1) / RHS_COL_SEG));
+    const double by = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * 33;
+    if (d == 1) KLAUNCH_N("k_rhs", (k_rhs_col<1, double>), gr, dim3(256), by, A, x->g, dt);
+    else KLAUNCH_N("k_rhs", (k_rhs_col<2, double>), gr, dim3(256), by, A, x->g, dt);
+  }
   swapU(x, 0); swapU(x, A.VN); swapSp(x); swapBlg(x);
   // MOC (fills M3:1950-1976): reads rho, v_t(2), Blg(2), p, e, widths, Bn;
   // writes momenta(3), E, Blg(2)
@@ -452,10 +463,8 @@ static int do_ct(mcf_ctx *x, double dt) {
 static int do_source(mcf_ctx *x, double dt) {
   if (x->cfg.ifthrm != 2) return MCF_OK;
   SrcArgs A{};
-  A.U5 = x->U[F_P];
   A.rho = x->U[F_RHO];
   for (int s = 0; s < 8; ++s) A.sp[s] = x->sp[s];
-  A.Tn = x->Tn; A.Pn = x->Pn;
   for (int d = 0; d < 3; ++d) A.gdx[d] = x->gdx[d];
   A.pmin = x->pmin; A.pmax = x->pmax;
   if (x->cfg.cooling == MCF_COOL_SURROGATE) {
@@ -473,25 +482,31 @@ static int do_source(mcf_ctx *x, double dt) {
   int l15[2] = {1, 5}, l5[1] = {5};
   CHK(fillU(x, 7, 1, 1, l15, 2));
   CHK(fillOT(x, 7, 1, OT_SPECIES));
-  Box bt = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1), bc = mkbox(1, nx, 1, ny, 1, nz);
-  KLAUNCH(k_src_temp, boxsize(bt), 8, A, x->g, bt, 1);
+  // stage 0: T and P_old from P, predictor into the second pressure buffer
+  double *P = x->U[F_P], *Q = x->Ub[F_P];
+  Box bc = mkbox(1, nx, 1, ny, 1, nz), bg = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1);
+  A.Tp = P; A.Pn = P; A.Pout = Q;
   if (A.sur) {
     KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
-    KLAUNCH_R(k_src_cell, true, boxsize(bc), 10, A, x->g, bc, dt, 0);
+    KLAUNCH_R(k_src_cell, true, boxsize(bc), 9, A, x->g, bc, dt, 0);
     KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 0);
     KLAUNCH_G(k_src_fallback, dim3(256), dim3(256), 0, A, dt, 0);
   } else {
-    KLAUNCH_R(k_src_cell, false, boxsize(bc), 10, A, x->g, bc, dt, 0);
+    KLAUNCH_R(k_src_cell, false, boxsize(bc), 9, A, x->g, bc, dt, 0);
   }
+  // ghosts of the predictor (the fill works on U[F_P])
+  x->U[F_P] = Q; x->Ub[F_P] = P;
   CHK(fillU(x, 7, 1, 1, l5, 1));
-  KLAUNCH(k_src_temp, boxsize(bt), 7, A, x->g, bt, 0);
+  x->U[F_P] = P; x->Ub[F_P] = Q;
+  // stage 1: T from the predictor, corrector from P_old written over P
+  A.Tp = Q; A.Pn = P; A.Pout = P;
   if (A.sur) {
     KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
-    KLAUNCH_R(k_src_cell, true, boxsize(bc), 10, A, x->g, bc, dt, 1);
+    KLAUNCH_R(k_src_cell, true, boxsize(bg), 9, A, x->g, bg, dt, 1);
     KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 1);
     KLAUNCH_G(k_src_fallback, dim3(256), dim3(256), 0, A, dt, 1);
   } else {
-    KLAUNCH_R(k_src_cell, false, boxsize(bc), 10, A, x->g, bc, dt, 1);
+    KLAUNCH_R(k_src_cell, false, boxsize(bg), 9, A, x->g, bg, dt, 1);
   }
   return MCF_OK;
 }
@@ -543,7 +558,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   hipError_t e = hipGetDevice(&x->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { g_err = std::string("HIP init: ") + hipGetErrorString(e); delete x; return MCF_E_HIP; }
-  const int nstate = 19, nbuf = 13, nsweep = 8, nscratch = 12;
+  const int nstate = 19, nbuf = 13, nsweep = 8, nscratch = 10;
   size_t nc = x->ncell;
   size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
   size_t ngrid = 0;
@@ -563,7 +578,6 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   for (int c = 0; c < 3; ++c) { x->vfc[c] = p; p += nc; }
   for (int t = 0; t < 4; ++t) { x->dq[t] = p; p += nc; }
   for (int c = 0; c < 3; ++c) { x->emf[c] = p; p += nc; }
-  x->Tn = p; p += nc; x->Pn = p; p += nc;
   x->divv = p; p += ndivv;
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
   x->red = p; p += 16 + 2 * DT_BLOCKS;

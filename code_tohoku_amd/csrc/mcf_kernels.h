@@ -58,6 +58,8 @@ struct SweepIO {
 
 // R = double: strict FP64; R = float: mixed precision
 template <int D, class R> __global__ void k_rhs(SweepIO A, Geo g, double dt);
+template <int D, class R> __global__ void k_rhs_col(SweepIO A, Geo g, double dt); // D = 1, 2; R = double
+constexpr int RHS_COL_SEG = 32;
 template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);
 
@@ -72,10 +74,11 @@ struct CtArgs {
 };
 
 struct SrcArgs {
-  double *U5;
+  const double *Tp;   // pressure the stage computes T from
+  double *Pout;       // result of the stage
+  double *Pn;         // pressure before SOURCE (P_old of both stages)
   const double *rho;
   const double *sp[8];
-  double *Tn, *Pn;
   const double *gdx[3];
   double pmin, pmax;
   // cooling surrogate (sur != 0): packed MLP operands, FP32 weights and the
@@ -111,7 +114,6 @@ template <class R> __global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orie
 template <class R> __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
-__global__ void k_src_temp(SrcArgs A, Geo g, Box b, int savep);
 template <bool SUR> __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
 __global__ void k_src_fallback(SrcArgs A, double dt, int stage);

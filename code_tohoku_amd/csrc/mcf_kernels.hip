@@ -280,34 +280,45 @@ __global__ void k_cc4(CtArgs A, Geo g, Box b) {
 // ---------------------------------------------------------------------------
 // SOURCE (M3:2448-2572), KI cooling + saturated conduction
 // ---------------------------------------------------------------------------
-__global__ void k_src_temp(SrcArgs A, Geo g, Box b, int savep) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  int c = ix3(g, i, j, k);
+// T = P/(kb n) of a cell, from the pressure array the stage reads T from
+__device__ __forceinline__ double src_T(const SrcArgs &A, int c) {
   const double *const *sp = A.sp;
-  A.Tn[c] = A.U5[c] / (KB * (sp[1][c] + sp[0][c] + sp[2][c] + sp[3][c] + sp[4][c]));
-  if (savep) A.Pn[c] = A.U5[c];
+  return A.Tp[c] / (KB * (sp[1][c] + sp[0][c] + sp[2][c] + sp[3][c] + sp[4][c]));
 }
 
-__device__ __forceinline__ double qflux(const double *Tn, int c, int s, double dxi, double dxp) {
-  double rt = sqrt((dxi * Tn[c + s] + dxp * Tn[c]) / (dxi + dxp));
-  return KCOND * dmin(rt, 3.873) * (Tn[c + s] - Tn[c]) * 2.0 / (dxi + dxp);
+__device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, double dxp) {
+  double rt = sqrt((dxi * Tcs + dxp * Tc) / (dxi + dxp));
+  return KCOND * dmin(rt, 3.873) * (Tcs - Tc) * 2.0 / (dxi + dxp);
 }
 
-// One stage of SOURCE's thermal update (M3:2461-2572).  With SUR the
-// backward-Euler cells (dt > 0.2*t_cool) are not solved here: they are
-// queued with their conduction increment for k_src_stiff.
+// One stage of SOURCE's thermal update (M3:2461-2572), out of place.
+// Stage 0 (predictor, dt/2): T from the pressure P (= A.Pn), result to
+// A.Pout.  Stage 1 (corrector, dt): T from the predictor (A.Tp), P_old =
+// A.Pn, result written over A.Pn (= A.Pout; each cell reads only its own
+// P_old there); its threads on the first ghost layer copy the predictor's
+// ghost values across, so the ghosts end as the reference leaves them.
+// The temperature of a cell and of its six neighbours is computed on the
+// fly.  With SUR the backward-Euler cells (dt > 0.2*t_cool) are not solved
+// here: they are queued with their conduction increment for k_src_stiff.
 template <bool SUR>
 __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
   int i, j, k;
-  const bool valid = box_ijk(b, i, j, k);
+  bool valid = box_ijk(b, i, j, k);
+  if (valid && stage == 1 && (i == 0 || j == 0 || k == 0 || i > g.n[0] || j > g.n[1] || k > g.n[2])) {
+    const int cg = ix3(g, i, j, k);
+    A.Pout[cg] = A.Tp[cg];
+    valid = false;
+  }
   if (!SUR && !valid) return;
   if (!valid) { i = j = k = 1; }
   int c = ix3(g, i, j, k);
   const double *const *sp = A.sp;
-  const double *Tn = A.Tn;
+  const int sy = g.st[1], sz = g.st[2];
+  const double T = src_T(A, c);
+  const double Txp = src_T(A, c + 1), Txm = src_T(A, c - 1);
+  const double Typ = src_T(A, c + sy), Tym = src_T(A, c - sy);
+  const double Tzp = src_T(A, c + sz), Tzm = src_T(A, c - sz);
   double rho = A.rho[c];
-  double T = Tn[c];
   double CooL = fcool_ki(rho, T);
   double gi = gammi1_of(sp, c);
   double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
@@ -315,14 +326,13 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
   const bool expl = dt <= 0.2 * Pn / (gi * fabs(CooL));
   double u = 0.0;
   if (stage == 0) {
-    u = A.U5[c];
+    u = Pn;
     if (expl) u = u - gi * CooL * dt * 0.5;
     else if (!SUR) u = imc(u, nsum, dt * 0.5, rho, gi);
   } else {
     if (expl) u = Pn - gi * CooL * dt;
     else if (!SUR) {
       Pn = imc(Pn, nsum, dt, rho, gi);
-      A.Pn[c] = Pn;
       u = Pn;
     }
   }
@@ -330,13 +340,12 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
   double tcd = gi * KCOND * sqrt(T);
   tcd = 0.49 * nsum * KB / tcd * (dxi * dxi);
   tcd = copysign(0.5, tcd - dt);
-  int sy = g.st[1], sz = g.st[2];
-  double qxp = qflux(Tn, c, 1, dxi, A.gdx[0][i + 2]);
-  double qxm = qflux(Tn, c - 1, 1, A.gdx[0][i], dxi);
-  double qyp = qflux(Tn, c, sy, dyj, A.gdx[1][j + 2]);
-  double qym = qflux(Tn, c - sy, sy, A.gdx[1][j], dyj);
-  double qzp = qflux(Tn, c, sz, dzk, A.gdx[2][k + 2]);
-  double qzm = qflux(Tn, c - sz, sz, A.gdx[2][k], dzk);
+  double qxp = qflux(T, Txp, dxi, A.gdx[0][i + 2]);
+  double qxm = qflux(Txm, T, A.gdx[0][i], dxi);
+  double qyp = qflux(T, Typ, dyj, A.gdx[1][j + 2]);
+  double qym = qflux(Tym, T, A.gdx[1][j], dyj);
+  double qzp = qflux(T, Tzp, dzk, A.gdx[2][k + 2]);
+  double qzm = qflux(Tzm, T, A.gdx[2][k], dzk);
   double div = (qxp - qxm) / dxi + (qyp - qym) / dyj + (qzp - qzm) / dzk;
   double dc;
   if (stage == 0) dc = (tcd + 0.5) * gi * dt * 0.5 * div;
@@ -361,7 +370,7 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
   u = u + dc;
   u = dmax(u, A.pmin);
   u = dmin(u, A.pmax);
-  A.U5[c] = u;
+  A.Pout[c] = u;
 }
 template __global__ void k_src_cell<false>(SrcArgs, Geo, Box, double, int);
 template __global__ void k_src_cell<true>(SrcArgs, Geo, Box, double, int);
@@ -371,12 +380,11 @@ template __global__ void k_src_cell<true>(SrcArgs, Geo, Box, double, int);
 // cell's conduction increment and the pressure clamps are then applied as
 // in k_src_cell.  Rejected cells go to a second queue for k_src_fallback,
 // so IMC runs on dense waves.  Waves stride over the queue.
-__device__ __forceinline__ void src_finish(const SrcArgs &A, int c, int stage, double Pi) {
-  if (stage == 1) A.Pn[c] = Pi;
+__device__ __forceinline__ void src_finish(const SrcArgs &A, int c, double Pi) {
   double u = Pi + A.qdc[c];
   u = dmax(u, A.pmin);
   u = dmin(u, A.pmax);
-  A.U5[c] = u;
+  A.Pout[c] = u;
 }
 
 __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int stage) {
@@ -392,7 +400,7 @@ __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int sta
     const double rho = A.rho[c];
     const double gi = gammi1_of(sp, c);
     const double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
-    const double P0 = stage == 0 ? A.U5[c] : A.Pn[c];
+    const double P0 = A.Pn[c];
     const double dts = stage == 0 ? dt * 0.5 : dt;
     float u0, u1, u2;
     sur_features(P0, nsum, dts, rho, gi, u0, u1, u2);
@@ -407,7 +415,7 @@ __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int sta
       fb = __shfl(fb, leader, 64);
       if (ok && fell) A.fqueue[fb + (unsigned)__popcll(mf & ((1ull << lane) - 1ull))] = c;
     }
-    if (ok && !fell) src_finish(A, c, stage, Pi);
+    if (ok && !fell) src_finish(A, c, Pi);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[0], (unsigned long long)nq);
 }
@@ -422,9 +430,8 @@ __global__ __launch_bounds__(256) void k_src_fallback(SrcArgs A, double dt, int 
     const double rho = A.rho[c];
     const double gi = gammi1_of(sp, c);
     const double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
-    const double P0 = stage == 0 ? A.U5[c] : A.Pn[c];
     const double dts = stage == 0 ? dt * 0.5 : dt;
-    src_finish(A, c, stage, imc(P0, nsum, dts, rho, gi));
+    src_finish(A, c, imc(A.Pn[c], nsum, dts, rho, gi));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[1], (unsigned long long)nf);
 }

@@ -344,6 +344,203 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_rhs(Sw
 }
 
 // ---------------------------------------------------------------------------
+// RHS along y or z: one thread per x column marching along the sweep axis
+// ---------------------------------------------------------------------------
+// Cell states of RHS phase 1 (M3:1380-1406) from the cell's stencil:
+// q[0..7] = QL(rho, v_n, p, |B_t|^2), QR(...).
+template <class R>
+__device__ __forceinline__ void rhs_states(double rho, double vn, double p, double b1, double b2, double r_m,
+                                           double r_p, double v_m, double v_p, double p_m, double p_p, double a_m,
+                                           double a_p, double c_m, double c_p, const double *spc, double dxm_d,
+                                           double dx0_d, double dxp_d, double dt, double facdep, R *q) {
+  const R dxm = (R)dxm_d, dx0 = (R)dx0_d, dxp = (R)dxp_d;
+  const bool mm = use_minmod_r<R>(p, rho);
+  const R g1 = vslope<R>(r_m, rho, r_p, dxm, dx0, dxp, mm);
+  const R gv = vslope<R>(v_m, vn, v_p, dxm, dx0, dxp, mm);
+  const R g5 = vslope<R>(p_m, p, p_p, dxm, dx0, dxp, mm);
+  const R gA = vslope<R>(a_m, b1, a_p, dxm, dx0, dxp, mm);
+  const R gB = vslope<R>(c_m, b2, c_p, dxm, dx0, dxp, mm);
+  R gamma;
+  if constexpr (Ar<R>::strict) {
+    double s4 = spc[0] + spc[1] + spc[3] + spc[4];
+    double gg = 3.0 * s4 + 5.0 * spc[2];
+    gamma = (5.0 * s4 + 7.0 * spc[2]) / gg;
+  } else {
+    float s4 = (float)(spc[0] + spc[1] + spc[3] + spc[4]);
+    float h2 = (float)spc[2];
+    gamma = Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
+  }
+  const R cm = Ar<R>::sqrt_(Ar<R>::div(gamma * (R)p + (R)(b1 * b1) + (R)(b2 * b2), (R)rho));
+  R depend = (R)0.5 * (dx0 - cm * (R)dt) * (R)facdep;
+  depend = rmax((R)0.0, depend);
+  q[0] = left_state_pos<R>(rho, depend, g1);
+  q[1] = (R)(vn + (double)(depend * gv));
+  q[2] = left_state_pos<R>(p, depend, g5);
+  R a1 = (R)(b1 + (double)(depend * gA)), a2 = (R)(b2 + (double)(depend * gB));
+  q[3] = a1 * a1 + a2 * a2;
+  q[4] = right_state_pos<R>(rho, depend, g1);
+  q[5] = (R)(vn - (double)(depend * gv));
+  q[6] = right_state_pos<R>(p, depend, g5);
+  a1 = (R)(b1 - (double)(depend * gA));
+  a2 = (R)(b2 - (double)(depend * gB));
+  q[7] = a1 * a1 + a2 * a2;
+}
+
+// RIEMAN at the interface between a left cell (species sl, states ql) and a
+// right cell (species sr, states qr) with the interface gamma of the mean
+// species (M3:1440-1446).
+template <class R>
+__device__ __forceinline__ void rhs_iface(const double *sl, const double *sr, const R *ql, const R *qr, R &Pa,
+                                          R &Va) {
+  R gm;
+  if constexpr (Ar<R>::strict) {
+    double ndHm = 0.5 * (sl[0] + sr[0]);
+    double ndpm = 0.5 * (sl[1] + sr[1]);
+    double ndHem = 0.5 * (sl[3] + sr[3]);
+    double ndHepm = 0.5 * (sl[4] + sr[4]);
+    double ndH2m = 0.5 * (sl[2] + sr[2]);
+    gm = (5.0 * (ndHm + ndpm + ndHem + ndHepm) + 7.0 * ndH2m) / (3.0 * (ndHm + ndpm + ndHem + ndHepm) + 5.0 * ndH2m);
+  } else {
+    float s4 = (float)(sl[0] + sr[0] + sl[1] + sr[1] + sl[3] + sr[3] + sl[4] + sr[4]);
+    float h2 = (float)(sl[2] + sr[2]);
+    gm = Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
+  }
+  rieman_r<R>(ql[0], ql[1], ql[2], ql[3], qr[4], qr[5], qr[6], qr[7], gm, Pa, Va);
+}
+
+constexpr int COL_SEG = RHS_COL_SEG; // positions per column segment
+
+// Column march for D = 1 (y) or 2 (z).  Thread = x column i of transverse
+// row tr, segment [p0, p1] of the sweep axis.  Each iteration loads the
+// stencil entering the window, forms the states of one cell, solves the
+// interface to its left and updates the cell left of that interface; the
+// loads of the next iteration are issued before this one's arithmetic.
+template <int D, class R>
+__global__ __launch_bounds__(256) void k_rhs_col(SweepIO A, Geo g, double dt) {
+  constexpr int VN = D + 1, VT1 = (D + 1) % 3 + 1, VT2 = (D + 2) % 3 + 1;
+  constexpr int BT1 = (D + 1) % 3, BT2 = (D + 2) % 3;
+  const int nx = g.n[0];
+  const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
+  const int i = 1 + 64 * (int)blockIdx.x + (int)(threadIdx.x & 63);
+  const int tr = 1 + 4 * (int)blockIdx.y + (int)(threadIdx.x >> 6);
+  const int p0 = 1 + COL_SEG * (int)blockIdx.z;
+  if (i > nx || tr > ntr || p0 > n) return;
+  const int p1 = min(n, p0 + COL_SEG - 1);
+  auto cell = [&](int p) { return D == 1 ? ix3s(g, i, p, tr) : ix3s(g, i, tr, p); };
+  const double *U1 = A.U[0], *UVN = A.U[VN], *U5 = A.U[4];
+  const double *B1 = A.bcc[BT1], *B2 = A.bcc[BT2];
+  const double *const *sp = A.sp;
+  const double *dxe = A.dxe, *xe = A.xe;
+  // stencil window at positions q-1, q, q+1 and the values of q+2 in flight
+  double wr[4], wv[4], wp[4], wa[4], wc[4];
+  double spw[3][5];    // species 0..4 of positions q-1, q, q+1
+  R qprev[8];          // states of cell q-1
+  R pa_prev = 0, va_prev = 0;
+  {
+    const int c0 = cell(p0 - 2), c1 = cell(p0 - 1), c2 = cell(p0), c3 = cell(p0 + 1);
+    wr[0] = U1[c0]; wr[1] = U1[c1]; wr[2] = U1[c2]; wr[3] = U1[c3];
+    wv[0] = UVN[c0]; wv[1] = UVN[c1]; wv[2] = UVN[c2]; wv[3] = UVN[c3];
+    wp[0] = U5[c0]; wp[1] = U5[c1]; wp[2] = U5[c2]; wp[3] = U5[c3];
+    wa[0] = B1[c0]; wa[1] = B1[c1]; wa[2] = B1[c2]; wa[3] = B1[c3];
+    wc[0] = B2[c0]; wc[1] = B2[c1]; wc[2] = B2[c2]; wc[3] = B2[c3];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { spw[0][f] = sp[f][c1]; spw[1][f] = sp[f][c2]; spw[2][f] = sp[f][c3]; }
+    rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
+                  wc[2], spw[0], dxe[p0 - 1], dxe[p0], dxe[p0 + 1], dt, A.facdep, qprev);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { spw[0][f] = spw[1][f]; spw[1][f] = spw[2][f]; }
+  }
+  // update inputs of the cell left of the current interface, loaded one
+  // iteration ahead
+  double vt1 = 0, vt2 = 0, bnc = 0, spx[3] = {0, 0, 0};
+  // q = position of the cell whose states are formed in this iteration;
+  // on entry the window holds q-1, q, q+1 and species of q-1, q
+  for (int q = p0; q <= p1 + 1; ++q) {
+    const int cl = cell(q - 1);
+    // issue the loads of the next iteration first
+    const int cf = cell(q + 2), cn = cell(q + 1), cq = cell(q);
+    wr[3] = U1[cf]; wv[3] = UVN[cf]; wp[3] = U5[cf]; wa[3] = B1[cf]; wc[3] = B2[cf];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) spw[2][f] = sp[f][cn];
+    double nvt1 = 0, nvt2 = 0, nbnc = 0, nspx[3] = {0, 0, 0};
+    if (q <= p1) {
+      nvt1 = A.U[VT1][cq]; nvt2 = A.U[VT2][cq]; nbnc = A.bcc[D][cq];
+      nspx[0] = sp[5][cq]; nspx[1] = sp[6][cq]; nspx[2] = sp[7][cq];
+    }
+    const double xq = xe[q], xqm = xe[q - 1], dxq = dxe[q];
+    R qcur[8];
+    rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
+                  wc[2], spw[1], dxe[q], dxe[q + 1], dxe[q + 2], dt, A.facdep, qcur);
+    // interface between q-1 and q
+    R Pa, Va;
+    rhs_iface<R>(spw[0], spw[1], qprev, qcur, Pa, Va);
+    if (q - 1 >= p0 || q - 1 == 0) {
+      A.va[cl] = (double)Va;
+      A.xlag[cl] = xq + (double)Va * dt;
+    }
+    if (q - 1 >= p0) {
+      // Lagrangian update of cell q-1 (M3:1448-1466)
+      const double rho = wr[0], vn = wv[0], p = wp[0], b1 = wa[0], b2 = wc[0];
+      double vv[3], bb[3];
+      vv[D] = vn; vv[(D + 1) % 3] = vt1; vv[(D + 2) % 3] = vt2;
+      bb[D] = bnc; bb[BT1] = b1; bb[BT2] = b2;
+      const double vx = vv[0], vy = vv[1], vz = vv[2];
+      const double bx = bb[0], by = bb[1], bz = bb[2];
+      const double dm = dxq * rho;
+      const double pa = (double)Pa, pam = (double)pa_prev, va = (double)Va, vam = (double)va_prev;
+      const double xl = xq + va * dt, xlm = xqm + vam * dt;
+      const double dxl = xl - xlm;
+      const double *spl = spw[0];
+      if constexpr (Ar<R>::strict) {
+        double s4 = spl[0] + spl[1] + spl[3] + spl[4];
+        double gg = 3.0 * s4 + 5.0 * spl[2];
+        double gi = (2.0 * s4 + 2.0 * spl[2]) / gg;
+        double e = p / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
+        A.Uo[0][cl] = dm / dxl;
+        A.Uo[VN][cl] = vn - dt / dm * (pa - pam);
+        A.u5e[cl] = e - dt / dm * (pa * va - pam * vam);
+        A.blgo[0][cl] = b1 * dxq / dxl;
+        A.blgo[1][cl] = b2 * dxq / dxl;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) A.spo[f][cl] = dxq * spl[f] / dxl;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) A.spo[5 + f][cl] = dxq * spx[f] / dxl;
+      } else {
+        double s4 = spl[0] + spl[1] + spl[3] + spl[4];
+        double irho = 1.0 / rho;
+        double e = p * (3.0 * s4 + 5.0 * spl[2]) / (2.0 * s4 + 2.0 * spl[2]) * irho +
+                   0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) * irho;
+        double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxq * idl;
+        A.Uo[0][cl] = dm * idl;
+        A.Uo[VN][cl] = vn - dtdm * (pa - pam);
+        A.u5e[cl] = e - dtdm * (pa * va - pam * vam);
+        A.blgo[0][cl] = b1 * cmp;
+        A.blgo[1][cl] = b2 * cmp;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) A.spo[f][cl] = spl[f] * cmp;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) A.spo[5 + f][cl] = spx[f] * cmp;
+      }
+      A.dxlagM[cl] = dxl;
+    }
+    // advance the window by one position
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { spw[0][f] = spw[1][f]; spw[1][f] = spw[2][f]; }
+#pragma unroll
+    for (int f = 0; f < 8; ++f) qprev[f] = qcur[f];
+    pa_prev = Pa;
+    va_prev = Va;
+    vt1 = nvt1; vt2 = nvt2; bnc = nbnc;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) spx[f] = nspx[f];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // MOC
 // ---------------------------------------------------------------------------
 template <class R> struct MocV {
@@ -624,6 +821,9 @@ __global__ __launch_bounds__(576, 7) void k_remap(SweepIO A, Geo g, double dt) {
 #pragma unroll
   for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
 }
+
+template __global__ void k_rhs_col<1, double>(SweepIO, Geo, double);
+template __global__ void k_rhs_col<2, double>(SweepIO, Geo, double);
 
 #define MCF_INST(D, R)                                                                                  \
   template __global__ void k_rhs<D, R>(SweepIO, Geo, double);                                          \

@@ -106,6 +106,21 @@ def test_multistep_vs_oracle_64():
     assert worst(errs) <= MULTI_TOL, errs
 
 
+def test_multistep_vs_oracle_long_columns():
+    """y and z longer than one column segment of the FP64 RHS (32 cells), one
+    of them not a multiple of it."""
+    p = small(24, 72, 40)
+    st = ics.random_state(p, seed=6)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2))
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
 @pytest.mark.parametrize("n", [64, 128])
 def test_divergence_free(n):
     """CT keeps the discrete div B of the face field at round-off."""
