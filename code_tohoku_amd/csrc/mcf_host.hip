@@ -46,8 +46,9 @@ struct mcf_ctx {
   double *Ub[5]{}, *spb[8]{};
   // sweep scratch
   double *u5e = nullptr, *blg[2]{}, *blgb[2]{}, *xlag = nullptr, *dxlagM = nullptr, *va = nullptr;
-  // CT scratch; SOURCE keeps its stiff-cell queues in dq (SOURCE and CT never overlap)
-  double *vfc[3]{}, *dq[4]{}, *emf[3]{};
+  // CT scratch
+  double *vfc[3]{}, *emf[3]{};
+  double *srcq[3]{}; // SOURCE: conduction increments and the two stiff-cell queues
   double *divv = nullptr, *red = nullptr;
   double *gx[3]{}, *gdx[3]{};
   std::vector<double> hx[3], hdx[3];
@@ -411,7 +412,6 @@ static CtArgs ct_args(mcf_ctx *x) {
   CtArgs A{};
   for (int f = 0; f < 8; ++f) A.U[f] = x->U[f];
   for (int c = 0; c < 3; ++c) { A.bcc[c] = x->bcc[c]; A.vfc[c] = x->vfc[c]; A.emf[c] = x->emf[c]; A.gdx[c] = x->gdx[c]; }
-  for (int t = 0; t < 4; ++t) A.dq[t] = x->dq[t];
   A.facdep = x->cfg.facdep;
   return A;
 }
@@ -440,20 +440,10 @@ static int do_ct(mcf_ctx *x, double dt) {
     if (o == 0) CHK(fillU(x, 7, 2, 1, l1, 1));
     int d1 = o, d2 = (o + 1) % 3, d3 = (o + 2) % 3;
     int lo[3], hi[3];
-    // slopes along d1 (dQ2, dQ4)
-    lo[d1] = 0; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
-    Box bs0 = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    if (x->mixed) KLAUNCH_R(k_cct_slopes, float, boxsize(bs0), 6, A, x->g, bs0, o, 0);
-    else KLAUNCH_R(k_cct_slopes, double, boxsize(bs0), 6, A, x->g, bs0, o, 0);
-    // slopes along d2 (dQ1, dQ3)
-    lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 0; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
-    Box bs1 = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    if (x->mixed) KLAUNCH_R(k_cct_slopes, float, boxsize(bs1), 6, A, x->g, bs1, o, 1);
-    else KLAUNCH_R(k_cct_slopes, double, boxsize(bs1), 6, A, x->g, bs1, o, 1);
     lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
     Box be = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    if (x->mixed) KLAUNCH_R(k_cct_emf, float, boxsize(be), 10, A, x->g, be, o, dt);
-    else KLAUNCH_R(k_cct_emf, double, boxsize(be), 10, A, x->g, be, o, dt);
+    if (x->mixed) KLAUNCH_R(k_cct, float, boxsize(be), 16, A, x->g, be, o, dt);
+    else KLAUNCH_R(k_cct, double, boxsize(be), 16, A, x->g, be, o, dt);
   }
   KLAUNCH(k_cc3, boxsize(b1), 9, A, x->g, b1, dt);
   return do_cc4(x);
@@ -467,46 +457,44 @@ static int do_source(mcf_ctx *x, double dt) {
   for (int s = 0; s < 8; ++s) A.sp[s] = x->sp[s];
   for (int d = 0; d < 3; ++d) A.gdx[d] = x->gdx[d];
   A.pmin = x->pmin; A.pmax = x->pmax;
+  A.cnt = x->cool_cnt;
+  A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2);
+  A.queue = reinterpret_cast<int *>(x->srcq[1]);
+  A.fqueue = reinterpret_cast<int *>(x->srcq[2]);
+  A.qdc = x->srcq[0];
   if (x->cfg.cooling == MCF_COOL_SURROGATE) {
     if (!x->mlp_ready) { g_err = "cooling surrogate selected but no weights loaded (mcf_mlp_set_weights / mcf_mlp_train)"; return MCF_E_STATE; }
     A.sur = 1;
     A.pk = x->mlp_pk;
     A.mw = x->mlp;
-    A.cnt = x->cool_cnt;
-    A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2);
-    A.queue = reinterpret_cast<int *>(x->dq[1]);
-    A.fqueue = reinterpret_cast<int *>(x->dq[2]);
-    A.qdc = x->dq[0];
   }
   int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
   int l15[2] = {1, 5}, l5[1] = {5};
   CHK(fillU(x, 7, 1, 1, l15, 2));
   CHK(fillOT(x, 7, 1, OT_SPECIES));
-  // stage 0: T and P_old from P, predictor into the second pressure buffer
-  double *P = x->U[F_P], *Q = x->Ub[F_P];
   Box bc = mkbox(1, nx, 1, ny, 1, nz), bg = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1);
-  A.Tp = P; A.Pn = P; A.Pout = Q;
-  if (A.sur) {
+  double *P = x->U[F_P], *Q = x->Ub[F_P];
+  for (int stage = 0; stage < 2; ++stage) {
+    if (stage == 0) {
+      // predictor: T and P_old from P, result into the second pressure buffer
+      A.Tp = P; A.Pn = P; A.Pout = Q;
+    } else {
+      // ghosts of the predictor (the fill works on U[F_P])
+      x->U[F_P] = Q; x->Ub[F_P] = P;
+      CHK(fillU(x, 7, 1, 1, l5, 1));
+      x->U[F_P] = P; x->Ub[F_P] = Q;
+      // corrector: T from the predictor, result written over P
+      A.Tp = Q; A.Pn = P; A.Pout = P;
+    }
+    const Box &b = stage == 0 ? bc : bg;
     KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
-    KLAUNCH_R(k_src_cell, true, boxsize(bc), 9, A, x->g, bc, dt, 0);
-    KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 0);
-    KLAUNCH_G(k_src_fallback, dim3(256), dim3(256), 0, A, dt, 0);
-  } else {
-    KLAUNCH_R(k_src_cell, false, boxsize(bc), 9, A, x->g, bc, dt, 0);
-  }
-  // ghosts of the predictor (the fill works on U[F_P])
-  x->U[F_P] = Q; x->Ub[F_P] = P;
-  CHK(fillU(x, 7, 1, 1, l5, 1));
-  x->U[F_P] = P; x->Ub[F_P] = Q;
-  // stage 1: T from the predictor, corrector from P_old written over P
-  A.Tp = Q; A.Pn = P; A.Pout = P;
-  if (A.sur) {
-    KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
-    KLAUNCH_R(k_src_cell, true, boxsize(bg), 9, A, x->g, bg, dt, 1);
-    KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, 1);
-    KLAUNCH_G(k_src_fallback, dim3(256), dim3(256), 0, A, dt, 1);
-  } else {
-    KLAUNCH_R(k_src_cell, false, boxsize(bg), 9, A, x->g, bg, dt, 1);
+    KLAUNCH(k_src_cell, boxsize(b), 9, A, x->g, b, dt, stage);
+    if (A.sur) {
+      KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, stage);
+      KLAUNCH_G(k_src_imc, dim3(256), dim3(256), 0, A, dt, stage, 1);
+    } else {
+      KLAUNCH_G(k_src_imc, dim3(256), dim3(256), 0, A, dt, stage, 0);
+    }
   }
   return MCF_OK;
 }
@@ -558,7 +546,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   hipError_t e = hipGetDevice(&x->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { g_err = std::string("HIP init: ") + hipGetErrorString(e); delete x; return MCF_E_HIP; }
-  const int nstate = 19, nbuf = 13, nsweep = 8, nscratch = 10;
+  const int nstate = 19, nbuf = 13, nsweep = 8, nscratch = 9;
   size_t nc = x->ncell;
   size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
   size_t ngrid = 0;
@@ -576,7 +564,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   x->blg[0] = p; p += nc; x->blg[1] = p; p += nc; x->blgb[0] = p; p += nc; x->blgb[1] = p; p += nc;
   x->xlag = p; p += nc; x->dxlagM = p; p += nc; x->va = p; p += nc;
   for (int c = 0; c < 3; ++c) { x->vfc[c] = p; p += nc; }
-  for (int t = 0; t < 4; ++t) { x->dq[t] = p; p += nc; }
+  for (int t = 0; t < 3; ++t) { x->srcq[t] = p; p += nc; }
   for (int c = 0; c < 3; ++c) { x->emf[c] = p; p += nc; }
   x->divv = p; p += ndivv;
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }

@@ -67,7 +67,6 @@ struct CtArgs {
   double *U[8];
   double *bcc[3];
   double *vfc[3];
-  double *dq[4];
   double *emf[3];
   const double *gdx[3]; // dx arrays, dx(i) at [i+1]
   double facdep;
@@ -110,13 +109,12 @@ struct DisArgs {
 __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
-template <class R> __global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode);
-template <class R> __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt);
+template <class R> __global__ void k_cct(CtArgs A, Geo g, Box b, int orient, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
-template <bool SUR> __global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
+__global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
-__global__ void k_src_fallback(SrcArgs A, double dt, int stage);
+__global__ void k_src_imc(SrcArgs A, double dt, int stage, int which);
 __global__ void k_zero_u32x2(unsigned *p);
 __global__ void k_dtmin(DtArgs A, Geo g, double *part);
 __global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal);

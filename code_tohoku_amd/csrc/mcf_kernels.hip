@@ -131,33 +131,16 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b) {
   }
 }
 
-// CCT slopes: mode 0 -> dQ2/dQ4 (along d1 of Vfc_d2, B_d2), mode 1 ->
-// dQ1/dQ3 (along d2 of Vfc_d1, B_d1) (M3:2203-2268).  The dQ arrays hold
-// values of type R (FP32 on the mixed path).
+// CCT (M3:2186-2445), one thread per edge of orientation (d1, d2).  The
+// edge's transverse slopes dQ (M3:2203-2268) are computed where they are
+// used: dQ1/dQ3 (along d2 of Vfc_d1 and B_d1) at the cells c0-s2 and c0,
+// dQ2/dQ4 (along d1 of Vfc_d2 and B_d2) at c0-s1 and c0, each with that
+// cell's limiter switch.  Then the four candidate pairs with their 2x2
+// corrections (M3:2278-2385), the flags and 1e13 weights (M3:2387-2415), the
+// weighted averages (M3:2419-2433) and EMF = 1/4 (v1 B2 - v2 B1) sqrt(dnc).
+// Arithmetic in R (FP32 on the mixed path).
 template <class R>
-__global__ void k_cct_slopes(CtArgs A, Geo g, Box b, int orient, int mode) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  int d1 = orient, d2 = (orient + 1) % 3;
-  int ax = mode == 0 ? d1 : d2;
-  int fv = mode == 0 ? d2 : d1;
-  int c = ix3(g, i, j, k);
-  int s = g.st[ax];
-  int p = ax == 0 ? i : ax == 1 ? j : k;
-  const double *dx = A.gdx[ax];
-  const R dxm = (R)dx[p], dx0 = (R)dx[p + 1], dxp = (R)dx[p + 2];
-  const bool mm = use_minmod_r<R>(A.U[4][c], A.U[0][c]);
-  const double *V = A.vfc[fv], *B = A.U[5 + fv];
-  R sv = vslope<R>(V[c - s], V[c], V[c + s], dxm, dx0, dxp, mm);
-  R sbv = vslope<R>(B[c - s], B[c], B[c + s], dxm, dx0, dxp, mm);
-  R *const *dq = reinterpret_cast<R *const *>(A.dq);
-  if (mode == 0) { dq[1][c] = sv; dq[3][c] = sbv; }
-  else { dq[0][c] = sv; dq[2][c] = sbv; }
-}
-
-// CCT edge EMF (M3:2271-2441): one thread per edge, arithmetic in R.
-template <class R>
-__global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt_d) {
+__global__ void k_cct(CtArgs A, Geo g, Box b, int orient, double dt_d) {
   using Q = Ar<R>;
   int i, j, k;
   if (!box_ijk(b, i, j, k)) return;
@@ -167,35 +150,55 @@ __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt_d) {
   const int p1 = d1 == 0 ? i : d1 == 1 ? j : k;
   const int p2 = d2 == 0 ? i : d2 == 1 ? j : k;
   const double *r = A.U[0];
+  const double *Vx = A.vfc[d1], *Vy = A.vfc[d2], *Bx = A.U[5 + d1], *By = A.U[5 + d2];
+  const double *dxe = A.gdx[d1], *dye = A.gdx[d2];
+  // transverse slopes at the four cells the candidates use
+  const bool mmA = use_minmod_r<R>(A.U[4][c0 - s2], r[c0 - s2]);
+  const bool mmB = use_minmod_r<R>(A.U[4][c0 - s1], r[c0 - s1]);
+  const bool mm0 = use_minmod_r<R>(A.U[4][c0], r[c0]);
+  R dQ0[2], dQ2[2], dQ1[2], dQ3[2]; // [0]: cell c0-s2 (dQ0/2) or c0-s1 (dQ1/3), [1]: c0
+  {
+    // along d2 at c0-s2 (position p2-1) and c0 (position p2)
+    const R ym = (R)dye[p2 - 1], y0 = (R)dye[p2], y1 = (R)dye[p2 + 1], y2 = (R)dye[p2 + 2];
+    dQ0[0] = vslope<R>(Vx[c0 - 2 * s2], Vx[c0 - s2], Vx[c0], ym, y0, y1, mmA);
+    dQ2[0] = vslope<R>(Bx[c0 - 2 * s2], Bx[c0 - s2], Bx[c0], ym, y0, y1, mmA);
+    dQ0[1] = vslope<R>(Vx[c0 - s2], Vx[c0], Vx[c0 + s2], y0, y1, y2, mm0);
+    dQ2[1] = vslope<R>(Bx[c0 - s2], Bx[c0], Bx[c0 + s2], y0, y1, y2, mm0);
+    // along d1 at c0-s1 (position p1-1) and c0 (position p1)
+    const R xm = (R)dxe[p1 - 1], x0 = (R)dxe[p1], x1 = (R)dxe[p1 + 1], x2 = (R)dxe[p1 + 2];
+    dQ1[0] = vslope<R>(Vy[c0 - 2 * s1], Vy[c0 - s1], Vy[c0], xm, x0, x1, mmB);
+    dQ3[0] = vslope<R>(By[c0 - 2 * s1], By[c0 - s1], By[c0], xm, x0, x1, mmB);
+    dQ1[1] = vslope<R>(Vy[c0 - s1], Vy[c0], Vy[c0 + s1], x0, x1, x2, mm0);
+    dQ3[1] = vslope<R>(By[c0 - s1], By[c0], By[c0 + s1], x0, x1, x2, mm0);
+  }
   double dncd;
   if (orient == 2) dncd = 0.25 * (r[c0] + r[c0 - s2] + r[c0 - s1] + r[c0 - s1 - s2]);
   else dncd = 0.25 * (r[c0] + r[c0 - s1] + r[c0 - s2] + r[c0 - s1 - s2]);
   const R dnc = (R)dncd;
   const R dt = (R)dt_d;
   const R dsqinv = Q::div((R)1.0, Q::sqrt_(dnc));
-  const double *Vx = A.vfc[d1], *Vy = A.vfc[d2], *Bx = A.U[5 + d1], *By = A.U[5 + d2];
-  const R *const *dq = reinterpret_cast<const R *const *>(A.dq);
   const R facm = (R)A.facdep * (R)0.5;
-  const double *dxe = A.gdx[d1], *dye = A.gdx[d2];
   R C1p[4], C1m[4], C2p[4], C2m[4];
 #pragma unroll
   for (int cs = 0; cs < 4; ++cs) {
     // 0: LL (S1=+1,S2=+1)  1: RL (-1,+1)  2: LR (+1,-1)  3: RR (-1,-1)
     const R S1 = (cs == 0 || cs == 2) ? (R)1.0 : (R)-1.0;
     const R S2 = (cs == 0 || cs == 1) ? (R)1.0 : (R)-1.0;
-    const int q1 = (cs == 0 || cs == 2) ? c0 - s2 : c0;
-    const int q2 = (cs == 0 || cs == 1) ? c0 - s1 : c0;
-    const R dyv = (R)((cs == 0 || cs == 2) ? dye[p2] : dye[p2 + 1]);
-    const R dxv = (R)((cs == 0 || cs == 1) ? dxe[p1] : dxe[p1 + 1]);
+    const int u1 = (cs == 0 || cs == 2) ? 0 : 1; // q1 = c0 - s2 or c0
+    const int u2 = (cs == 0 || cs == 1) ? 0 : 1; // q2 = c0 - s1 or c0
+    const int q1 = u1 ? c0 : c0 - s2;
+    const int q2 = u2 ? c0 : c0 - s1;
+    const R dyv = (R)(u1 ? dye[p2 + 1] : dye[p2]);
+    const R dxv = (R)(u2 ? dxe[p1 + 1] : dxe[p1]);
     const R vx = (R)Vx[q1], bx = (R)Bx[q1], vy = (R)Vy[q2], by = (R)By[q2];
     R a1p = vx + bx * dsqinv;
-    R da1p = (dq[0][q1] + dq[2][q1] * dsqinv) * facm;
+    R da1p = (dQ0[u1] + dQ2[u1] * dsqinv) * facm;
     R a1m = vx - bx * dsqinv;
-    R da1m = (dq[0][q1] - dq[2][q1] * dsqinv) * facm;
+    R da1m = (dQ0[u1] - dQ2[u1] * dsqinv) * facm;
     R a2m = vy - by * dsqinv;
-    R da2m = (dq[1][q2] - dq[3][q2] * dsqinv) * facm;
+    R da2m = (dQ1[u2] - dQ3[u2] * dsqinv) * facm;
     R a2p = vy + by * dsqinv;
-    R da2p = (dq[1][q2] + dq[3][q2] * dsqinv) * facm;
+    R da2p = (dQ1[u2] + dQ3[u2] * dsqinv) * facm;
     R t;
     t = a1p + da1p * dyv * S1 - da1p * (a2m + da2m * dxv * S2) * dt;
     C1p[cs] = Q::div(t, (R)1.0 - da1p * da2m * dt * dt);
@@ -245,10 +248,8 @@ __global__ void k_cct_emf(CtArgs A, Geo g, Box b, int orient, double dt_d) {
   A.emf[(orient + 2) % 3][c0] = (double)((R)0.25 * (v1a * B2a - v2a * B1a) * Q::sqrt_(dnc));
 }
 
-template __global__ void k_cct_slopes<double>(CtArgs, Geo, Box, int, int);
-template __global__ void k_cct_slopes<float>(CtArgs, Geo, Box, int, int);
-template __global__ void k_cct_emf<double>(CtArgs, Geo, Box, int, double);
-template __global__ void k_cct_emf<float>(CtArgs, Geo, Box, int, double);
+template __global__ void k_cct<double>(CtArgs, Geo, Box, int, double);
+template __global__ void k_cct<float>(CtArgs, Geo, Box, int, double);
 
 // CC(3): face-B induction (M3:2159-2169)
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt) {
@@ -298,9 +299,9 @@ __device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, doubl
 // P_old there); its threads on the first ghost layer copy the predictor's
 // ghost values across, so the ghosts end as the reference leaves them.
 // The temperature of a cell and of its six neighbours is computed on the
-// fly.  With SUR the backward-Euler cells (dt > 0.2*t_cool) are not solved
-// here: they are queued with their conduction increment for k_src_stiff.
-template <bool SUR>
+// fly.  The backward-Euler cells (dt > 0.2*t_cool) are not solved here:
+// they are queued with their conduction increment for k_src_stiff (the
+// surrogate) or k_src_imc, so the bisection runs on dense waves.
 __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
   int i, j, k;
   bool valid = box_ijk(b, i, j, k);
@@ -309,7 +310,6 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
     A.Pout[cg] = A.Tp[cg];
     valid = false;
   }
-  if (!SUR && !valid) return;
   if (!valid) { i = j = k = 1; }
   int c = ix3(g, i, j, k);
   const double *const *sp = A.sp;
@@ -325,17 +325,8 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
   double Pn = A.Pn[c];
   const bool expl = dt <= 0.2 * Pn / (gi * fabs(CooL));
   double u = 0.0;
-  if (stage == 0) {
-    u = Pn;
-    if (expl) u = u - gi * CooL * dt * 0.5;
-    else if (!SUR) u = imc(u, nsum, dt * 0.5, rho, gi);
-  } else {
-    if (expl) u = Pn - gi * CooL * dt;
-    else if (!SUR) {
-      Pn = imc(Pn, nsum, dt, rho, gi);
-      u = Pn;
-    }
-  }
+  if (stage == 0) u = Pn - gi * CooL * dt * 0.5;
+  else u = Pn - gi * CooL * dt;
   double dxi = A.gdx[0][i + 1], dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
   double tcd = gi * KCOND * sqrt(T);
   tcd = 0.49 * nsum * KB / tcd * (dxi * dxi);
@@ -350,7 +341,7 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
   double dc;
   if (stage == 0) dc = (tcd + 0.5) * gi * dt * 0.5 * div;
   else dc = (tcd + 0.5) * gi * dt * div;
-  if constexpr (SUR) {
+  {
     const bool stiff = valid && !expl;
     const unsigned long long m = __ballot(stiff);
     if (m) {
@@ -372,13 +363,11 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
   u = dmin(u, A.pmax);
   A.Pout[c] = u;
 }
-template __global__ void k_src_cell<false>(SrcArgs, Geo, Box, double, int);
-template __global__ void k_src_cell<true>(SrcArgs, Geo, Box, double, int);
 
 // The queued backward-Euler cells of one SOURCE stage, 64 per wave: the
 // surrogate predicts, Newton steps refine, the residual test accepts; the
 // cell's conduction increment and the pressure clamps are then applied as
-// in k_src_cell.  Rejected cells go to a second queue for k_src_fallback,
+// in k_src_cell.  Rejected cells go to a second queue for k_src_imc,
 // so IMC runs on dense waves.  Waves stride over the queue.
 __device__ __forceinline__ void src_finish(const SrcArgs &A, int c, double Pi) {
   double u = Pi + A.qdc[c];
@@ -420,12 +409,15 @@ __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int sta
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[0], (unsigned long long)nq);
 }
 
-// IMC for the cells the surrogate's residual test rejected.
-__global__ __launch_bounds__(256) void k_src_fallback(SrcArgs A, double dt, int stage) {
-  const unsigned nf = A.qn[1];
+// IMC (M3:2898-2921) on a queue of backward-Euler cells: all of them on the
+// plain path (which = 0), the ones the surrogate's residual test rejected
+// (which = 1).
+__global__ __launch_bounds__(256) void k_src_imc(SrcArgs A, double dt, int stage, int which) {
+  const unsigned nq = A.qn[which];
+  const int *q = which ? A.fqueue : A.queue;
   const unsigned stride = gridDim.x * blockDim.x;
-  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nf; e += stride) {
-    const int c = A.fqueue[e];
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nq; e += stride) {
+    const int c = q[e];
     const double *const *sp = A.sp;
     const double rho = A.rho[c];
     const double gi = gammi1_of(sp, c);
@@ -433,7 +425,7 @@ __global__ __launch_bounds__(256) void k_src_fallback(SrcArgs A, double dt, int 
     const double dts = stage == 0 ? dt * 0.5 : dt;
     src_finish(A, c, imc(A.Pn[c], nsum, dts, rho, gi));
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[1], (unsigned long long)nf);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[which], (unsigned long long)nq);
 }
 
 __global__ void k_zero_u32x2(unsigned *p) {

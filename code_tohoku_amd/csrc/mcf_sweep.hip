@@ -444,13 +444,11 @@ __global__ __launch_bounds__(256) void k_rhs_col(SweepIO A, Geo g, double dt) {
     wa[0] = B1[c0]; wa[1] = B1[c1]; wa[2] = B1[c2]; wa[3] = B1[c3];
     wc[0] = B2[c0]; wc[1] = B2[c1]; wc[2] = B2[c2]; wc[3] = B2[c3];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) { spw[0][f] = sp[f][c1]; spw[1][f] = sp[f][c2]; spw[2][f] = sp[f][c3]; }
+    for (int f = 0; f < 5; ++f) { spw[0][f] = sp[f][c1]; spw[1][f] = sp[f][c2]; }
     rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
                   wc[2], spw[0], dxe[p0 - 1], dxe[p0], dxe[p0 + 1], dt, A.facdep, qprev);
 #pragma unroll
     for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
-#pragma unroll
-    for (int f = 0; f < 5; ++f) { spw[0][f] = spw[1][f]; spw[1][f] = spw[2][f]; }
   }
   // update inputs of the cell left of the current interface, loaded one
   // iteration ahead
@@ -460,7 +458,9 @@ __global__ __launch_bounds__(256) void k_rhs_col(SweepIO A, Geo g, double dt) {
   for (int q = p0; q <= p1 + 1; ++q) {
     const int cl = cell(q - 1);
     // issue the loads of the next iteration first
-    const int cf = cell(q + 2), cn = cell(q + 1), cq = cell(q);
+    // (position q + 2 is clamped to the last ghost: its value is not used
+    // after the last iteration of the last segment)
+    const int cf = cell(min(q + 2, n + 2)), cn = cell(q + 1), cq = cell(q);
     wr[3] = U1[cf]; wv[3] = UVN[cf]; wp[3] = U5[cf]; wa[3] = B1[cf]; wc[3] = B2[cf];
 #pragma unroll
     for (int f = 0; f < 5; ++f) spw[2][f] = sp[f][cn];

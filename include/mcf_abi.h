@@ -169,8 +169,9 @@ int mcf_cool_samples(mcf_ctx *ctx, int64_t n, uint64_t seed, float *X, float *Y)
  * *ms = kernel time. */
 int mcf_cool_solve(mcf_ctx *ctx, const double *P0, const double *n, const double *dt, const double *rho,
                    const double *gammi1, int64_t ncell, int mode, double *P, int32_t *flag, double *ms);
-/* counts[0] = backward-Euler cells solved in SOURCE since the last reset,
- * counts[1] = of those, the IMC fallbacks (surrogate mode only). */
+/* counts[0] = backward-Euler cells solved in SOURCE since the last reset
+ * (both cooling modes), counts[1] = of those, the cells the surrogate
+ * rejected and IMC solved (surrogate mode). */
 int mcf_cool_stats(mcf_ctx *ctx, int64_t *counts, int reset);
 
 /* Waits for all work of the context. */
