@@ -62,6 +62,9 @@ struct mcf_ctx {
   // per-kernel event timing (mcf_profile)
   bool prof = false;
   bool mixed = false; // MCF_PREC_MIXED: FP32 reconstruction / Riemann / EMF
+  // y/z sweeps run RHS and MOC as one column march (k_rhsmoc_col): needs a
+  // uniform spacing along the axis (ghosts included) and n >= 8
+  bool fuse[3]{false, false, false};
   // cooling surrogate: FP32 weights, Adam moments, gradient (4 x MLP_NP),
   // packed BF16 operands, counters {backward-Euler cells, IMC fallbacks}
   float *mlp = nullptr;
@@ -367,34 +370,41 @@ static dim3 sweep_grid(const mcf_ctx *x, int d) {
 // MHD (M3:1264-1324): RHS -> MOC -> REMAP -> primitives
 static int do_sweep(mcf_ctx *x, int d, double dt) {
   SweepIO A = sweep_io(x, d);
-  // RHS (fills M3:1346-1357): reads rho, v(3), p, Bcc(3), species; writes
-  // rho, v_n, e, Blg(2), species, interface velocity / position, widths
-  int lr[3] = {1, A.VN + 1, 5};
-  CHK(fillU(x, 1 << d, 2, 1, lr, 3));
-  CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
-  A = sweep_io(x, d);
-  if (d == 0 || x->mixed) {
-    SWEEP_LAUNCH(k_rhs, d, true, 33, A, x->g, dt);
-  } else {
-    // FP64 along y/z: one thread per x column marching along the sweep axis
-    // (measured faster than the LDS tiles there; the mixed kernels are faster
-    // as tiles, where more of them fit per CU)
+  if (x->fuse[d]) {
+    // y/z: RHS and MOC in one column march over the periodic axis; it reads
+    // only interior cells (periodic images), so no fills precede it
     const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
-    dim3 gr((unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4), (unsigned)((n + RHS_COL_SEG - 1) / RHS_COL_SEG));
+    dim3 gr((unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4), (unsigned)((n + FUSE_SEG - 1) / FUSE_SEG));
     const double by = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * 33;
-    if (d == 1) KLAUNCH_N("k_rhs", (k_rhs_col<1, double>), gr, dim3(256), by, A, x->g, dt);
-    else KLAUNCH_N("k_rhs", (k_rhs_col<2, double>), gr, dim3(256), by, A, x->g, dt);
+    if (x->mixed) {
+      if (d == 1) KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<1, float>), gr, dim3(256), by, A, x->g, dt);
+      else KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<2, float>), gr, dim3(256), by, A, x->g, dt);
+    } else {
+      if (d == 1) KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<1, double>), gr, dim3(256), by, A, x->g, dt);
+      else KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<2, double>), gr, dim3(256), by, A, x->g, dt);
+    }
+    for (int f = 0; f < 5; ++f) swapU(x, f);
+    swapSp(x); swapBlg(x);
+    CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
+  } else {
+    // RHS (fills M3:1346-1357): reads rho, v(3), p, Bcc(3), species; writes
+    // rho, v_n, e, Blg(2), species, interface velocity / position, widths
+    int lr[3] = {1, A.VN + 1, 5};
+    CHK(fillU(x, 1 << d, 2, 1, lr, 3));
+    CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
+    A = sweep_io(x, d);
+    SWEEP_LAUNCH(k_rhs, d, true, 33, A, x->g, dt);
+    swapU(x, 0); swapU(x, A.VN); swapSp(x); swapBlg(x);
+    // MOC (fills M3:1950-1976): reads rho, v_t(2), Blg(2), p, e, widths, Bn;
+    // writes momenta(3), E, Blg(2)
+    int lm[4] = {1, A.VT1 + 1, A.VT2 + 1, A.BN + 1};
+    CHK(fillU(x, 1 << d, 2, 1, lm, 4));
+    CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
+    CHK(fillOT(x, 1 << d, 2, OT_BLG));
+    A = sweep_io(x, d);
+    SWEEP_LAUNCH(k_moc, d, true, 20, A, x->g, dt);
+    swapU(x, A.VN); swapU(x, A.VT1); swapU(x, A.VT2); swapU(x, 4); swapBlg(x);
   }
-  swapU(x, 0); swapU(x, A.VN); swapSp(x); swapBlg(x);
-  // MOC (fills M3:1950-1976): reads rho, v_t(2), Blg(2), p, e, widths, Bn;
-  // writes momenta(3), E, Blg(2)
-  int lm[4] = {1, A.VT1 + 1, A.VT2 + 1, A.BN + 1};
-  CHK(fillU(x, 1 << d, 2, 1, lm, 4));
-  CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
-  CHK(fillOT(x, 1 << d, 2, OT_BLG));
-  A = sweep_io(x, d);
-  SWEEP_LAUNCH(k_moc, d, true, 20, A, x->g, dt);
-  swapU(x, A.VN); swapU(x, A.VT1); swapU(x, A.VT2); swapU(x, 4); swapBlg(x);
   // REMAP (fills M3:1571-1594): reads rho, m(3), E, Blg(2), species,
   // widths, interface velocity, Bcc_n; writes rho, v(3), p, species
   CHK(fillOT(x, 1 << d, 2, OT_BLG));
@@ -570,6 +580,11 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
   x->red = p; p += 16 + 2 * DT_BLOCKS;
   HIPCHK(hipMemsetAsync(x->pool, 0, total * sizeof(double), x->stream));
+  for (int d = 1; d < 3; ++d) {
+    bool uni = n[d] >= 8;
+    for (int k = 0; k < n[d] + 4 && uni; ++k) uni = std::memcmp(&x->hdx[d][k], &x->hdx[d][0], sizeof(double)) == 0;
+    x->fuse[d] = uni;
+  }
   for (int d = 0; d < 3; ++d) {
     HIPCHK(hipMemcpyAsync(x->gx[d], x->hx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(x->gdx[d], x->hdx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));

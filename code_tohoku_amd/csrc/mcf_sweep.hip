@@ -556,26 +556,22 @@ __device__ __forceinline__ void moc_slopes(const SweepIO &A, int c, int s, bool 
   for (int v = 0; v < 5; ++v) gs[v] = vslope<R>(Q[v][c - s], Q[v][c], Q[v][c + s], dxm, dx0, dxp, mm);
 }
 
-// One characteristic interface (M3:2000-2051) between c and cp = c + s, from
-// the slopes gi (cell c) and gp (cell cp).
+// One characteristic interface (M3:2000-2051) from register values: v0 and
+// v1 hold (rho, v_t1, v_t2, Blg1, Blg2, dx_lag) of the left and right cell,
+// gi and gp their slopes, Bnap_d the normal field at the Lagrangian
+// interface position.
 template <class R>
-__device__ __forceinline__ MocV<R> moc_iface(const SweepIO &A, int c, int cp, int q, const R *gi, const R *gp,
-                                             double dt) {
-  const double *U1 = A.U[0], *UT1 = A.U[A.VT1], *UT2 = A.U[A.VT2], *UBn = A.U[A.BN];
-  const double *Bl1 = A.blg[0], *Bl2 = A.blg[1];
-  const double *xe = A.xe;
-  const int cpp = cp + (cp - c);
-  const double dl1 = A.dxlagM[c], dl2 = A.dxlagM[cp];
-  // B_n at the Lagrangian interface position (M3:2008)
-  const double Bnap_d = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
+__device__ __forceinline__ MocV<R> moc_iface_r(const double *v0, const double *v1, const R *gi, const R *gp,
+                                               double Bnap_d, double facdep, double dt) {
+  const double dl1 = v0[5], dl2 = v1[5];
   R deni;
   if constexpr (Ar<R>::strict) {
-    deni = sqrt((dl1 + dl2) / (dl1 * U1[cp] + dl2 * U1[c]));
+    deni = sqrt((dl1 + dl2) / (dl1 * v1[0] + dl2 * v0[0]));
   } else {
-    deni = Ar<float>::sqrt_(Ar<float>::div((float)(dl1 + dl2), (float)(dl1 * U1[cp] + dl2 * U1[c])));
+    deni = Ar<float>::sqrt_(Ar<float>::div((float)(dl1 + dl2), (float)(dl1 * v1[0] + dl2 * v0[0])));
   }
   const R Bnap = (R)Bnap_d;
-  const R fac = (R)A.facdep;
+  const R fac = (R)facdep;
   const R rdt = (R)dt;
   R depend1, depend2, ca;
   R QL11, QL12, QL13, QR11, QR12, QR13, QL21, QL22, QR21, QR22;
@@ -586,16 +582,16 @@ __device__ __forceinline__ MocV<R> moc_iface(const SweepIO &A, int c, int cp, in
     ca = fabs(Bnap * deni) * (R)0.5 * rdt;
     ca = rmin((R)dl2, ca);
     depend2 = ((R)0.5 * (R)dl2 - ca) * fac;
-    QL11 = (R)(UT1[c] + (double)(depend1 * gi[1]));
-    QR11 = (R)(UT1[cp] - (double)(depend2 * gp[1]));
-    QL12 = (R)(Bl1[c] + (double)(depend1 * gi[3]));
-    QR12 = (R)(Bl1[cp] - (double)(depend2 * gp[3]));
-    QL13 = left_state_pos<R>(U1[c], depend1, gi[0]);
-    QR13 = right_state_pos<R>(U1[cp], depend2, gp[0]);
-    QL21 = (R)(UT2[c] + (double)(depend1 * gi[2]));
-    QR21 = (R)(UT2[cp] - (double)(depend2 * gp[2]));
-    QL22 = (R)(Bl2[c] + (double)(depend1 * gi[4]));
-    QR22 = (R)(Bl2[cp] - (double)(depend2 * gp[4]));
+    QL11 = (R)(v0[1] + (double)(depend1 * gi[1]));
+    QR11 = (R)(v1[1] - (double)(depend2 * gp[1]));
+    QL12 = (R)(v0[3] + (double)(depend1 * gi[3]));
+    QR12 = (R)(v1[3] - (double)(depend2 * gp[3]));
+    QL13 = left_state_pos<R>(v0[0], depend1, gi[0]);
+    QR13 = right_state_pos<R>(v1[0], depend2, gp[0]);
+    QL21 = (R)(v0[2] + (double)(depend1 * gi[2]));
+    QR21 = (R)(v1[2] - (double)(depend2 * gp[2]));
+    QL22 = (R)(v0[4] + (double)(depend1 * gi[4]));
+    QR22 = (R)(v1[4] - (double)(depend2 * gp[4]));
   } else {
     ca = fabs(Bnap * deni) * (R)0.5 * rdt;
     ca = rmin((R)dl2, ca);
@@ -603,21 +599,40 @@ __device__ __forceinline__ MocV<R> moc_iface(const SweepIO &A, int c, int cp, in
     ca = fabs(Bnap * deni) * (R)0.5 * rdt;
     ca = rmin((R)dl1, ca);
     depend2 = ((R)0.5 * (R)dl1 - ca) * fac;
-    QL11 = (R)(UT1[cp] - (double)(depend1 * gp[1]));
-    QR11 = (R)(UT1[c] + (double)(depend2 * gi[1]));
-    QL12 = (R)(Bl1[cp] - (double)(depend1 * gp[3]));
-    QR12 = (R)(Bl1[c] + (double)(depend2 * gi[3]));
-    QL13 = right_state_pos<R>(U1[cp], depend1, gp[0]);
-    QR13 = left_state_pos<R>(U1[c], depend2, gi[0]);
-    QL21 = (R)(UT2[cp] - (double)(depend1 * gp[2]));
-    QR21 = (R)(UT2[c] + (double)(depend2 * gi[2]));
-    QL22 = (R)(Bl2[cp] - (double)(depend1 * gp[4]));
-    QR22 = (R)(Bl2[c] + (double)(depend2 * gi[4]));
+    QL11 = (R)(v1[1] - (double)(depend1 * gp[1]));
+    QR11 = (R)(v0[1] + (double)(depend2 * gi[1]));
+    QL12 = (R)(v1[3] - (double)(depend1 * gp[3]));
+    QR12 = (R)(v0[3] + (double)(depend2 * gi[3]));
+    QL13 = right_state_pos<R>(v1[0], depend1, gp[0]);
+    QR13 = left_state_pos<R>(v0[0], depend2, gi[0]);
+    QL21 = (R)(v1[2] - (double)(depend1 * gp[2]));
+    QR21 = (R)(v0[2] + (double)(depend2 * gi[2]));
+    QL22 = (R)(v1[4] - (double)(depend1 * gp[4]));
+    QR22 = (R)(v0[4] + (double)(depend2 * gi[4]));
   }
   MocV<R> o;
   chareq_r<R>(QL11, QL12, QL13, QR11, QR12, QR13, o.Bay, o.Vay);
   chareq_r<R>(QL21, QL22, QL13, QR21, QR22, QR13, o.Baz, o.Vaz);
   return o;
+}
+
+// B_n at the Lagrangian position of interface q (M3:2008): linear
+// interpolation of the face values f0, f1, f2 at faces q, q+1, q+2.
+__device__ __forceinline__ double moc_bn(double f0, double f1, double f2, double xlag, const double *xe, int q) {
+  return f1 + (xlag - xe[q + 1]) * (f2 - f0) / (xe[q + 2] - xe[q]);
+}
+
+// The interface between c and cp = c + s at position q, from the slopes gi
+// (cell c) and gp (cell cp), with the cell values read from the arrays.
+template <class R>
+__device__ __forceinline__ MocV<R> moc_iface(const SweepIO &A, int c, int cp, int q, const R *gi, const R *gp,
+                                             double dt) {
+  const double *UBn = A.U[A.BN];
+  const int cpp = cp + (cp - c);
+  const double v0[6] = {A.U[0][c], A.U[A.VT1][c], A.U[A.VT2][c], A.blg[0][c], A.blg[1][c], A.dxlagM[c]};
+  const double v1[6] = {A.U[0][cp], A.U[A.VT1][cp], A.U[A.VT2][cp], A.blg[0][cp], A.blg[1][cp], A.dxlagM[cp]};
+  const double Bnap_d = moc_bn(UBn[c], UBn[cp], UBn[cpp], A.xlag[c], A.xe, q);
+  return moc_iface_r<R>(v0, v1, gi, gp, Bnap_d, A.facdep, dt);
 }
 
 // The limiter switch of a cell in MOC uses the specific total energy e left
@@ -699,8 +714,8 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_moc(Sw
   const int c = ix3s(g, t.i, t.j, t.k), cm = c - s, cp = c + s, cpp = cp + s;
   const double *UBn = A.U[A.BN];
   const double *xe = A.xe;
-  const double Bnap = UBn[cp] + (A.xlag[c] - xe[q + 1]) * (UBn[cpp] - UBn[c]) / (xe[q + 2] - xe[q]);
-  const double Bnam = UBn[c] + (A.xlag[cm] - xe[q]) * (UBn[cp] - UBn[cm]) / (xe[q + 1] - xe[q - 1]);
+  const double Bnap = moc_bn(UBn[c], UBn[cp], UBn[cpp], A.xlag[c], xe, q);
+  const double Bnam = moc_bn(UBn[cm], UBn[c], UBn[cp], A.xlag[cm], xe, q - 1);
   const double Vnap = A.va[c], Vnam = A.va[cm];
   const double dxl = A.dxlagM[c];
   const double rho = U1[c];
@@ -726,6 +741,238 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_moc(Sw
                                           (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
     A.blgo[0][c] = A.blg[0][c] + f * (Bnap * VayL - Bnam * VayR);
     A.blgo[1][c] = A.blg[1][c] + f * (Bnap * VazL - Bnam * VazR);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RHS + MOC fused along a periodic axis (y or z)
+// ---------------------------------------------------------------------------
+// One thread per x column i of transverse row tr marches along the sweep
+// axis over positions p0-4 .. p1+5 and produces the MOC results of cells
+// p0..p1 (one segment), so the Lagrangian state of RHS never goes to HBM.
+// Per iteration q: RHS states of cell q, the Riemann interface q-1|q and the
+// Lagrangian update of cell k = q-1 (RHS result O(k)); then the MOC slope of
+// cell k-1, the characteristic interface k-2|k-1 and the MOC update of cell
+// k-2.
+//
+// Periodicity: every value at a position outside 1..n is the value of the
+// periodic image (the ghost fills are copies), so loads and the positions
+// used for the Lagrangian update of a cell go through the image position.
+// This needs a grid spacing that is the same at every position of the axis
+// (checked by the host); the absolute positions xe still enter per cell.
+// What does not follow the image is the MOC limiter switch: it uses e from
+// RHS for cells 1..n and the pre-sweep pressure for cells outside (ghost
+// cells), and each interface at a block edge of the reproduced MPI run
+// (k_moc) is evaluated in both blocks' views.
+template <class R> struct OR {
+  double v[6]; // rho, v_t1, v_t2, Blg1, Blg2, dx_lag (the MOC slope fields)
+  double vn, e;
+  bool mme, mmp; // limiter switch from e and from the pre-sweep pressure
+};
+
+template <int D, class R>
+__global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt) {
+  constexpr int VN = D + 1, VT1 = (D + 1) % 3 + 1, VT2 = (D + 2) % 3 + 1;
+  constexpr int BT1 = (D + 1) % 3, BT2 = (D + 2) % 3;
+  const int nx = g.n[0];
+  const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
+  const int i = 1 + 64 * (int)blockIdx.x + (int)(threadIdx.x & 63);
+  const int tr = 1 + 4 * (int)blockIdx.y + (int)(threadIdx.x >> 6);
+  const int p0 = 1 + FUSE_SEG * (int)blockIdx.z;
+  if (i > nx || tr > ntr || p0 > n) return;
+  const int p1 = min(n, p0 + FUSE_SEG - 1);
+  const int nblk = A.nblk;
+  // image position of p (n >= 8, so one shift suffices for p0-6 .. p1+6)
+  auto img = [&](int p) { return p < 1 ? p + n : (p > n ? p - n : p); };
+  auto at = [&](int p) { return D == 1 ? ix3s(g, i, p, tr) : ix3s(g, i, tr, p); };
+  auto cell = [&](int p) { return at(img(p)); };
+  const double *U1 = A.U[0], *UVN = A.U[VN], *U5 = A.U[4];
+  const double *B1 = A.bcc[BT1], *B2 = A.bcc[BT2];
+  const double *UBn = A.U[5 + D];
+  const double *const *sp = A.sp;
+  const double *xe = A.xe;
+  const double dxu = A.dxe[1];
+  // RHS window: positions q-1, q, q+1 and the values of q+2 in flight
+  double wr[4], wv[4], wp[4], wa[4], wc[4];
+  double spw[3][5];
+  R qprev[8];
+  R pa_prev = 0, va_prev = 0;
+  const int ps = p0 - 2, pe = p1 + 3;
+  {
+    const int c0 = cell(ps - 2), c1 = cell(ps - 1), c2 = cell(ps), c3 = cell(ps + 1);
+    wr[0] = U1[c0]; wr[1] = U1[c1]; wr[2] = U1[c2]; wr[3] = U1[c3];
+    wv[0] = UVN[c0]; wv[1] = UVN[c1]; wv[2] = UVN[c2]; wv[3] = UVN[c3];
+    wp[0] = U5[c0]; wp[1] = U5[c1]; wp[2] = U5[c2]; wp[3] = U5[c3];
+    wa[0] = B1[c0]; wa[1] = B1[c1]; wa[2] = B1[c2]; wa[3] = B1[c3];
+    wc[0] = B2[c0]; wc[1] = B2[c1]; wc[2] = B2[c2]; wc[3] = B2[c3];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { spw[0][f] = sp[f][c1]; spw[1][f] = sp[f][c2]; }
+    rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
+                  wc[2], spw[0], dxu, dxu, dxu, dt, A.facdep, qprev);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
+  }
+  // inputs of the Lagrangian update of cell q-1 (loaded one iteration ahead)
+  double vt1 = 0, vt2 = 0, bnc = 0, fb = 0, spx[3] = {0, 0, 0};
+  // MOC windows: RHS results of k-2, k-1, k; faces k-2, k-1, k; interface
+  // velocities of k-2|k-1, k-1|k, k|k+1; slopes of k-2 and k-1 (regular and,
+  // next to block edges, with the ghost switch)
+  OR<R> o0{}, o1{}, o2{};
+  double f0 = 0, f1 = 0, f2 = 0;
+  double va0 = 0, va1 = 0, va2 = 0;
+  R G0[5], G1[5], H0[5], H1[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) { G0[v] = G1[v] = H0[v] = H1[v] = (R)0; }
+  MocV<R> rvp{};           // right view of interface J-1
+  double bnp = 0, vap = 0; // B_n and V* of interface J-1
+  for (int q = ps; q <= pe; ++q) {
+    const int cf = cell(q + 2), cn = cell(q + 1), cq = cell(q);
+    wr[3] = U1[cf]; wv[3] = UVN[cf]; wp[3] = U5[cf]; wa[3] = B1[cf]; wc[3] = B2[cf];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) spw[2][f] = sp[f][cn];
+    const double nvt1 = A.U[VT1][cq], nvt2 = A.U[VT2][cq], nbnc = A.bcc[D][cq], nfb = UBn[cq];
+    const double nsp0 = sp[5][cq], nsp1 = sp[6][cq], nsp2 = sp[7][cq];
+    R qcur[8];
+    rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
+                  wc[2], spw[1], dxu, dxu, dxu, dt, A.facdep, qcur);
+    R Pa, Va;
+    rhs_iface<R>(spw[0], spw[1], qprev, qcur, Pa, Va);
+    const int k = q - 1;
+    if (k >= p0 - 1 && k <= p1 && k >= 0 && (k >= p0 || k == 0)) A.va[at(k)] = (double)Va;
+    if (k >= ps) {
+      // Lagrangian update of cell k (M3:1448-1466) at its image position
+      const int kk = img(k);
+      const double xq = xe[kk + 1], xqm = xe[kk];
+      const double rho = wr[0], vn = wv[0], p = wp[0], b1 = wa[0], b2 = wc[0];
+      double vv[3], bb[3];
+      vv[D] = vn; vv[(D + 1) % 3] = vt1; vv[(D + 2) % 3] = vt2;
+      bb[D] = bnc; bb[BT1] = b1; bb[BT2] = b2;
+      const double vx = vv[0], vy = vv[1], vz = vv[2];
+      const double bx = bb[0], by = bb[1], bz = bb[2];
+      const double dm = dxu * rho;
+      const double pa = (double)Pa, pam = (double)pa_prev, va = (double)Va, vam = (double)va_prev;
+      const double xl = xq + va * dt, xlm = xqm + vam * dt;
+      const double dxl = xl - xlm;
+      const double *spl = spw[0];
+      double rhoL, vnL, eL, bl1, bl2, cmp_s[8];
+      if constexpr (Ar<R>::strict) {
+        double s4 = spl[0] + spl[1] + spl[3] + spl[4];
+        double gg = 3.0 * s4 + 5.0 * spl[2];
+        double gi = (2.0 * s4 + 2.0 * spl[2]) / gg;
+        double e = p / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
+        rhoL = dm / dxl;
+        vnL = vn - dt / dm * (pa - pam);
+        eL = e - dt / dm * (pa * va - pam * vam);
+        bl1 = b1 * dxu / dxl;
+        bl2 = b2 * dxu / dxl;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) cmp_s[f] = dxu * spl[f] / dxl;
+        cmp_s[5] = dxu * spx[0] / dxl; cmp_s[6] = dxu * spx[1] / dxl; cmp_s[7] = dxu * spx[2] / dxl;
+      } else {
+        double s4 = spl[0] + spl[1] + spl[3] + spl[4];
+        double irho = 1.0 / rho;
+        double e = p * (3.0 * s4 + 5.0 * spl[2]) / (2.0 * s4 + 2.0 * spl[2]) * irho +
+                   0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) * irho;
+        double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxu * idl;
+        rhoL = dm * idl;
+        vnL = vn - dtdm * (pa - pam);
+        eL = e - dtdm * (pa * va - pam * vam);
+        bl1 = b1 * cmp;
+        bl2 = b2 * cmp;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) cmp_s[f] = spl[f] * cmp;
+        cmp_s[5] = spx[0] * cmp; cmp_s[6] = spx[1] * cmp; cmp_s[7] = spx[2] * cmp;
+      }
+      if (k >= p0 && k <= p1) {
+        const int ck = at(k);
+        A.Uo[0][ck] = rhoL;
+#pragma unroll
+        for (int f = 0; f < 8; ++f) A.spo[f][ck] = cmp_s[f];
+        A.dxlagM[ck] = dxl;
+      }
+      o2.v[0] = rhoL; o2.v[1] = vt1; o2.v[2] = vt2; o2.v[3] = bl1; o2.v[4] = bl2; o2.v[5] = dxl;
+      o2.vn = vnL; o2.e = eL;
+      o2.mme = use_minmod_r<R>(eL, rhoL);
+      o2.mmp = use_minmod_r<R>(p, rhoL);
+      f2 = fb;
+      va2 = va;
+      if (k >= p0) {
+        // MOC slope of cell k-1 (M3:2011-2020)
+        const int km = k - 1;
+        const bool inner = km >= 1 && km <= n;
+        const bool mm = inner ? o1.mme : o1.mmp;
+        const R dxm = (R)o0.v[5], dx0 = (R)o1.v[5], dxp = (R)o2.v[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) G1[v] = vslope<R>(o0.v[v], o1.v[v], o2.v[v], dxm, dx0, dxp, mm);
+        if ((km % nblk == 0 || (km - 1) % nblk == 0) && o1.mmp != mm) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) H1[v] = vslope<R>(o0.v[v], o1.v[v], o2.v[v], dxm, dx0, dxp, o1.mmp);
+        }
+        if (k >= p0 + 1) {
+          // interface J = k-2 between cells J and J+1 (local positions 0..n)
+          const int J = k - 2;
+          const double bnJ = moc_bn(f0, f1, f2, xe[J + 1] + va0 * dt, xe, J);
+          MocV<R> L, Rv;
+          if (J % nblk != 0) {
+            L = moc_iface_r<R>(o0.v, o1.v, G0, G1, bnJ, A.facdep, dt);
+            Rv = L;
+          } else {
+            const bool mm0p = o0.mmp, mm0e = J >= 1 ? o0.mme : mm0p;
+            const bool mm1p = o1.mmp, mm1e = J + 1 <= n ? o1.mme : mm1p;
+            L = moc_iface_r<R>(o0.v, o1.v, G0, mm1p != mm1e ? H1 : G1, bnJ, A.facdep, dt);
+            if (mm0p != mm0e) Rv = moc_iface_r<R>(o0.v, o1.v, H0, G1, bnJ, A.facdep, dt);
+            else if (mm1p != mm1e) Rv = moc_iface_r<R>(o0.v, o1.v, G0, G1, bnJ, A.facdep, dt);
+            else Rv = L;
+          }
+          if (J >= p0) {
+            // MOC update of cell J (M3:2053-2095)
+            const int cJ = at(J);
+            const double Bnap = bnJ, Bnam = bnp, Vnap = va0, Vnam = vap;
+            const double dxl0 = o0.v[5], rho0 = o0.v[0];
+            const double BayL = (double)L.Bay, VayL = (double)L.Vay, BazL = (double)L.Baz, VazL = (double)L.Vaz;
+            const double BayR = (double)rvp.Bay, VayR = (double)rvp.Vay, BazR = (double)rvp.Baz, VazR = (double)rvp.Vaz;
+            if constexpr (Ar<R>::strict) {
+              A.Uo[VN][cJ] = o0.vn * rho0 + dt / dxl0 * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+              A.Uo[VT1][cJ] = o0.v[1] * rho0 + dt / dxl0 * (Bnap * BayL - Bnam * BayR);
+              A.Uo[VT2][cJ] = o0.v[2] * rho0 + dt / dxl0 * (Bnap * BazL - Bnam * BazR);
+              A.Uo[4][cJ] = o0.e * rho0 + dt / dxl0 *
+                                              ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                               (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+              A.blgo[0][cJ] = o0.v[3] + dt / dxl0 * (Bnap * VayL - Bnam * VayR);
+              A.blgo[1][cJ] = o0.v[4] + dt / dxl0 * (Bnap * VazL - Bnam * VazR);
+            } else {
+              const double fq = dt / dxl0;
+              A.Uo[VN][cJ] = o0.vn * rho0 + fq * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+              A.Uo[VT1][cJ] = o0.v[1] * rho0 + fq * (Bnap * BayL - Bnam * BayR);
+              A.Uo[VT2][cJ] = o0.v[2] * rho0 + fq * (Bnap * BazL - Bnam * BazR);
+              A.Uo[4][cJ] = o0.e * rho0 + fq * ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                                (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+              A.blgo[0][cJ] = o0.v[3] + fq * (Bnap * VayL - Bnam * VayR);
+              A.blgo[1][cJ] = o0.v[4] + fq * (Bnap * VazL - Bnam * VazR);
+            }
+          }
+          rvp = Rv;
+          bnp = bnJ;
+          vap = va0;
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { G0[v] = G1[v]; H0[v] = H1[v]; }
+      }
+      o0 = o1; o1 = o2;
+      f0 = f1; f1 = f2;
+      va0 = va1; va1 = va2;
+    }
+    // advance the RHS window by one position
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { wr[j] = wr[j + 1]; wv[j] = wv[j + 1]; wp[j] = wp[j + 1]; wa[j] = wa[j + 1]; wc[j] = wc[j + 1]; }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { spw[0][f] = spw[1][f]; spw[1][f] = spw[2][f]; }
+#pragma unroll
+    for (int f = 0; f < 8; ++f) qprev[f] = qcur[f];
+    pa_prev = Pa;
+    va_prev = Va;
+    vt1 = nvt1; vt2 = nvt2; bnc = nbnc; fb = nfb;
+    spx[0] = nsp0; spx[1] = nsp1; spx[2] = nsp2;
   }
 }
 
@@ -824,6 +1071,10 @@ __global__ __launch_bounds__(576, 7) void k_remap(SweepIO A, Geo g, double dt) {
 
 template __global__ void k_rhs_col<1, double>(SweepIO, Geo, double);
 template __global__ void k_rhs_col<2, double>(SweepIO, Geo, double);
+template __global__ void k_rhsmoc_col<1, double>(SweepIO, Geo, double);
+template __global__ void k_rhsmoc_col<2, double>(SweepIO, Geo, double);
+template __global__ void k_rhsmoc_col<1, float>(SweepIO, Geo, double);
+template __global__ void k_rhsmoc_col<2, float>(SweepIO, Geo, double);
 
 #define MCF_INST(D, R)                                                                                  \
   template __global__ void k_rhs<D, R>(SweepIO, Geo, double);                                          \
