@@ -375,7 +375,10 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // only interior cells (periodic images), so no fills precede it
     const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
     dim3 gr((unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4), (unsigned)((n + FUSE_SEG - 1) / FUSE_SEG));
-    const double by = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * 33;
+    // algorithmic bytes: 14 arrays read, 9 written (RHS+MOC); 18 read, 13
+    // written (REMAP)
+    const double cb = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2];
+    const double by = cb * 23, byr = cb * 31;
     if (x->mixed) {
       if (d == 1) KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<1, float>), gr, dim3(256), by, A, x->g, dt);
       else KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<2, float>), gr, dim3(256), by, A, x->g, dt);
@@ -384,8 +387,20 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
       else KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<2, double>), gr, dim3(256), by, A, x->g, dt);
     }
     for (int f = 0; f < 5; ++f) swapU(x, f);
-    swapSp(x); swapBlg(x);
-    CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
+    swapBlg(x);
+    // REMAP as a column march over the same segments (periodic images, no
+    // fills); it forms the Lagrangian species from the pre-sweep ones
+    A = sweep_io(x, d);
+    if (x->mixed) {
+      if (d == 1) KLAUNCH_N("k_remap", (k_remap_col<1, float>), gr, dim3(256), byr, A, x->g, dt);
+      else KLAUNCH_N("k_remap", (k_remap_col<2, float>), gr, dim3(256), byr, A, x->g, dt);
+    } else {
+      if (d == 1) KLAUNCH_N("k_remap", (k_remap_col<1, double>), gr, dim3(256), byr, A, x->g, dt);
+      else KLAUNCH_N("k_remap", (k_remap_col<2, double>), gr, dim3(256), byr, A, x->g, dt);
+    }
+    for (int f = 0; f < 5; ++f) swapU(x, f);
+    swapSp(x);
+    return MCF_OK;
   } else {
     // RHS (fills M3:1346-1357): reads rho, v(3), p, Bcc(3), species; writes
     // rho, v_n, e, Blg(2), species, interface velocity / position, widths

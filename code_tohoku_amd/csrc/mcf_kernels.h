@@ -58,9 +58,8 @@ struct SweepIO {
 
 // R = double: strict FP64; R = float: mixed precision
 template <int D, class R> __global__ void k_rhs(SweepIO A, Geo g, double dt);
-template <int D, class R> __global__ void k_rhs_col(SweepIO A, Geo g, double dt); // D = 1, 2; R = double
-constexpr int RHS_COL_SEG = 32;
 template <int D, class R> __global__ void k_rhsmoc_col(SweepIO A, Geo g, double dt); // D = 1, 2
+template <int D, class R> __global__ void k_remap_col(SweepIO A, Geo g, double dt); // D = 1, 2
 constexpr int FUSE_SEG = 32; // cells per column segment of k_rhsmoc_col
 template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);

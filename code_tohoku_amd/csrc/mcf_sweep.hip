@@ -20,6 +20,12 @@
 // before the tile (cell and interface), wave 9 the position after it
 // (cell only).  All loads stay coalesced along x.
 //
+// Along the periodic y and z axes RHS and MOC instead run as one column
+// march (k_rhsmoc_col): one thread per x column carries the stencils and
+// the Lagrangian state of RHS in registers into MOC, so only MOC's results
+// go to HBM.  The tiled RHS and MOC kernels remain for the x sweep and for
+// axes shorter than 8 cells.
+//
 // Outputs go to the other buffer of each written field (the host swaps the
 // pointers), so a tile never reads cells another tile has already updated.
 //
@@ -344,7 +350,7 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_rhs(Sw
 }
 
 // ---------------------------------------------------------------------------
-// RHS along y or z: one thread per x column marching along the sweep axis
+// RHS from register values (the column march of k_rhsmoc_col)
 // ---------------------------------------------------------------------------
 // Cell states of RHS phase 1 (M3:1380-1406) from the cell's stencil:
 // q[0..7] = QL(rho, v_n, p, |B_t|^2), QR(...).
@@ -406,138 +412,6 @@ __device__ __forceinline__ void rhs_iface(const double *sl, const double *sr, co
     gm = Ar<float>::div(5.0f * s4 + 7.0f * h2, 3.0f * s4 + 5.0f * h2);
   }
   rieman_r<R>(ql[0], ql[1], ql[2], ql[3], qr[4], qr[5], qr[6], qr[7], gm, Pa, Va);
-}
-
-constexpr int COL_SEG = RHS_COL_SEG; // positions per column segment
-
-// Column march for D = 1 (y) or 2 (z).  Thread = x column i of transverse
-// row tr, segment [p0, p1] of the sweep axis.  Each iteration loads the
-// stencil entering the window, forms the states of one cell, solves the
-// interface to its left and updates the cell left of that interface; the
-// loads of the next iteration are issued before this one's arithmetic.
-template <int D, class R>
-__global__ __launch_bounds__(256) void k_rhs_col(SweepIO A, Geo g, double dt) {
-  constexpr int VN = D + 1, VT1 = (D + 1) % 3 + 1, VT2 = (D + 2) % 3 + 1;
-  constexpr int BT1 = (D + 1) % 3, BT2 = (D + 2) % 3;
-  const int nx = g.n[0];
-  const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
-  const int i = 1 + 64 * (int)blockIdx.x + (int)(threadIdx.x & 63);
-  const int tr = 1 + 4 * (int)blockIdx.y + (int)(threadIdx.x >> 6);
-  const int p0 = 1 + COL_SEG * (int)blockIdx.z;
-  if (i > nx || tr > ntr || p0 > n) return;
-  const int p1 = min(n, p0 + COL_SEG - 1);
-  auto cell = [&](int p) { return D == 1 ? ix3s(g, i, p, tr) : ix3s(g, i, tr, p); };
-  const double *U1 = A.U[0], *UVN = A.U[VN], *U5 = A.U[4];
-  const double *B1 = A.bcc[BT1], *B2 = A.bcc[BT2];
-  const double *const *sp = A.sp;
-  const double *dxe = A.dxe, *xe = A.xe;
-  // stencil window at positions q-1, q, q+1 and the values of q+2 in flight
-  double wr[4], wv[4], wp[4], wa[4], wc[4];
-  double spw[3][5];    // species 0..4 of positions q-1, q, q+1
-  R qprev[8];          // states of cell q-1
-  R pa_prev = 0, va_prev = 0;
-  {
-    const int c0 = cell(p0 - 2), c1 = cell(p0 - 1), c2 = cell(p0), c3 = cell(p0 + 1);
-    wr[0] = U1[c0]; wr[1] = U1[c1]; wr[2] = U1[c2]; wr[3] = U1[c3];
-    wv[0] = UVN[c0]; wv[1] = UVN[c1]; wv[2] = UVN[c2]; wv[3] = UVN[c3];
-    wp[0] = U5[c0]; wp[1] = U5[c1]; wp[2] = U5[c2]; wp[3] = U5[c3];
-    wa[0] = B1[c0]; wa[1] = B1[c1]; wa[2] = B1[c2]; wa[3] = B1[c3];
-    wc[0] = B2[c0]; wc[1] = B2[c1]; wc[2] = B2[c2]; wc[3] = B2[c3];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) { spw[0][f] = sp[f][c1]; spw[1][f] = sp[f][c2]; }
-    rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
-                  wc[2], spw[0], dxe[p0 - 1], dxe[p0], dxe[p0 + 1], dt, A.facdep, qprev);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
-  }
-  // update inputs of the cell left of the current interface, loaded one
-  // iteration ahead
-  double vt1 = 0, vt2 = 0, bnc = 0, spx[3] = {0, 0, 0};
-  // q = position of the cell whose states are formed in this iteration;
-  // on entry the window holds q-1, q, q+1 and species of q-1, q
-  for (int q = p0; q <= p1 + 1; ++q) {
-    const int cl = cell(q - 1);
-    // issue the loads of the next iteration first
-    // (position q + 2 is clamped to the last ghost: its value is not used
-    // after the last iteration of the last segment)
-    const int cf = cell(min(q + 2, n + 2)), cn = cell(q + 1), cq = cell(q);
-    wr[3] = U1[cf]; wv[3] = UVN[cf]; wp[3] = U5[cf]; wa[3] = B1[cf]; wc[3] = B2[cf];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) spw[2][f] = sp[f][cn];
-    double nvt1 = 0, nvt2 = 0, nbnc = 0, nspx[3] = {0, 0, 0};
-    if (q <= p1) {
-      nvt1 = A.U[VT1][cq]; nvt2 = A.U[VT2][cq]; nbnc = A.bcc[D][cq];
-      nspx[0] = sp[5][cq]; nspx[1] = sp[6][cq]; nspx[2] = sp[7][cq];
-    }
-    const double xq = xe[q], xqm = xe[q - 1], dxq = dxe[q];
-    R qcur[8];
-    rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
-                  wc[2], spw[1], dxe[q], dxe[q + 1], dxe[q + 2], dt, A.facdep, qcur);
-    // interface between q-1 and q
-    R Pa, Va;
-    rhs_iface<R>(spw[0], spw[1], qprev, qcur, Pa, Va);
-    if (q - 1 >= p0 || q - 1 == 0) {
-      A.va[cl] = (double)Va;
-      A.xlag[cl] = xq + (double)Va * dt;
-    }
-    if (q - 1 >= p0) {
-      // Lagrangian update of cell q-1 (M3:1448-1466)
-      const double rho = wr[0], vn = wv[0], p = wp[0], b1 = wa[0], b2 = wc[0];
-      double vv[3], bb[3];
-      vv[D] = vn; vv[(D + 1) % 3] = vt1; vv[(D + 2) % 3] = vt2;
-      bb[D] = bnc; bb[BT1] = b1; bb[BT2] = b2;
-      const double vx = vv[0], vy = vv[1], vz = vv[2];
-      const double bx = bb[0], by = bb[1], bz = bb[2];
-      const double dm = dxq * rho;
-      const double pa = (double)Pa, pam = (double)pa_prev, va = (double)Va, vam = (double)va_prev;
-      const double xl = xq + va * dt, xlm = xqm + vam * dt;
-      const double dxl = xl - xlm;
-      const double *spl = spw[0];
-      if constexpr (Ar<R>::strict) {
-        double s4 = spl[0] + spl[1] + spl[3] + spl[4];
-        double gg = 3.0 * s4 + 5.0 * spl[2];
-        double gi = (2.0 * s4 + 2.0 * spl[2]) / gg;
-        double e = p / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
-        A.Uo[0][cl] = dm / dxl;
-        A.Uo[VN][cl] = vn - dt / dm * (pa - pam);
-        A.u5e[cl] = e - dt / dm * (pa * va - pam * vam);
-        A.blgo[0][cl] = b1 * dxq / dxl;
-        A.blgo[1][cl] = b2 * dxq / dxl;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) A.spo[f][cl] = dxq * spl[f] / dxl;
-#pragma unroll
-        for (int f = 0; f < 3; ++f) A.spo[5 + f][cl] = dxq * spx[f] / dxl;
-      } else {
-        double s4 = spl[0] + spl[1] + spl[3] + spl[4];
-        double irho = 1.0 / rho;
-        double e = p * (3.0 * s4 + 5.0 * spl[2]) / (2.0 * s4 + 2.0 * spl[2]) * irho +
-                   0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) * irho;
-        double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxq * idl;
-        A.Uo[0][cl] = dm * idl;
-        A.Uo[VN][cl] = vn - dtdm * (pa - pam);
-        A.u5e[cl] = e - dtdm * (pa * va - pam * vam);
-        A.blgo[0][cl] = b1 * cmp;
-        A.blgo[1][cl] = b2 * cmp;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) A.spo[f][cl] = spl[f] * cmp;
-#pragma unroll
-        for (int f = 0; f < 3; ++f) A.spo[5 + f][cl] = spx[f] * cmp;
-      }
-      A.dxlagM[cl] = dxl;
-    }
-    // advance the window by one position
-#pragma unroll
-    for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
-#pragma unroll
-    for (int f = 0; f < 5; ++f) { spw[0][f] = spw[1][f]; spw[1][f] = spw[2][f]; }
-#pragma unroll
-    for (int f = 0; f < 8; ++f) qprev[f] = qcur[f];
-    pa_prev = Pa;
-    va_prev = Va;
-    vt1 = nvt1; vt2 = nvt2; bnc = nbnc;
-#pragma unroll
-    for (int f = 0; f < 3; ++f) spx[f] = nspx[f];
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -749,7 +623,8 @@ __global__ __launch_bounds__(D == 0 ? 576 : 640, stage_waves<R>()) void k_moc(Sw
 // ---------------------------------------------------------------------------
 // One thread per x column i of transverse row tr marches along the sweep
 // axis over positions p0-4 .. p1+5 and produces the MOC results of cells
-// p0..p1 (one segment), so the Lagrangian state of RHS never goes to HBM.
+// p0..p1 (one segment), so the Lagrangian state of RHS never goes to HBM
+// (its species are formed again by k_remap_col, which follows).
 // Per iteration q: RHS states of cell q, the Riemann interface q-1|q and the
 // Lagrangian update of cell k = q-1 (RHS result O(k)); then the MOC slope of
 // cell k-1, the characteristic interface k-2|k-1 and the MOC update of cell
@@ -813,7 +688,7 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
     for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
   }
   // inputs of the Lagrangian update of cell q-1 (loaded one iteration ahead)
-  double vt1 = 0, vt2 = 0, bnc = 0, fb = 0, spx[3] = {0, 0, 0};
+  double vt1 = 0, vt2 = 0, bnc = 0, fb = 0;
   // MOC windows: RHS results of k-2, k-1, k; faces k-2, k-1, k; interface
   // velocities of k-2|k-1, k-1|k, k|k+1; slopes of k-2 and k-1 (regular and,
   // next to block edges, with the ghost switch)
@@ -831,7 +706,6 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
 #pragma unroll
     for (int f = 0; f < 5; ++f) spw[2][f] = sp[f][cn];
     const double nvt1 = A.U[VT1][cq], nvt2 = A.U[VT2][cq], nbnc = A.bcc[D][cq], nfb = UBn[cq];
-    const double nsp0 = sp[5][cq], nsp1 = sp[6][cq], nsp2 = sp[7][cq];
     R qcur[8];
     rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
                   wc[2], spw[1], dxu, dxu, dxu, dt, A.facdep, qcur);
@@ -854,7 +728,7 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
       const double xl = xq + va * dt, xlm = xqm + vam * dt;
       const double dxl = xl - xlm;
       const double *spl = spw[0];
-      double rhoL, vnL, eL, bl1, bl2, cmp_s[8];
+      double rhoL, vnL, eL, bl1, bl2;
       if constexpr (Ar<R>::strict) {
         double s4 = spl[0] + spl[1] + spl[3] + spl[4];
         double gg = 3.0 * s4 + 5.0 * spl[2];
@@ -865,9 +739,6 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
         eL = e - dt / dm * (pa * va - pam * vam);
         bl1 = b1 * dxu / dxl;
         bl2 = b2 * dxu / dxl;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) cmp_s[f] = dxu * spl[f] / dxl;
-        cmp_s[5] = dxu * spx[0] / dxl; cmp_s[6] = dxu * spx[1] / dxl; cmp_s[7] = dxu * spx[2] / dxl;
       } else {
         double s4 = spl[0] + spl[1] + spl[3] + spl[4];
         double irho = 1.0 / rho;
@@ -879,15 +750,10 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
         eL = e - dtdm * (pa * va - pam * vam);
         bl1 = b1 * cmp;
         bl2 = b2 * cmp;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) cmp_s[f] = spl[f] * cmp;
-        cmp_s[5] = spx[0] * cmp; cmp_s[6] = spx[1] * cmp; cmp_s[7] = spx[2] * cmp;
       }
       if (k >= p0 && k <= p1) {
         const int ck = at(k);
         A.Uo[0][ck] = rhoL;
-#pragma unroll
-        for (int f = 0; f < 8; ++f) A.spo[f][ck] = cmp_s[f];
         A.dxlagM[ck] = dxl;
       }
       o2.v[0] = rhoL; o2.v[1] = vt1; o2.v[2] = vt2; o2.v[3] = bl1; o2.v[4] = bl2; o2.v[5] = dxl;
@@ -972,7 +838,6 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
     pa_prev = Pa;
     va_prev = Va;
     vt1 = nvt1; vt2 = nvt2; bnc = nbnc; fb = nfb;
-    spx[0] = nsp0; spx[1] = nsp1; spx[2] = nsp2;
   }
 }
 
@@ -1069,12 +934,134 @@ __global__ __launch_bounds__(576, 7) void k_remap(SweepIO A, Geo g, double dt) {
   for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
 }
 
-template __global__ void k_rhs_col<1, double>(SweepIO, Geo, double);
-template __global__ void k_rhs_col<2, double>(SweepIO, Geo, double);
+// REMAP + primitives along a periodic axis (y or z) as a column march: one
+// thread per x column, cells p0..p1 of a segment, faces p0-1..p1.  The 15
+// remapped quantities of positions J-1..J+2 sit in registers; each
+// iteration takes one new position (loaded one iteration ahead), forms the
+// upwind flux through face J and updates cell J.  Values outside 1..n are
+// read at the periodic image (the ghost fills are copies), so no fills
+// precede this kernel.  The Lagrangian species are formed here from the
+// pre-sweep species and dx_lag, with RHS's expression for them, instead of
+// being stored by RHS.
+template <int D, class R>
+__global__ __launch_bounds__(256) void k_remap_col(SweepIO A, Geo g, double dt) {
+  const int nx = g.n[0];
+  const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
+  const int i = 1 + 64 * (int)blockIdx.x + (int)(threadIdx.x & 63);
+  const int tr = 1 + 4 * (int)blockIdx.y + (int)(threadIdx.x >> 6);
+  const int p0 = 1 + FUSE_SEG * (int)blockIdx.z;
+  if (i > nx || tr > ntr || p0 > n) return;
+  const int p1 = min(n, p0 + FUSE_SEG - 1);
+  auto img = [&](int p) { return p < 1 ? p + n : (p > n ? p - n : p); };
+  auto at = [&](int p) { return D == 1 ? ix3s(g, i, p, tr) : ix3s(g, i, tr, p); };
+  auto cell = [&](int p) { return at(img(p)); };
+  const double dxu = A.dxe[1];
+  const double *xe = A.xe;
+  // raw values of one position: rho, m(3), E, Blg(2), species(8), dx_lag
+  auto fetch = [&](int p, double *r) {
+    const int c = cell(p);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) r[v] = A.U[v][c];
+    r[5] = A.blg[0][c];
+    r[6] = A.blg[1][c];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) r[7 + f] = A.sp[f][c];
+    r[15] = A.dxlagM[c];
+  };
+  // window entry from raw values: the Lagrangian species (M3:1466)
+  auto enter = [&](const double *r, double *q, double &w) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) q[v] = r[v];
+    w = r[15];
+    if constexpr (Ar<R>::strict) {
+#pragma unroll
+      for (int f = 0; f < 8; ++f) q[7 + f] = dxu * r[7 + f] / w;
+    } else {
+      const double cmp = dxu * (1.0 / w);
+#pragma unroll
+      for (int f = 0; f < 8; ++f) q[7 + f] = r[7 + f] * cmp;
+    }
+  };
+  double Q0[15], Q1[15], Q2[15], Q3[15], W0, W1, W2, W3;
+  {
+    double r[16];
+    fetch(p0 - 2, r); enter(r, Q0, W0);
+    fetch(p0 - 1, r); enter(r, Q1, W1);
+    fetch(p0, r); enter(r, Q2, W2);
+  }
+  double nr[16];
+  fetch(p0 + 1, nr);
+  double nva = A.va[at(p0 - 1)];
+  double Fp[15];
+#pragma unroll
+  for (int v = 0; v < 15; ++v) Fp[v] = 0.0;
+  for (int J = p0 - 1; J <= p1; ++J) {
+    enter(nr, Q3, W3);
+    const double vaJ = nva;
+    fetch(J + 3, nr);
+    nva = A.va[at(min(J + 1, n))];
+    // upwind linear flux through face J (M3:1600-1640)
+    const double vadt = vaJ * dt;
+    const bool up = !(vadt <= 0.0);
+    const double u4 = up ? Q1[4] : Q2[4], u0 = up ? Q1[0] : Q2[0];
+    const double depend = up ? 0.5 * (W1 - vadt) * A.facdep : 0.5 * (W2 + vadt) * A.facdep;
+    const R dxm = (R)(up ? W0 : W1), dx0 = (R)(up ? W1 : W2), dxp = (R)(up ? W2 : W3);
+    const bool mm = use_minmod_r<R>(u4, u0);
+    const R rdep = (R)depend;
+    double F[15];
+#pragma unroll
+    for (int v = 0; v < 15; ++v) {
+      const R gr = vslope<R>(up ? Q0[v] : Q1[v], up ? Q1[v] : Q2[v], up ? Q2[v] : Q3[v], dxm, dx0, dxp, mm);
+      const double qu = up ? Q1[v] : Q2[v];
+      double face;
+      if constexpr (Ar<R>::strict) face = up ? qu + depend * gr : qu - depend * gr;
+      else face = up ? qu + (double)(rdep * gr) : qu - (double)(rdep * gr);
+      F[v] = -(vadt * face);
+    }
+    if (J >= p0) {
+      // remapped conserved state (M3:1645-1686) and primitives (M3:1291-1321)
+      const double dxE = xe[J + 1] - xe[J];
+      double Qn[15];
+#pragma unroll
+      for (int v = 0; v < 15; ++v) {
+        if constexpr (Ar<R>::strict) Qn[v] = (Q1[v] * W1 + F[v] - Fp[v]) / dxE;
+        else Qn[v] = (Q1[v] * W1 + F[v] - Fp[v]) * (1.0 / dxE);
+      }
+      const int c = at(J);
+      const double bn = A.bcc[D][c];
+      double rho = dmax(1.27 * A.rmin, Qn[0]);
+      double invd = 1.0 / rho;
+      double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
+      double s4 = Qn[7] + Qn[8] + Qn[10] + Qn[11];
+      double gg = 3.0 * s4 + 5.0 * Qn[9];
+      double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
+      double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rho;
+      double emag = 0.5 * (bn * bn + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
+      double p = (Qn[4] - ekin - emag) * gi;
+      p = dmax(A.pmin, p);
+      p = dmin(A.pmax, p);
+      A.Uo[0][c] = rho;
+      A.Uo[1][c] = vx;
+      A.Uo[2][c] = vy;
+      A.Uo[3][c] = vz;
+      A.Uo[4][c] = p;
+#pragma unroll
+      for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
+    }
+#pragma unroll
+    for (int v = 0; v < 15; ++v) { Fp[v] = F[v]; Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+    W0 = W1; W1 = W2; W2 = W3;
+  }
+}
+
 template __global__ void k_rhsmoc_col<1, double>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_col<2, double>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_col<1, float>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_col<2, float>(SweepIO, Geo, double);
+template __global__ void k_remap_col<1, double>(SweepIO, Geo, double);
+template __global__ void k_remap_col<2, double>(SweepIO, Geo, double);
+template __global__ void k_remap_col<1, float>(SweepIO, Geo, double);
+template __global__ void k_remap_col<2, float>(SweepIO, Geo, double);
 
 #define MCF_INST(D, R)                                                                                  \
   template __global__ void k_rhs<D, R>(SweepIO, Geo, double);                                          \

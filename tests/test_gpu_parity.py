@@ -107,8 +107,8 @@ def test_multistep_vs_oracle_64():
 
 
 def test_multistep_vs_oracle_long_columns():
-    """y and z longer than one column segment of the FP64 RHS (32 cells), one
-    of them not a multiple of it."""
+    """y and z longer than one column segment of the fused RHS+MOC march
+    (32 cells), one of them not a multiple of it."""
     p = small(24, 72, 40)
     st = ics.random_state(p, seed=6)
     o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
@@ -119,6 +119,32 @@ def test_multistep_vs_oracle_long_columns():
     np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
     errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
     assert worst(errs) <= MULTI_TOL, errs
+
+
+@pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
+def test_short_axis_uses_stage_kernels(precision):
+    """An axis shorter than 8 cells runs the y/z sweep as three stage
+    kernels (RHS, MOC, REMAP with fills between them) instead of the fused
+    column march; both must give the oracle's result (FP64) and agree with
+    each other to FP32 round-off (mixed)."""
+    p = small(24, 16, 6)
+    st = ics.random_state(p, seed=12)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2), precision=precision)
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    if precision == mcf.PREC_FP64:
+        dts_g = g.run(2)
+        np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+        errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+        assert worst(errs) <= MULTI_TOL, errs
+    else:
+        for dt in dts_o:
+            g.step(dt)
+        got = interior(g.download(), p)
+        ref = interior(o.state(), p)
+        for f in range(8):
+            assert np.abs(got[f] - ref[f]).max() <= 1e-5 * np.abs(ref[f]).max(), f
 
 
 @pytest.mark.parametrize("n", [64, 128])
