@@ -245,6 +245,22 @@ def main():
                 r["traffic_note"] = info
         return r
 
+    def kernel_table(rep, traffic, steps, top=8):
+        """Per-kernel achieved bandwidth (algorithmic bytes) and, where the
+        counter runs measured it, HBM traffic per launch."""
+        per = traffic[0] if traffic is not None else None
+        rows = []
+        for name, (cnt, ms, by) in sorted(rep.items(), key=lambda kv: -kv[1][1])[:top]:
+            row = {"kernel": name, "launches_per_step": cnt / steps, "avg_launch_us": round(1e3 * ms / cnt, 2),
+                   "ms_per_step": round(ms / steps, 4), "alg_bytes_per_launch": by / cnt,
+                   "achieved_GBs": round(by / (ms * 1e-3) / 1e9, 1),
+                   "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            if per is not None and name in per:
+                row["traffic_bytes_per_launch"] = per[name]["bytes"]
+                row["traffic_GBs"] = round(per[name]["bytes"] / (ms / cnt * 1e-3) / 1e9, 1)
+            rows.append(row)
+        return rows
+
     step_gbs = ALG_BYTES_PER_ZONE * n ** 3 / (ms_step * 1e-3) / 1e9
     out = {"metric": "MHD zone-updates/s at 256^3 (mixed+surrogate vs FP64) + % HBM roofline",
            "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -254,8 +270,10 @@ def main():
                       "block_split": list(nsplit), "precision": "mixed", "cooling": "surrogate",
                       "surrogate_weights": wsrc},
            "roofline": roof(rep, pmc),
+           "kernels": kernel_table(rep, pmc, args.steps),
            "speedup_vs_fp64": value / value64,
            "fp64": {"value": value64, "ms_per_step": 1e3 * el64 / args.steps, "roofline": roof(rep64, None),
+                    "kernels": kernel_table(rep64, None, args.steps),
                     "dt_mean": float(sum(dts64) / len(dts64))},
            "surrogate": {"backward_euler_cells": nstiff, "imc_fallbacks": nfall,
                          "fallback_fraction": (nfall / nstiff) if nstiff else None,

@@ -380,11 +380,11 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     const double cb = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2];
     const double by = cb * 23, byr = cb * 31;
     if (x->mixed) {
-      if (d == 1) KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<1, float>), gr, dim3(256), by, A, x->g, dt);
-      else KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<2, float>), gr, dim3(256), by, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<1, float>), gr, dim3(256), by, A, x->g, dt);
+      else KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<2, float>), gr, dim3(256), by, A, x->g, dt);
     } else {
-      if (d == 1) KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<1, double>), gr, dim3(256), by, A, x->g, dt);
-      else KLAUNCH_N("k_rhsmoc", (k_rhsmoc_col<2, double>), gr, dim3(256), by, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<1, double>), gr, dim3(256), by, A, x->g, dt);
+      else KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<2, double>), gr, dim3(256), by, A, x->g, dt);
     }
     for (int f = 0; f < 5; ++f) swapU(x, f);
     swapBlg(x);
@@ -392,11 +392,11 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // fills); it forms the Lagrangian species from the pre-sweep ones
     A = sweep_io(x, d);
     if (x->mixed) {
-      if (d == 1) KLAUNCH_N("k_remap", (k_remap_col<1, float>), gr, dim3(256), byr, A, x->g, dt);
-      else KLAUNCH_N("k_remap", (k_remap_col<2, float>), gr, dim3(256), byr, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_remap_col", (k_remap_col<1, float>), gr, dim3(256), byr, A, x->g, dt);
+      else KLAUNCH_N("k_remap_col", (k_remap_col<2, float>), gr, dim3(256), byr, A, x->g, dt);
     } else {
-      if (d == 1) KLAUNCH_N("k_remap", (k_remap_col<1, double>), gr, dim3(256), byr, A, x->g, dt);
-      else KLAUNCH_N("k_remap", (k_remap_col<2, double>), gr, dim3(256), byr, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_remap_col", (k_remap_col<1, double>), gr, dim3(256), byr, A, x->g, dt);
+      else KLAUNCH_N("k_remap_col", (k_remap_col<2, double>), gr, dim3(256), byr, A, x->g, dt);
     }
     for (int f = 0; f < 5; ++f) swapU(x, f);
     swapSp(x);
@@ -513,7 +513,11 @@ static int do_source(mcf_ctx *x, double dt) {
     }
     const Box &b = stage == 0 ? bc : bg;
     KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
-    KLAUNCH(k_src_cell, boxsize(b), 9, A, x->g, b, dt, stage);
+    {
+      dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + 7) / 8), (unsigned)((b.cnt[2] + SRC_ZSEG - 1) / SRC_ZSEG));
+      if (x->mixed) KLAUNCH_N("k_src_col", k_src_col<float>, gr, dim3(512), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
+      else KLAUNCH_N("k_src_col", k_src_col<double>, gr, dim3(512), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
+    }
     if (A.sur) {
       KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, stage);
       KLAUNCH_G(k_src_imc, dim3(256), dim3(256), 0, A, dt, stage, 1);

@@ -18,6 +18,22 @@ struct Box {
   int cnt[3];
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Renumber them so that every XCD takes one contiguous range of tiles,
+// ordered with the first two grid axes fastest: tiles that share halo rows
+// (neighbours along x for the x sweep, along the sweep axis for y/z) then
+// run on the same XCD at about the same time.  The renumbering is a
+// bijection; grids whose tile count is not a multiple of 8 keep the
+// dispatch order.
+__device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, nt = gx * gy * gridDim.z;
+  unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  if ((nt & 7u) == 0u) t = (t & 7u) * (nt >> 3) + (t >> 3);
+  bx = (int)(t % gx);
+  by = (int)((t / gx) % gy);
+  bz = (int)(t / (gx * gy));
+}
+
 struct FillList {
   double *a[16];
   int nf;
@@ -113,7 +129,8 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_cct(CtArgs A, Geo g, Box b, int orient, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
-__global__ void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage);
+template <class R> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);
+constexpr int SRC_ZSEG = 32; // planes per block of k_src_col
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
 __global__ void k_src_imc(SrcArgs A, double dt, int stage, int which);
 __global__ void k_zero_u32x2(unsigned *p);

@@ -298,54 +298,156 @@ __device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, doubl
 // A.Pn, result written over A.Pn (= A.Pout; each cell reads only its own
 // P_old there); its threads on the first ghost layer copy the predictor's
 // ghost values across, so the ghosts end as the reference leaves them.
-// The temperature of a cell and of its six neighbours is computed on the
-// fly.  The backward-Euler cells (dt > 0.2*t_cool) are not solved here:
-// they are queued with their conduction increment for k_src_stiff (the
-// surrogate) or k_src_imc, so the bisection runs on dense waves.
-__global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, double dt, int stage) {
-  int i, j, k;
-  bool valid = box_ijk(b, i, j, k);
-  if (valid && stage == 1 && (i == 0 || j == 0 || k == 0 || i > g.n[0] || j > g.n[1] || k > g.n[2])) {
-    const int cg = ix3(g, i, j, k);
-    A.Pout[cg] = A.Tp[cg];
-    valid = false;
-  }
-  if (!valid) { i = j = k = 1; }
-  int c = ix3(g, i, j, k);
+// The backward-Euler cells (dt > 0.2*t_cool) are not solved here: they are
+// queued with their conduction increment for k_src_stiff (the surrogate) or
+// k_src_imc, so the bisection runs on dense waves.
+//
+// The cells are visited as a march along z: a block is 8 x rows of 64 cells
+// (one wave per row) over SRC_ZSEG planes.  The temperature, gamma-1 and n
+// of a thread's own column at k-1, k, k+1 stay in registers; the x
+// neighbours' temperatures come from the adjacent lanes (the row's first
+// and last lane compute their outer neighbour), the y neighbours' from an
+// LDS ring of the block's temperature planes (its 8 rows and the row on
+// either side, computed by the first and last wave), so every temperature
+// is formed once per block.
+// R = double: the reference's arithmetic, operation by operation.
+// R = float (mixed precision): temperatures and their differences in FP64,
+// the rates (cooling function, conduction fluxes, time-step tests) in FP32,
+// the pressure accumulated in FP64.
+template <class R> struct SrcW {
+  double T;
+  R gi, ns;
+};
+template <class R> __device__ __forceinline__ SrcW<R> src_w(const SrcArgs &A, int c) {
   const double *const *sp = A.sp;
-  const int sy = g.st[1], sz = g.st[2];
-  const double T = src_T(A, c);
-  const double Txp = src_T(A, c + 1), Txm = src_T(A, c - 1);
-  const double Typ = src_T(A, c + sy), Tym = src_T(A, c - sy);
-  const double Tzp = src_T(A, c + sz), Tzm = src_T(A, c - sz);
-  double rho = A.rho[c];
-  double CooL = fcool_ki(rho, T);
-  double gi = gammi1_of(sp, c);
-  double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
-  double Pn = A.Pn[c];
-  const bool expl = dt <= 0.2 * Pn / (gi * fabs(CooL));
-  double u = 0.0;
-  if (stage == 0) u = Pn - gi * CooL * dt * 0.5;
-  else u = Pn - gi * CooL * dt;
-  double dxi = A.gdx[0][i + 1], dyj = A.gdx[1][j + 1], dzk = A.gdx[2][k + 1];
-  double tcd = gi * KCOND * sqrt(T);
-  tcd = 0.49 * nsum * KB / tcd * (dxi * dxi);
-  tcd = copysign(0.5, tcd - dt);
-  double qxp = qflux(T, Txp, dxi, A.gdx[0][i + 2]);
-  double qxm = qflux(Txm, T, A.gdx[0][i], dxi);
-  double qyp = qflux(T, Typ, dyj, A.gdx[1][j + 2]);
-  double qym = qflux(Tym, T, A.gdx[1][j], dyj);
-  double qzp = qflux(T, Tzp, dzk, A.gdx[2][k + 2]);
-  double qzm = qflux(Tzm, T, A.gdx[2][k], dzk);
-  double div = (qxp - qxm) / dxi + (qyp - qym) / dyj + (qzp - qzm) / dzk;
-  double dc;
-  if (stage == 0) dc = (tcd + 0.5) * gi * dt * 0.5 * div;
-  else dc = (tcd + 0.5) * gi * dt * div;
-  {
+  const double s0 = sp[0][c], s1 = sp[1][c], s2 = sp[2][c], s3 = sp[3][c], s4 = sp[4][c];
+  SrcW<R> w;
+  w.T = A.Tp[c] / (KB * (s1 + s0 + s2 + s3 + s4));
+  const double q4 = s0 + s1 + s3 + s4;
+  if constexpr (Ar<R>::strict) {
+    w.gi = (2.0 * q4 + 2.0 * s2) / (3.0 * q4 + 5.0 * s2);
+  } else {
+    w.gi = Ar<float>::div((float)(2.0 * q4 + 2.0 * s2), (float)(3.0 * q4 + 5.0 * s2));
+  }
+  w.ns = (R)(s0 + s1 + s3 + s4 + s2);
+  return w;
+}
+
+__device__ __forceinline__ float fcool_ki_f(float rho, float T) {
+  const float N = rho * (1.0f / 1.27f);
+  float Laml = 1.0e7f * __expf(Ar<float>::div(-1.184e5f, T * 1.0e3f + 1.0e3f));
+  Laml = fminf(Laml, 5.0e3f);
+  const float Lamc = 1.4e-2f * Ar<float>::sqrt_(T * 1.0e3f) * __expf(Ar<float>::div(-9.2e1f, T * 1.0e3f));
+  return (N * N * (Laml + Lamc) - N) * 3.94656e1f;
+}
+
+// conduction flux of qflux with the weights and the temperature difference
+// in FP64, the rest in FP32
+__device__ __forceinline__ float qflux_f(double Tc, double Tcs, double dxi, double dxp) {
+  const float rt = Ar<float>::sqrt_(Ar<float>::div((float)(dxi * Tcs + dxp * Tc), (float)(dxi + dxp)));
+  return (float)KCOND * fminf(rt, 3.873f) * (float)(Tcs - Tc) * Ar<float>::div(2.0f, (float)(dxi + dxp));
+}
+
+template <class R>
+__global__ __launch_bounds__(512) void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage) {
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int lane = threadIdx.x & 63;
+  const int hi0 = b.lo[0] + b.cnt[0] - 1, hi1 = b.lo[1] + b.cnt[1] - 1, hi2 = b.lo[2] + b.cnt[2] - 1;
+  const int i = b.lo[0] + 64 * bx + lane;
+  const int j = b.lo[1] + 8 * by + (int)(threadIdx.x >> 6);
+  const int k0 = b.lo[2] + SRC_ZSEG * bz;
+  const int k1 = min(hi2, k0 + SRC_ZSEG - 1);
+  // rows past the box repeat the last row and lanes past the box load cell
+  // hi0+1 (the last cell's right neighbour); neither stores anything
+  const bool own = i <= hi0 && j <= hi1;
+  const int ic = min(i, hi0 + 1), jc = min(j, hi1 + 1);
+  const int w = (int)(threadIdx.x >> 6);
+  const int j0 = b.lo[1] + 8 * by;
+  // temperature planes k, k+1, k+2 of rows j0-1 .. j0+8 (slot r+1 = row j0+r)
+  __shared__ double sT[3][10][64];
+  // the first and last wave also form the rows outside the block
+  const int jh = w == 0 ? j0 - 1 : min(j0 + 8, hi1 + 1);
+  const int rh = w == 0 ? 0 : 9;
+  const bool halo = w == 0 || w == 7;
+  const int sz = g.st[2];
+  const double dxi = A.gdx[0][ic + 1], dxp = A.gdx[0][ic + 2], dxm = A.gdx[0][ic];
+  const double dyj = A.gdx[1][jc + 1], dyp = A.gdx[1][jc + 2], dym = A.gdx[1][jc];
+  const bool ghost_ij = i == 0 || j == 0 || i > g.n[0] || j > g.n[1];
+  SrcW<R> wm = src_w<R>(A, ix3(g, ic, jc, k0 - 1));
+  SrcW<R> wc = src_w<R>(A, ix3(g, ic, jc, k0));
+  SrcW<R> wq = src_w<R>(A, ix3(g, ic, jc, k0 + 1));
+  sT[k0 % 3][w + 1][lane] = wc.T;
+  sT[(k0 + 1) % 3][w + 1][lane] = wq.T;
+  if (halo) {
+    sT[k0 % 3][rh][lane] = src_T(A, ix3(g, ic, jh, k0));
+    sT[(k0 + 1) % 3][rh][lane] = src_T(A, ix3(g, ic, jh, k0 + 1));
+  }
+  for (int k = k0; k <= k1; ++k) {
+    const int c = ix3(g, ic, jc, k);
+    // slot of plane k+2 was last read for plane k-1, before this barrier
+    __syncthreads();
+    SrcW<R> wn{};
+    if (k < k1) {
+      wn = src_w<R>(A, c + 2 * sz);
+      sT[(k + 2) % 3][w + 1][lane] = wn.T;
+      if (halo) sT[(k + 2) % 3][rh][lane] = src_T(A, ix3(g, ic, jh, k + 2));
+    }
+    const double T = wc.T;
+    const double Txu = __shfl_up(T, 1, 64), Txd = __shfl_down(T, 1, 64);
+    double Te = 0.0;
+    if (lane == 0 || lane == 63) Te = src_T(A, lane == 0 ? c - 1 : c + 1);
+    const double Txm = lane == 0 ? Te : Txu, Txp = lane == 63 ? Te : Txd;
+    // plane k was written two iterations (two barriers) ago
+    const double Typ = sT[k % 3][w + 2][lane], Tym = sT[k % 3][w][lane];
+    const double Tzp = wq.T, Tzm = wm.T;
+    const bool ghost = stage == 1 && (ghost_ij || k == 0 || k > g.n[2]);
+    if (own && ghost) A.Pout[c] = A.Tp[c];
+    const bool valid = own && !ghost;
+    const double rho = A.rho[c];
+    const double Pn = A.Pn[c];
+    const double dzk = A.gdx[2][k + 1], dzp = A.gdx[2][k + 2], dzm = A.gdx[2][k];
+    double u, dc;
+    bool expl;
+    if constexpr (Ar<R>::strict) {
+      const double CooL = fcool_ki(rho, T);
+      const double gi = wc.gi, nsum = wc.ns;
+      expl = dt <= 0.2 * Pn / (gi * fabs(CooL));
+      u = stage == 0 ? Pn - gi * CooL * dt * 0.5 : Pn - gi * CooL * dt;
+      double tcd = gi * KCOND * sqrt(T);
+      tcd = 0.49 * nsum * KB / tcd * (dxi * dxi);
+      tcd = copysign(0.5, tcd - dt);
+      const double qxp = qflux(T, Txp, dxi, dxp);
+      const double qxm = qflux(Txm, T, dxm, dxi);
+      const double qyp = qflux(T, Typ, dyj, dyp);
+      const double qym = qflux(Tym, T, dym, dyj);
+      const double qzp = qflux(T, Tzp, dzk, dzp);
+      const double qzm = qflux(Tzm, T, dzm, dzk);
+      const double div = (qxp - qxm) / dxi + (qyp - qym) / dyj + (qzp - qzm) / dzk;
+      dc = stage == 0 ? (tcd + 0.5) * gi * dt * 0.5 * div : (tcd + 0.5) * gi * dt * div;
+    } else {
+      const float Tf = (float)T, dtf = (float)dt;
+      const float CooL = fcool_ki_f((float)rho, Tf);
+      const float gi = wc.gi, nsum = wc.ns;
+      expl = dtf <= Ar<float>::div(0.2f * (float)Pn, gi * fabsf(CooL));
+      const float cool = stage == 0 ? gi * CooL * dtf * 0.5f : gi * CooL * dtf;
+      u = Pn - (double)cool;
+      const float dxf = (float)dxi;
+      float tcd = Ar<float>::div(0.49f * nsum * (float)KB, gi * (float)KCOND * Ar<float>::sqrt_(Tf)) * (dxf * dxf);
+      tcd = copysignf(0.5f, tcd - dtf);
+      const float qxp = qflux_f(T, Txp, dxi, dxp);
+      const float qxm = qflux_f(Txm, T, dxm, dxi);
+      const float qyp = qflux_f(T, Typ, dyj, dyp);
+      const float qym = qflux_f(Tym, T, dym, dyj);
+      const float qzp = qflux_f(T, Tzp, dzk, dzp);
+      const float qzm = qflux_f(Tzm, T, dzm, dzk);
+      const float div = (qxp - qxm) * __builtin_amdgcn_rcpf(dxf) + (qyp - qym) * __builtin_amdgcn_rcpf((float)dyj) +
+                        (qzp - qzm) * __builtin_amdgcn_rcpf((float)dzk);
+      dc = (double)(stage == 0 ? (tcd + 0.5f) * gi * dtf * 0.5f * div : (tcd + 0.5f) * gi * dtf * div);
+    }
     const bool stiff = valid && !expl;
     const unsigned long long m = __ballot(stiff);
     if (m) {
-      const int lane = threadIdx.x & 63;
       const int leader = __ffsll((long long)m) - 1;
       unsigned base = 0;
       if (lane == leader) base = atomicAdd(A.qn, (unsigned)__popcll(m));
@@ -356,18 +458,22 @@ __global__ __launch_bounds__(256) void k_src_cell(SrcArgs A, Geo g, Box b, doubl
         A.qdc[c] = dc;
       }
     }
-    if (!valid || stiff) return;
+    if (valid && !stiff) {
+      u = u + dc;
+      u = dmax(u, A.pmin);
+      u = dmin(u, A.pmax);
+      A.Pout[c] = u;
+    }
+    wm = wc; wc = wq; wq = wn;
   }
-  u = u + dc;
-  u = dmax(u, A.pmin);
-  u = dmin(u, A.pmax);
-  A.Pout[c] = u;
 }
+template __global__ void k_src_col<double>(SrcArgs, Geo, Box, double, int);
+template __global__ void k_src_col<float>(SrcArgs, Geo, Box, double, int);
 
 // The queued backward-Euler cells of one SOURCE stage, 64 per wave: the
 // surrogate predicts, Newton steps refine, the residual test accepts; the
 // cell's conduction increment and the pressure clamps are then applied as
-// in k_src_cell.  Rejected cells go to a second queue for k_src_imc,
+// in k_src_col.  Rejected cells go to a second queue for k_src_imc,
 // so IMC runs on dense waves.  Waves stride over the queue.
 __device__ __forceinline__ void src_finish(const SrcArgs &A, int c, double Pi) {
   double u = Pi + A.qdc[c];

@@ -82,22 +82,6 @@ struct Roles {
   bool u_ok;         // phase 3: cell 1..N updated by this thread
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Renumber them so that every XCD takes one contiguous range of tiles,
-// ordered with the first two grid axes fastest: tiles that share halo rows
-// (neighbours along x for the x sweep, along the sweep axis for y/z) then
-// run on the same XCD at about the same time.  The renumbering is a
-// bijection; grids whose tile count is not a multiple of 8 keep the
-// dispatch order.
-__device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
-  const unsigned gx = gridDim.x, gy = gridDim.y, nt = gx * gy * gridDim.z;
-  unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  if ((nt & 7u) == 0u) t = (t & 7u) * (nt >> 3) + (t >> 3);
-  bx = (int)(t % gx);
-  by = (int)((t / gx) % gy);
-  bz = (int)(t / (gx * gy));
-}
-
 template <int D>
 __device__ __forceinline__ Roles roles(const Geo &g) {
   Roles r;
