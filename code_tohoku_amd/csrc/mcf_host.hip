@@ -370,7 +370,31 @@ static dim3 sweep_grid(const mcf_ctx *x, int d) {
 // MHD (M3:1264-1324): RHS -> MOC -> REMAP -> primitives
 static int do_sweep(mcf_ctx *x, int d, double dt) {
   SweepIO A = sweep_io(x, d);
-  if (x->fuse[d]) {
+  if (d == 0 && x->fuse[0]) {
+    // x: RHS and MOC in one kernel, waves of 64 cells along x exchanging
+    // neighbours by shuffles; the ghost cells both stages read are filled
+    // first (rho, v(3), p, face B_x, species), as the two stages' fills do
+    int l6[6] = {1, 2, 3, 4, 5, 6};
+    CHK(fillU(x, 1, 2, 1, l6, 6));
+    CHK(fillOT(x, 1, 2, OT_SPECIES));
+    A = sweep_io(x, 0);
+    XSide S{};
+    const int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
+    for (int s = 0; s < 2; ++s) {
+      S.inflow[s] = bc[s] == 4;
+      S.blg[s][0] = x->bbrv[7][1][s + 1];
+      S.blg[s][1] = x->bbrv[8][1][s + 1];
+      S.dxl[s] = x->hdx[0][s == 0 ? 1 : x->g.n[0] + 2];
+    }
+    const int nx = x->g.n[0];
+    dim3 gr((unsigned)((nx + XCELLS - 1) / XCELLS), (unsigned)((x->g.n[1] + 3) / 4), (unsigned)x->g.n[2]);
+    const double by = 8.0 * (double)nx * x->g.n[1] * x->g.n[2] * 33;
+    if (x->mixed) KLAUNCH_N("k_rhsmoc_x", k_rhsmoc_x<float>, gr, dim3(256), by, A, x->g, S, dt);
+    else KLAUNCH_N("k_rhsmoc_x", k_rhsmoc_x<double>, gr, dim3(256), by, A, x->g, S, dt);
+    for (int f = 0; f < 5; ++f) swapU(x, f);
+    swapSp(x); swapBlg(x);
+    CHK(fillOT(x, 1, 2, OT_DXLAG));
+  } else if (x->fuse[d]) {
     // y/z: RHS and MOC in one column march over the periodic axis; it reads
     // only interior cells (periodic images), so no fills precede it
     const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
@@ -599,9 +623,11 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
   x->red = p; p += 16 + 2 * DT_BLOCKS;
   HIPCHK(hipMemsetAsync(x->pool, 0, total * sizeof(double), x->stream));
-  for (int d = 1; d < 3; ++d) {
+  for (int d = 0; d < 3; ++d) {
     bool uni = n[d] >= 8;
     for (int k = 0; k < n[d] + 4 && uni; ++k) uni = std::memcmp(&x->hdx[d][k], &x->hdx[d][0], sizeof(double)) == 0;
+    // x with inflow on both sides needs no periodic images, so no uniform spacing
+    if (d == 0 && n[0] >= 8 && cfg->bcx1 == 4 && cfg->bcx2 == 4) uni = true;
     x->fuse[d] = uni;
   }
   for (int d = 0; d < 3; ++d) {

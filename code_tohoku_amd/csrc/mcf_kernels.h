@@ -72,10 +72,21 @@ struct SweepIO {
   double spmin[8];
 };
 
+// Ghost values of the x sweep's RHS results as MOC reads them, per side
+// (0 = left, 1 = right): inflow sides (BC=4) take the fill rules of the MOC
+// stage, periodic sides the periodic images.
+struct XSide {
+  int inflow[2];
+  double blg[2][2]; // Blg(t) ghost value per side
+  double dxl[2];    // dx_lag ghost value per side
+};
+
 // R = double: strict FP64; R = float: mixed precision
 template <int D, class R> __global__ void k_rhs(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_rhsmoc_col(SweepIO A, Geo g, double dt); // D = 1, 2
 template <int D, class R> __global__ void k_remap_col(SweepIO A, Geo g, double dt); // D = 1, 2
+template <class R> __global__ void k_rhsmoc_x(SweepIO A, Geo g, XSide S, double dt);
+constexpr int XHALO = 4, XCELLS = 64 - 2 * XHALO; // k_rhsmoc_x: halo and cells per wave
 constexpr int FUSE_SEG = 32; // cells per column segment of k_rhsmoc_col
 template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);

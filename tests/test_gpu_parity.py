@@ -121,6 +121,39 @@ def test_multistep_vs_oracle_long_columns():
     assert worst(errs) <= MULTI_TOL, errs
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_multistep_vs_oracle_long_rows(periodic):
+    """x longer than one wave of the fused x sweep (56 cells), not a multiple
+    of it, with inflow and with periodic x boundaries."""
+    p = small(72, 16, 16, periodic=periodic)
+    st = ics.random_state(p, seed=31 if periodic else 32)
+    o = oracle.Oracle(p, st, nsplit=(3, 2, 2))
+    g = mcf.Domain(p, nsplit=(3, 2, 2))
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
+def test_nonuniform_periodic_x_uses_stage_kernels():
+    """Periodic x with two spacings (16 cells over 10 pc, 8 over 10 pc) runs
+    the x sweep as the three stage kernels; FP64 result against the oracle."""
+    p = Input3D(np1x=16, np2x=8, np1y=8, np2y=8, np1z=8, np2z=8,
+                ql1x=10.0, ql2x=10.0, ql1y=5.0, ql2y=5.0, ql1z=5.0, ql2z=5.0)
+    p.bcx1 = p.bcx2 = 3
+    st = ics.random_state(p, seed=33)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2))
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
 @pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
 def test_short_axis_uses_stage_kernels(precision):
     """An axis shorter than 8 cells runs the y/z sweep as three stage
