@@ -451,7 +451,14 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
   CHK(fillU(x, 1 << d, 2, 2, l5, 5));
   CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
   A = sweep_io(x, d);
-  SWEEP_LAUNCH(k_remap, d, false, 31, A, x->g, dt);
+  if (d == 0 && x->fuse[0]) {
+    dim3 gr((unsigned)((x->g.n[0] + XRCELLS - 1) / XRCELLS), (unsigned)((x->g.n[1] + 3) / 4), (unsigned)x->g.n[2]);
+    const double by = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * 31;
+    if (x->mixed) KLAUNCH_N("k_remap_x", k_remap_x<float>, gr, dim3(256), by, A, x->g, dt);
+    else KLAUNCH_N("k_remap_x", k_remap_x<double>, gr, dim3(256), by, A, x->g, dt);
+  } else {
+    SWEEP_LAUNCH(k_remap, d, false, 31, A, x->g, dt);
+  }
   for (int f = 0; f < 5; ++f) swapU(x, f);
   swapSp(x);
   return MCF_OK;

@@ -1224,6 +1224,76 @@ __global__ __launch_bounds__(256) void k_rhsmoc_x(SweepIO A, Geo g, XSide S, dou
   }
 }
 
+// REMAP + primitives along x with the layout of k_rhsmoc_x: a wave holds
+// positions c0-2 .. c0+61 of one row and updates cells c0 .. c0+59.  Lane p
+// forms the upwind flux through face p|p+1 from cells p-1..p+2 (neighbours
+// by shuffles) and updates cell p with the flux of face p-1|p from the lane
+// below.  Ghost cells are read as the REMAP stage's fills leave them.
+template <class R>
+__global__ __launch_bounds__(256) void k_remap_x(SweepIO A, Geo g, double dt) {
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int lane = threadIdx.x & 63;
+  const int n = g.n[0];
+  const int j = 1 + 4 * by + (int)(threadIdx.x >> 6), k = 1 + bz;
+  if (j > g.n[1]) return; // whole waves
+  const int c0 = 1 + XRCELLS * bx;
+  const int p = c0 - 2 + lane;
+  const int c = ix3s(g, min(max(p, -1), n + 2), j, k);
+  double Q[15];
+#pragma unroll
+  for (int v = 0; v < 15; ++v) Q[v] = remap_field(A, v)[c];
+  const double W1 = A.dxlagM[c];
+  const double W0 = sh_up(W1), W2 = sh_dn(W1), W3 = sh_dn(W2);
+  // upwind linear flux through face p|p+1 (M3:1600-1640)
+  const double vadt = A.va[ix3s(g, min(max(p, 0), n), j, k)] * dt;
+  const bool up = !(vadt <= 0.0);
+  const double u4n = sh_dn(Q[4]), u0n = sh_dn(Q[0]);
+  const double u4 = up ? Q[4] : u4n, u0 = up ? Q[0] : u0n;
+  const double depend = up ? 0.5 * (W1 - vadt) * A.facdep : 0.5 * (W2 + vadt) * A.facdep;
+  const R dxm = (R)(up ? W0 : W1), dx0 = (R)(up ? W1 : W2), dxp = (R)(up ? W2 : W3);
+  const bool mm = use_minmod_r<R>(u4, u0);
+  const R rdep = (R)depend;
+  const bool own = lane >= 2 && lane < 2 + XRCELLS && p <= n;
+  const double dxE = own ? A.xe[p + 1] - A.xe[p] : 1.0;
+  double Qn[15];
+#pragma unroll
+  for (int v = 0; v < 15; ++v) {
+    const double q0 = sh_up(Q[v]), q1 = Q[v], q2 = sh_dn(Q[v]), q3 = sh_dn(q2);
+    const R gr = vslope<R>(up ? q0 : q1, up ? q1 : q2, up ? q2 : q3, dxm, dx0, dxp, mm);
+    const double qu = up ? q1 : q2;
+    double face;
+    if constexpr (Ar<R>::strict) face = up ? qu + depend * gr : qu - depend * gr;
+    else face = up ? qu + (double)(rdep * gr) : qu - (double)(rdep * gr);
+    const double F = -(vadt * face);
+    const double Fl = sh_up(F);
+    // remapped conserved state (M3:1645-1686)
+    if constexpr (Ar<R>::strict) Qn[v] = (q1 * W1 + F - Fl) / dxE;
+    else Qn[v] = (q1 * W1 + F - Fl) * (1.0 / dxE);
+  }
+  if (!own) return;
+  // primitives (M3:1291-1321)
+  const double bn = A.bcc[0][c];
+  double rho = dmax(1.27 * A.rmin, Qn[0]);
+  double invd = 1.0 / rho;
+  double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
+  double s4 = Qn[7] + Qn[8] + Qn[10] + Qn[11];
+  double gg = 3.0 * s4 + 5.0 * Qn[9];
+  double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
+  double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rho;
+  double emag = 0.5 * (bn * bn + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
+  double pr = (Qn[4] - ekin - emag) * gi;
+  pr = dmax(A.pmin, pr);
+  pr = dmin(A.pmax, pr);
+  A.Uo[0][c] = rho;
+  A.Uo[1][c] = vx;
+  A.Uo[2][c] = vy;
+  A.Uo[3][c] = vz;
+  A.Uo[4][c] = pr;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
+}
+
 template __global__ void k_rhsmoc_col<1, double>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_col<2, double>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_col<1, float>(SweepIO, Geo, double);
@@ -1234,6 +1304,8 @@ template __global__ void k_remap_col<1, float>(SweepIO, Geo, double);
 template __global__ void k_remap_col<2, float>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_x<double>(SweepIO, Geo, XSide, double);
 template __global__ void k_rhsmoc_x<float>(SweepIO, Geo, XSide, double);
+template __global__ void k_remap_x<double>(SweepIO, Geo, double);
+template __global__ void k_remap_x<float>(SweepIO, Geo, double);
 
 #define MCF_INST(D, R)                                                                                  \
   template __global__ void k_rhs<D, R>(SweepIO, Geo, double);                                          \
