@@ -398,7 +398,7 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // y/z: RHS and MOC in one column march over the periodic axis; it reads
     // only interior cells (periodic images), so no fills precede it
     const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
-    dim3 gr((unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4), (unsigned)((n + FUSE_SEG - 1) / FUSE_SEG));
+    dim3 gr((unsigned)((n + FUSE_SEG - 1) / FUSE_SEG), (unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4));
     // algorithmic bytes: 14 arrays read, 9 written (RHS+MOC); 18 read, 13
     // written (REMAP)
     const double cb = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2];

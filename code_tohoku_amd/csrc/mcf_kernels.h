@@ -89,7 +89,7 @@ template <class R> __global__ void k_rhsmoc_x(SweepIO A, Geo g, XSide S, double 
 constexpr int XHALO = 4, XCELLS = 64 - 2 * XHALO; // k_rhsmoc_x: halo and cells per wave
 template <class R> __global__ void k_remap_x(SweepIO A, Geo g, double dt);
 constexpr int XRCELLS = 60; // k_remap_x: cells per wave (halo 2)
-constexpr int FUSE_SEG = 32; // cells per column segment of k_rhsmoc_col
+constexpr int FUSE_SEG = 64; // cells per column segment of k_rhsmoc_col
 template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);
 

@@ -635,9 +635,13 @@ __global__ __launch_bounds__(256) void k_rhsmoc_col(SweepIO A, Geo g, double dt)
   constexpr int BT1 = (D + 1) % 3, BT2 = (D + 2) % 3;
   const int nx = g.n[0];
   const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
-  const int i = 1 + 64 * (int)blockIdx.x + (int)(threadIdx.x & 63);
-  const int tr = 1 + 4 * (int)blockIdx.y + (int)(threadIdx.x >> 6);
-  const int p0 = 1 + FUSE_SEG * (int)blockIdx.z;
+  // grid: segments fastest, so the segments of a column (which share halo
+  // cells) run together on one XCD
+  int bseg, bxc, btr;
+  xcd_tile(bseg, bxc, btr);
+  const int i = 1 + 64 * bxc + (int)(threadIdx.x & 63);
+  const int tr = 1 + 4 * btr + (int)(threadIdx.x >> 6);
+  const int p0 = 1 + FUSE_SEG * bseg;
   if (i > nx || tr > ntr || p0 > n) return;
   const int p1 = min(n, p0 + FUSE_SEG - 1);
   const int nblk = A.nblk;
@@ -931,9 +935,13 @@ template <int D, class R>
 __global__ __launch_bounds__(256) void k_remap_col(SweepIO A, Geo g, double dt) {
   const int nx = g.n[0];
   const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
-  const int i = 1 + 64 * (int)blockIdx.x + (int)(threadIdx.x & 63);
-  const int tr = 1 + 4 * (int)blockIdx.y + (int)(threadIdx.x >> 6);
-  const int p0 = 1 + FUSE_SEG * (int)blockIdx.z;
+  // grid: segments fastest, so the segments of a column (which share halo
+  // cells) run together on one XCD
+  int bseg, bxc, btr;
+  xcd_tile(bseg, bxc, btr);
+  const int i = 1 + 64 * bxc + (int)(threadIdx.x & 63);
+  const int tr = 1 + 4 * btr + (int)(threadIdx.x >> 6);
+  const int p0 = 1 + FUSE_SEG * bseg;
   if (i > nx || tr > ntr || p0 > n) return;
   const int p1 = min(n, p0 + FUSE_SEG - 1);
   auto img = [&](int p) { return p < 1 ? p + n : (p > n ? p - n : p); };

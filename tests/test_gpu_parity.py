@@ -107,8 +107,8 @@ def test_multistep_vs_oracle_64():
 
 
 def test_multistep_vs_oracle_long_columns():
-    """y and z longer than one column segment of the fused RHS+MOC march
-    (32 cells), one of them not a multiple of it."""
+    """y longer than one column segment of the fused y/z sweep kernels (64
+    cells) and not a multiple of it; z shorter than one segment."""
     p = small(24, 72, 40)
     st = ics.random_state(p, seed=6)
     o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
