@@ -492,15 +492,11 @@ static int do_ct(mcf_ctx *x, double dt) {
   KLAUNCH(k_cc1, boxsize(b1), 6, A, x->g, b1);
   CHK(fillOT(x, 7, 2, OT_VFC));
   CHK(fillU(x, 7, 2, 1, l5, 1));
-  for (int o = 0; o < 3; ++o) {
-    if (o == 0) CHK(fillU(x, 7, 2, 1, l1, 1));
-    int d1 = o, d2 = (o + 1) % 3, d3 = (o + 2) % 3;
-    int lo[3], hi[3];
-    lo[d1] = 1; hi[d1] = n[d1] + 1; lo[d2] = 1; hi[d2] = n[d2] + 1; lo[d3] = 1; hi[d3] = n[d3];
-    Box be = mkbox(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-    if (x->mixed) KLAUNCH_R(k_cct, float, boxsize(be), 16, A, x->g, be, o, dt);
-    else KLAUNCH_R(k_cct, double, boxsize(be), 16, A, x->g, be, o, dt);
-  }
+  CHK(fillU(x, 7, 2, 1, l1, 1));
+  // the three edge orientations in one launch over (1..n+1)^3: reads rho, p,
+  // Vfc(3), face B(3), writes EMF(3)
+  if (x->mixed) KLAUNCH_N("k_cct3", k_cct3<float>, dim3(nblocks(boxsize(b1))), dim3(TPB), 8.0 * boxsize(b1) * 11, A, x->g, b1, dt);
+  else KLAUNCH_N("k_cct3", k_cct3<double>, dim3(nblocks(boxsize(b1))), dim3(TPB), 8.0 * boxsize(b1) * 11, A, x->g, b1, dt);
   KLAUNCH(k_cc3, boxsize(b1), 9, A, x->g, b1, dt);
   return do_cc4(x);
 }

@@ -139,7 +139,7 @@ struct DisArgs {
 __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
-template <class R> __global__ void k_cct(CtArgs A, Geo g, Box b, int orient, double dt);
+template <class R> __global__ void k_cct3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);

@@ -131,7 +131,7 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b) {
   }
 }
 
-// CCT (M3:2186-2445), one thread per edge of orientation (d1, d2).  The
+// CCT (M3:2186-2445) at the edge of orientation (d1, d2) of a cell.  The
 // edge's transverse slopes dQ (M3:2203-2268) are computed where they are
 // used: dQ1/dQ3 (along d2 of Vfc_d1 and B_d1) at the cells c0-s2 and c0,
 // dQ2/dQ4 (along d1 of Vfc_d2 and B_d2) at c0-s1 and c0, each with that
@@ -140,10 +140,9 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b) {
 // weighted averages (M3:2419-2433) and EMF = 1/4 (v1 B2 - v2 B1) sqrt(dnc).
 // Arithmetic in R (FP32 on the mixed path).
 template <class R>
-__global__ void k_cct(CtArgs A, Geo g, Box b, int orient, double dt_d) {
+__device__ __forceinline__ void cct_edge(const CtArgs &A, const Geo &g, int i, int j, int k, int orient,
+                                         double dt_d) {
   using Q = Ar<R>;
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
   const int d1 = orient, d2 = (orient + 1) % 3;
   const int c0 = ix3(g, i, j, k);
   const int s1 = g.st[d1], s2 = g.st[d2];
@@ -248,8 +247,20 @@ __global__ void k_cct(CtArgs A, Geo g, Box b, int orient, double dt_d) {
   A.emf[(orient + 2) % 3][c0] = (double)((R)0.25 * (v1a * B2a - v2a * B1a) * Q::sqrt_(dnc));
 }
 
-template __global__ void k_cct<double>(CtArgs, Geo, Box, int, double);
-template __global__ void k_cct<float>(CtArgs, Geo, Box, int, double);
+// The three edge orientations of a cell in one thread, so the face fields
+// each orientation shares with another are read once.  Box (1..n+1)^3; the
+// edges of orientation o exist where the position along its edge direction
+// d3 = (o+2)%3 is at most n[d3].
+template <class R>
+__global__ void k_cct3(CtArgs A, Geo g, Box b, double dt_d) {
+  int i, j, k;
+  if (!box_ijk(b, i, j, k)) return;
+  if (k <= g.n[2]) cct_edge<R>(A, g, i, j, k, 0, dt_d);
+  if (i <= g.n[0]) cct_edge<R>(A, g, i, j, k, 1, dt_d);
+  if (j <= g.n[1]) cct_edge<R>(A, g, i, j, k, 2, dt_d);
+}
+template __global__ void k_cct3<double>(CtArgs, Geo, Box, double);
+template __global__ void k_cct3<float>(CtArgs, Geo, Box, double);
 
 // CC(3): face-B induction (M3:2159-2169)
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt) {
