@@ -541,9 +541,9 @@ static int do_source(mcf_ctx *x, double dt) {
     const Box &b = stage == 0 ? bc : bg;
     KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
     {
-      dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + 7) / 8), (unsigned)((b.cnt[2] + SRC_ZSEG - 1) / SRC_ZSEG));
-      if (x->mixed) KLAUNCH_N("k_src_col", k_src_col<float>, gr, dim3(512), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
-      else KLAUNCH_N("k_src_col", k_src_col<double>, gr, dim3(512), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
+      dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + SRC_ROWS - 1) / SRC_ROWS), (unsigned)((b.cnt[2] + SRC_ZSEG - 1) / SRC_ZSEG));
+      if (x->mixed) KLAUNCH_N("k_src_col", k_src_col<float>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
+      else KLAUNCH_N("k_src_col", k_src_col<double>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
     }
     if (A.sur) {
       KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, stage);

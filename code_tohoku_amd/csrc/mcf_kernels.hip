@@ -313,8 +313,8 @@ __device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, doubl
 // queued with their conduction increment for k_src_stiff (the surrogate) or
 // k_src_imc, so the bisection runs on dense waves.
 //
-// The cells are visited as a march along z: a block is 8 x rows of 64 cells
-// (one wave per row) over SRC_ZSEG planes.  The temperature, gamma-1 and n
+// The cells are visited as a march along z: a block is SRC_ROWS x rows of 64
+// cells (one wave per row) over SRC_ZSEG planes.  The temperature, gamma-1 and n
 // of a thread's own column at k-1, k, k+1 stay in registers; the x
 // neighbours' temperatures come from the adjacent lanes (the row's first
 // and last lane compute their outer neighbour), the y neighbours' from an
@@ -360,13 +360,13 @@ __device__ __forceinline__ float qflux_f(double Tc, double Tcs, double dxi, doub
 }
 
 template <class R>
-__global__ __launch_bounds__(512) void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage) {
+__global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage) {
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   const int lane = threadIdx.x & 63;
   const int hi0 = b.lo[0] + b.cnt[0] - 1, hi1 = b.lo[1] + b.cnt[1] - 1, hi2 = b.lo[2] + b.cnt[2] - 1;
   const int i = b.lo[0] + 64 * bx + lane;
-  const int j = b.lo[1] + 8 * by + (int)(threadIdx.x >> 6);
+  const int j = b.lo[1] + SRC_ROWS * by + (int)(threadIdx.x >> 6);
   const int k0 = b.lo[2] + SRC_ZSEG * bz;
   const int k1 = min(hi2, k0 + SRC_ZSEG - 1);
   // rows past the box repeat the last row and lanes past the box load cell
@@ -374,13 +374,13 @@ __global__ __launch_bounds__(512) void k_src_col(SrcArgs A, Geo g, Box b, double
   const bool own = i <= hi0 && j <= hi1;
   const int ic = min(i, hi0 + 1), jc = min(j, hi1 + 1);
   const int w = (int)(threadIdx.x >> 6);
-  const int j0 = b.lo[1] + 8 * by;
+  const int j0 = b.lo[1] + SRC_ROWS * by;
   // temperature planes k, k+1, k+2 of rows j0-1 .. j0+8 (slot r+1 = row j0+r)
-  __shared__ double sT[3][10][64];
+  __shared__ double sT[3][SRC_ROWS + 2][64];
   // the first and last wave also form the rows outside the block
-  const int jh = w == 0 ? j0 - 1 : min(j0 + 8, hi1 + 1);
-  const int rh = w == 0 ? 0 : 9;
-  const bool halo = w == 0 || w == 7;
+  const int jh = w == 0 ? j0 - 1 : min(j0 + SRC_ROWS, hi1 + 1);
+  const int rh = w == 0 ? 0 : SRC_ROWS + 1;
+  const bool halo = w == 0 || w == SRC_ROWS - 1;
   const int sz = g.st[2];
   const double dxi = A.gdx[0][ic + 1], dxp = A.gdx[0][ic + 2], dxm = A.gdx[0][ic];
   const double dyj = A.gdx[1][jc + 1], dyp = A.gdx[1][jc + 2], dym = A.gdx[1][jc];
