@@ -925,7 +925,7 @@ __global__ __launch_bounds__(576, 7) void k_remap(SweepIO A, Geo g, double dt) {
 // REMAP + primitives along a periodic axis (y or z) as a column march: one
 // thread per x column, cells p0..p1 of a segment, faces p0-1..p1.  The 15
 // remapped quantities of positions J-1..J+2 sit in registers; each
-// iteration takes one new position (loaded one iteration ahead), forms the
+// iteration takes one new position (in FP64 loaded one iteration ahead), forms the
 // upwind flux through face J and updates cell J.  Values outside 1..n are
 // read at the periodic image (the ghost fills are copies), so no fills
 // precede this kernel.  The Lagrangian species are formed here from the
@@ -981,17 +981,29 @@ __global__ __launch_bounds__(256) void k_remap_col(SweepIO A, Geo g, double dt) 
     fetch(p0 - 1, r); enter(r, Q1, W1);
     fetch(p0, r); enter(r, Q2, W2);
   }
+  // FP64: the next position's loads are issued one iteration ahead; the
+  // FP32 instance keeps fewer registers live without it and runs faster
   double nr[16];
-  fetch(p0 + 1, nr);
-  double nva = A.va[at(p0 - 1)];
+  double nva = 0.0;
+  if constexpr (Ar<R>::strict) {
+    fetch(p0 + 1, nr);
+    nva = A.va[at(p0 - 1)];
+  }
   double Fp[15];
 #pragma unroll
   for (int v = 0; v < 15; ++v) Fp[v] = 0.0;
   for (int J = p0 - 1; J <= p1; ++J) {
-    enter(nr, Q3, W3);
-    const double vaJ = nva;
-    fetch(J + 3, nr);
-    nva = A.va[at(min(J + 1, n))];
+    double vaJ;
+    if constexpr (Ar<R>::strict) {
+      enter(nr, Q3, W3);
+      vaJ = nva;
+      fetch(J + 3, nr);
+      nva = A.va[at(min(J + 1, n))];
+    } else {
+      fetch(J + 2, nr);
+      enter(nr, Q3, W3);
+      vaJ = A.va[at(J)];
+    }
     // upwind linear flux through face J (M3:1600-1640)
     const double vadt = vaJ * dt;
     const bool up = !(vadt <= 0.0);
