@@ -140,7 +140,7 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b) {
 // weighted averages (M3:2419-2433) and EMF = 1/4 (v1 B2 - v2 B1) sqrt(dnc).
 // Arithmetic in R (FP32 on the mixed path).
 template <class R>
-__device__ __forceinline__ void cct_edge(const CtArgs &A, const Geo &g, int i, int j, int k, int orient,
+__device__ __forceinline__ double cct_edge(const CtArgs &A, const Geo &g, int i, int j, int k, int orient,
                                          double dt_d) {
   using Q = Ar<R>;
   const int d1 = orient, d2 = (orient + 1) % 3;
@@ -244,7 +244,7 @@ __device__ __forceinline__ void cct_edge(const CtArgs &A, const Geo &g, int i, i
     c2p = Q::div(c2p, W1[0] * fabs(C1m[0]) + W1[1] * fabs(C1m[1]) + W1[2] * fabs(C1m[2]) + W1[3] * fabs(C1m[3]));
   }
   R v1a = c1p + c1m, v2a = c2p + c2m, B1a = c1p - c1m, B2a = c2p - c2m;
-  A.emf[(orient + 2) % 3][c0] = (double)((R)0.25 * (v1a * B2a - v2a * B1a) * Q::sqrt_(dnc));
+  return (double)((R)0.25 * (v1a * B2a - v2a * B1a) * Q::sqrt_(dnc));
 }
 
 // The three edge orientations of a cell in one thread, so the face fields
@@ -255,9 +255,16 @@ template <class R>
 __global__ void k_cct3(CtArgs A, Geo g, Box b, double dt_d) {
   int i, j, k;
   if (!box_ijk(b, i, j, k)) return;
-  if (k <= g.n[2]) cct_edge<R>(A, g, i, j, k, 0, dt_d);
-  if (i <= g.n[0]) cct_edge<R>(A, g, i, j, k, 1, dt_d);
-  if (j <= g.n[1]) cct_edge<R>(A, g, i, j, k, 2, dt_d);
+  // the three EMFs are stored after all are formed, so the loads and limiter
+  // switches the orientations share are not repeated after a store
+  const int c = ix3(g, i, j, k);
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  if (k <= g.n[2]) e0 = cct_edge<R>(A, g, i, j, k, 0, dt_d);
+  if (i <= g.n[0]) e1 = cct_edge<R>(A, g, i, j, k, 1, dt_d);
+  if (j <= g.n[1]) e2 = cct_edge<R>(A, g, i, j, k, 2, dt_d);
+  if (k <= g.n[2]) A.emf[2][c] = e0;
+  if (i <= g.n[0]) A.emf[0][c] = e1;
+  if (j <= g.n[1]) A.emf[1][c] = e2;
 }
 template __global__ void k_cct3<double>(CtArgs, Geo, Box, double);
 template __global__ void k_cct3<float>(CtArgs, Geo, Box, double);
