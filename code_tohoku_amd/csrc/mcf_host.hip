@@ -385,8 +385,24 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
       S.blg[s][0] = x->bbrv[7][1][s + 1];
       S.blg[s][1] = x->bbrv[8][1][s + 1];
       S.dxl[s] = x->hdx[0][s == 0 ? 1 : x->g.n[0] + 2];
+      S.vin[s] = x->bbrv[2][1][s + 1];
+      S.mt[s][0] = x->bbrv[3][2][s + 1];
+      S.mt[s][1] = x->bbrv[4][2][s + 1];
+      S.ein[s] = x->bbrv[5][1][s + 1];
     }
+    for (int t = 0; t < 8; ++t) S.spin[t] = x->bbrv_cm[t];
     const int nx = x->g.n[0];
+    if (S.inflow[0] && S.inflow[1]) {
+      // inflow on both sides: the whole sweep in one kernel (the REMAP
+      // stage's ghosts are the fill rules, applied in the kernel)
+      dim3 gs((unsigned)((nx + XSCELLS - 1) / XSCELLS), (unsigned)((x->g.n[1] + 3) / 4), (unsigned)x->g.n[2]);
+      const double bys = 8.0 * (double)nx * x->g.n[1] * x->g.n[2] * 32;
+      if (x->mixed) KLAUNCH_N("k_sweep_x", k_sweep_x<float>, gs, dim3(256), bys, A, x->g, S, dt);
+      else KLAUNCH_N("k_sweep_x", k_sweep_x<double>, gs, dim3(256), bys, A, x->g, S, dt);
+      for (int f = 0; f < 5; ++f) swapU(x, f);
+      swapSp(x);
+      return MCF_OK;
+    }
     dim3 gr((unsigned)((nx + XCELLS - 1) / XCELLS), (unsigned)((x->g.n[1] + 3) / 4), (unsigned)x->g.n[2]);
     const double by = 8.0 * (double)nx * x->g.n[1] * x->g.n[2] * 33;
     if (x->mixed) KLAUNCH_N("k_rhsmoc_x", k_rhsmoc_x<float>, gr, dim3(256), by, A, x->g, S, dt);

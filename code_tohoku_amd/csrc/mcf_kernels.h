@@ -79,6 +79,10 @@ struct XSide {
   int inflow[2];
   double blg[2][2]; // Blg(t) ghost value per side
   double dxl[2];    // dx_lag ghost value per side
+  // inflow ghost rules of the REMAP stage (k_sweep_x): momentum_x = rho *
+  // vin, momenta y, z constants, E = ein*1.5 + (rho vin^2 + Bcc_x^2 +
+  // Blg^2)/2, species = spin * rho
+  double vin[2], mt[2][2], ein[2], spin[8];
 };
 
 // R = double: strict FP64; R = float: mixed precision
@@ -89,6 +93,8 @@ template <class R> __global__ void k_rhsmoc_x(SweepIO A, Geo g, XSide S, double 
 constexpr int XHALO = 4, XCELLS = 64 - 2 * XHALO; // k_rhsmoc_x: halo and cells per wave
 template <class R> __global__ void k_remap_x(SweepIO A, Geo g, double dt);
 constexpr int XRCELLS = 60; // k_remap_x: cells per wave (halo 2)
+template <class R> __global__ void k_sweep_x(SweepIO A, Geo g, XSide S, double dt);
+constexpr int XSCELLS = 52; // k_sweep_x: cells per wave (halo 6)
 constexpr int FUSE_SEG = 64; // cells per column segment of k_rhsmoc_col
 template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);

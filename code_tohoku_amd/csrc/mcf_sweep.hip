@@ -1244,6 +1244,230 @@ __global__ __launch_bounds__(256) void k_rhsmoc_x(SweepIO A, Geo g, XSide S, dou
   }
 }
 
+// The whole x sweep in one kernel when both x sides are inflow: the
+// k_rhsmoc_x chain, then REMAP from the MOC results of lanes p-2..p+2 held
+// in registers (the inflow ghosts take the REMAP stage's fill rules), so no
+// Lagrangian or MOC result goes to HBM and no fill runs inside the sweep.
+// Halo 6: a wave updates the 52 cells c0..c0+51.
+template <class R>
+__global__ __launch_bounds__(256) void k_sweep_x(SweepIO A, Geo g, XSide S, double dt) {
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int lane = threadIdx.x & 63;
+  const int n = g.n[0];
+  const int j = 1 + 4 * by + (int)(threadIdx.x >> 6), k = 1 + bz;
+  if (j > g.n[1]) return; // whole waves
+  const int c0 = 1 + XSCELLS * bx;
+  const int p = c0 - 6 + lane;
+  // position whose values the lane holds, and the one of its geometry
+  int m = p;
+  if (p < 1) m = S.inflow[0] ? p : p + n;
+  else if (p > n) m = S.inflow[1] ? p : p - n;
+  m = min(max(m, -1), n + 2);
+  const int mg = min(max(m, 0), n + 1);
+  const int c = ix3s(g, m, j, k);
+  const double *const *sp = A.sp;
+  const double rho = A.U[0][c], vn = A.U[1][c], pp = A.U[4][c];
+  const double b1 = A.bcc[1][c], b2 = A.bcc[2][c], bnc = A.bcc[0][c];
+  const double vt1 = A.U[2][c], vt2 = A.U[3][c];
+  double spc[8];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) spc[f] = sp[f][c];
+  const double *dxe = A.dxe, *xe = A.xe;
+  // RHS states of p (M3:1380-1406)
+  R q[8];
+  rhs_states<R>(rho, vn, pp, b1, b2, sh_up(rho), sh_dn(rho), sh_up(vn), sh_dn(vn), sh_up(pp), sh_dn(pp), sh_up(b1),
+                sh_dn(b1), sh_up(b2), sh_dn(b2), spc, dxe[mg], dxe[mg + 1], dxe[mg + 2], dt, A.facdep, q);
+  // Riemann interface p-1|p
+  R ql[8];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) ql[v] = sh_up(q[v]);
+#pragma unroll
+  for (int v = 4; v < 8; ++v) ql[v] = q[v];
+  double spl[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) spl[f] = sh_up(spc[f]);
+  R Pa, Va;
+  rhs_iface<R>(spl, spc, ql, q, Pa, Va);
+  const R Pr = sh_dn(Pa), Vr = sh_dn(Va);
+  // Lagrangian update of p (M3:1448-1466); interfaces p-1|p (Pa, Va) and
+  // p|p+1 (Pr, Vr)
+  const double xq = xe[mg + 1], xqm = xe[mg], dxq = dxe[mg + 1];
+  const double dm = dxq * rho;
+  const double pa = (double)Pr, pam = (double)Pa, va = (double)Vr, vam = (double)Va;
+  const double xl = xq + va * dt, xlm = xqm + vam * dt;
+  double dxl = xl - xlm;
+  double rhoL, vnL, eL, bl1, bl2, spL[8];
+  {
+    const double vx = vn, vy = vt1, vz = vt2, bx_ = bnc, by_ = b1, bz_ = b2;
+    if constexpr (Ar<R>::strict) {
+      double s4 = spc[0] + spc[1] + spc[3] + spc[4];
+      double gg = 3.0 * s4 + 5.0 * spc[2];
+      double gi = (2.0 * s4 + 2.0 * spc[2]) / gg;
+      double e = pp / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx_ * bx_ + by_ * by_ + bz_ * bz_) / rho;
+      rhoL = dm / dxl;
+      vnL = vn - dt / dm * (pa - pam);
+      eL = e - dt / dm * (pa * va - pam * vam);
+      bl1 = b1 * dxq / dxl;
+      bl2 = b2 * dxq / dxl;
+#pragma unroll
+      for (int f = 0; f < 8; ++f) spL[f] = dxq * spc[f] / dxl;
+    } else {
+      double s4 = spc[0] + spc[1] + spc[3] + spc[4];
+      double irho = 1.0 / rho;
+      double e = pp * (3.0 * s4 + 5.0 * spc[2]) / (2.0 * s4 + 2.0 * spc[2]) * irho +
+                 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx_ * bx_ + by_ * by_ + bz_ * bz_) * irho;
+      double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxq * idl;
+      rhoL = dm * idl;
+      vnL = vn - dtdm * (pa - pam);
+      eL = e - dtdm * (pa * va - pam * vam);
+      bl1 = b1 * cmp;
+      bl2 = b2 * cmp;
+#pragma unroll
+      for (int f = 0; f < 8; ++f) spL[f] = spc[f] * cmp;
+    }
+  }
+  // inflow ghosts as the MOC stage's fills leave them
+  const int side = p < 1 ? 0 : 1;
+  if ((p < 1 || p > n) && S.inflow[side]) {
+    rhoL = rho;
+    bl1 = S.blg[side][0];
+    bl2 = S.blg[side][1];
+    dxl = S.dxl[side];
+  }
+  // MOC slope of p (M3:2011-2020)
+  const bool mme = use_minmod_r<R>(eL, rhoL), mmp = use_minmod_r<R>(pp, rhoL);
+  const bool inner = p >= 1 && p <= n;
+  const bool mm = inner ? mme : mmp;
+  double v0[6] = {rhoL, vt1, vt2, bl1, bl2, dxl};
+  double vm[6], vp[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) { vm[v] = sh_up(v0[v]); vp[v] = sh_dn(v0[v]); }
+  const R dxm_r = (R)vm[5], dx0_r = (R)dxl, dxp_r = (R)vp[5];
+  R G[5], H[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) { G[v] = vslope<R>(vm[v], v0[v], vp[v], dxm_r, dx0_r, dxp_r, mm); H[v] = G[v]; }
+  const int nblk = A.nblk;
+  if ((p % nblk == 0 || (p - 1) % nblk == 0) && mmp != mm) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) H[v] = vslope<R>(vm[v], v0[v], vp[v], dxm_r, dx0_r, dxp_r, mmp);
+  }
+  // characteristic interface J = p between p and p+1 (local positions 0..n)
+  R G1[5], H1[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) { G1[v] = sh_dn(G[v]); H1[v] = sh_dn(H[v]); }
+  const int fl = (mme ? 1 : 0) | (mmp ? 2 : 0);
+  const int fl1 = __shfl_down(fl, 1u, 64);
+  const int J = min(max(p, 0), n + 1);
+  const double fown = A.U[5][ix3s(g, min(max(p, -1), n + 2), j, k)];
+  const double f1 = sh_dn(fown), f2 = sh_dn(f1);
+  const double bnJ = moc_bn(fown, f1, f2, xe[J + 1] + va * dt, xe, J);
+  MocV<R> L, Rv;
+  if (J % nblk != 0) {
+    L = moc_iface_r<R>(v0, vp, G, G1, bnJ, A.facdep, dt);
+    Rv = L;
+  } else {
+    const bool mm0p = mmp, mm0e = J >= 1 ? mme : mm0p;
+    const bool mm1p = (fl1 & 2) != 0, mm1e = J + 1 <= n ? (fl1 & 1) != 0 : mm1p;
+    L = moc_iface_r<R>(v0, vp, G, mm1p != mm1e ? H1 : G1, bnJ, A.facdep, dt);
+    if (mm0p != mm0e) Rv = moc_iface_r<R>(v0, vp, H, G1, bnJ, A.facdep, dt);
+    else if (mm1p != mm1e) Rv = moc_iface_r<R>(v0, vp, G, G1, bnJ, A.facdep, dt);
+    else Rv = L;
+  }
+  // MOC update of p (M3:2053-2095): left view of interface p, right view
+  // of interface p-1
+  MocV<R> rvp;
+  rvp.Bay = sh_up(Rv.Bay); rvp.Vay = sh_up(Rv.Vay); rvp.Baz = sh_up(Rv.Baz); rvp.Vaz = sh_up(Rv.Vaz);
+  const double bnm = sh_up(bnJ);
+  const double Bnap = bnJ, Bnam = bnm, Vnap = va, Vnam = vam;
+  const double BayL = (double)L.Bay, VayL = (double)L.Vay, BazL = (double)L.Baz, VazL = (double)L.Vaz;
+  const double BayR = (double)rvp.Bay, VayR = (double)rvp.Vay, BazR = (double)rvp.Baz, VazR = (double)rvp.Vaz;
+  double Q[15];
+  if constexpr (Ar<R>::strict) {
+    Q[1] = vnL * rhoL + dt / dxl * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+    Q[2] = vt1 * rhoL + dt / dxl * (Bnap * BayL - Bnam * BayR);
+    Q[3] = vt2 * rhoL + dt / dxl * (Bnap * BazL - Bnam * BazR);
+    Q[4] = eL * rhoL + dt / dxl *
+                          ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                           (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+    Q[5] = bl1 + dt / dxl * (Bnap * VayL - Bnam * VayR);
+    Q[6] = bl2 + dt / dxl * (Bnap * VazL - Bnam * VazR);
+  } else {
+    const double fq = dt / dxl;
+    Q[1] = vnL * rhoL + fq * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+    Q[2] = vt1 * rhoL + fq * (Bnap * BayL - Bnam * BayR);
+    Q[3] = vt2 * rhoL + fq * (Bnap * BazL - Bnam * BazR);
+    Q[4] = eL * rhoL + fq * ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                             (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+    Q[5] = bl1 + fq * (Bnap * VayL - Bnam * VayR);
+    Q[6] = bl2 + fq * (Bnap * VazL - Bnam * VazR);
+  }
+  Q[0] = rhoL;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) Q[7 + f] = spL[f];
+  // inflow ghosts as the REMAP stage's fills leave them (M3:818-884)
+  if (p < 1 || p > n) {
+    const double v = S.vin[side];
+    Q[0] = rho;
+    Q[1] = rho * v;
+    Q[2] = S.mt[side][0];
+    Q[3] = S.mt[side][1];
+    Q[4] = S.ein[side] * 1.5 + 0.5 * (rho * (v * v) + bnc * bnc + bl1 * bl1 + bl2 * bl2);
+    Q[5] = bl1;
+    Q[6] = bl2;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) Q[7 + f] = S.spin[f] * rho;
+  }
+  // REMAP of cell p (M3:1600-1686): upwind flux through face p|p+1 from
+  // cells p-1..p+2, the flux of face p-1|p from the lane below
+  const double W1 = dxl;
+  const double W0 = sh_up(W1), W2 = sh_dn(W1), W3 = sh_dn(W2);
+  const double vadt = va * dt;
+  const bool up = !(vadt <= 0.0);
+  const double u4n = sh_dn(Q[4]), u0n = sh_dn(Q[0]);
+  const double u4 = up ? Q[4] : u4n, u0 = up ? Q[0] : u0n;
+  const double depend = up ? 0.5 * (W1 - vadt) * A.facdep : 0.5 * (W2 + vadt) * A.facdep;
+  const R rdxm = (R)(up ? W0 : W1), rdx0 = (R)(up ? W1 : W2), rdxp = (R)(up ? W2 : W3);
+  const bool rmm = use_minmod_r<R>(u4, u0);
+  const R rdep = (R)depend;
+  const bool own = lane >= 6 && lane < 6 + XSCELLS && p <= n;
+  const double dxE = own ? xe[p + 1] - xe[p] : 1.0;
+  double Qn[15];
+#pragma unroll
+  for (int v = 0; v < 15; ++v) {
+    const double q0 = sh_up(Q[v]), q1 = Q[v], q2 = sh_dn(Q[v]), q3 = sh_dn(q2);
+    const R gr = vslope<R>(up ? q0 : q1, up ? q1 : q2, up ? q2 : q3, rdxm, rdx0, rdxp, rmm);
+    const double qu = up ? q1 : q2;
+    double face;
+    if constexpr (Ar<R>::strict) face = up ? qu + depend * gr : qu - depend * gr;
+    else face = up ? qu + (double)(rdep * gr) : qu - (double)(rdep * gr);
+    const double F = -(vadt * face);
+    const double Fl = sh_up(F);
+    if constexpr (Ar<R>::strict) Qn[v] = (q1 * W1 + F - Fl) / dxE;
+    else Qn[v] = (q1 * W1 + F - Fl) * (1.0 / dxE);
+  }
+  if (!own) return;
+  // primitives (M3:1291-1321)
+  double rr = dmax(1.27 * A.rmin, Qn[0]);
+  double invd = 1.0 / rr;
+  double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
+  double s4 = Qn[7] + Qn[8] + Qn[10] + Qn[11];
+  double gg = 3.0 * s4 + 5.0 * Qn[9];
+  double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
+  double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rr;
+  double emag = 0.5 * (bnc * bnc + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
+  double pr = (Qn[4] - ekin - emag) * gi;
+  pr = dmax(A.pmin, pr);
+  pr = dmin(A.pmax, pr);
+  A.Uo[0][c] = rr;
+  A.Uo[1][c] = vx;
+  A.Uo[2][c] = vy;
+  A.Uo[3][c] = vz;
+  A.Uo[4][c] = pr;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
+}
+
 // REMAP + primitives along x with the layout of k_rhsmoc_x: a wave holds
 // positions c0-2 .. c0+61 of one row and updates cells c0 .. c0+59.  Lane p
 // forms the upwind flux through face p|p+1 from cells p-1..p+2 (neighbours
@@ -1326,6 +1550,8 @@ template __global__ void k_rhsmoc_x<double>(SweepIO, Geo, XSide, double);
 template __global__ void k_rhsmoc_x<float>(SweepIO, Geo, XSide, double);
 template __global__ void k_remap_x<double>(SweepIO, Geo, double);
 template __global__ void k_remap_x<float>(SweepIO, Geo, double);
+template __global__ void k_sweep_x<double>(SweepIO, Geo, XSide, double);
+template __global__ void k_sweep_x<float>(SweepIO, Geo, XSide, double);
 
 #define MCF_INST(D, R)                                                                                  \
   template __global__ void k_rhs<D, R>(SweepIO, Geo, double);                                          \
