@@ -1250,7 +1250,7 @@ __global__ __launch_bounds__(256) void k_rhsmoc_x(SweepIO A, Geo g, XSide S, dou
 // Lagrangian or MOC result goes to HBM and no fill runs inside the sweep.
 // Halo 6: a wave updates the 52 cells c0..c0+51.
 template <class R>
-__global__ __launch_bounds__(256) void k_sweep_x(SweepIO A, Geo g, XSide S, double dt) {
+__global__ __launch_bounds__(256, Ar<R>::strict ? 3 : 4) void k_sweep_x(SweepIO A, Geo g, XSide S, double dt) {
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   const int lane = threadIdx.x & 63;
