@@ -95,7 +95,7 @@ template <class R> __global__ void k_remap_x(SweepIO A, Geo g, double dt);
 constexpr int XRCELLS = 60; // k_remap_x: cells per wave (halo 2)
 template <class R> __global__ void k_sweep_x(SweepIO A, Geo g, XSide S, double dt);
 constexpr int XSCELLS = 52; // k_sweep_x: cells per wave (halo 6)
-constexpr int FUSE_SEG = 64; // cells per column segment of k_rhsmoc_col
+constexpr int FUSE_SEG = 128; // cells per column segment of k_rhsmoc_col
 template <int D, class R> __global__ void k_moc(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_remap(SweepIO A, Geo g, double dt);
 

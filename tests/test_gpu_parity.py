@@ -107,9 +107,9 @@ def test_multistep_vs_oracle_64():
 
 
 def test_multistep_vs_oracle_long_columns():
-    """y longer than one column segment of the fused y/z sweep kernels (64
+    """y longer than one column segment of the fused y/z sweep kernels (128
     cells) and not a multiple of it; z shorter than one segment."""
-    p = small(24, 72, 40)
+    p = small(24, 136, 40)
     st = ics.random_state(p, seed=6)
     o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
     g = mcf.Domain(p, nsplit=(2, 2, 2))
