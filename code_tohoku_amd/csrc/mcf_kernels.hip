@@ -321,13 +321,11 @@ __device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, doubl
 // k_src_imc, so the bisection runs on dense waves.
 //
 // The cells are visited as a march along z: a block is SRC_ROWS x rows of 64
-// cells (one wave per row) over SRC_ZSEG planes.  The temperature, gamma-1 and n
-// of a thread's own column at k-1, k, k+1 stay in registers; the x
-// neighbours' temperatures come from the adjacent lanes (the row's first
-// and last lane compute their outer neighbour), the y neighbours' from an
-// LDS ring of the block's temperature planes (its 8 rows and the row on
-// either side, computed by the first and last wave), so every temperature
-// is formed once per block.
+// cells (one wave per row) over SRC_ZSEG planes.  Per plane the whole block
+// forms the temperatures of its rows plus a one-cell frame in x and y
+// (coalesced rows) into a 4-plane LDS ring, so each cell reads its six
+// neighbours from LDS and every temperature is formed once per block (one
+// barrier per plane).
 // R = double: the reference's arithmetic, operation by operation.
 // R = float (mixed precision): temperatures and their differences in FP64,
 // the rates (cooling function, conduction fluxes, time-step tests) in FP32,
@@ -376,49 +374,55 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
   const int j = b.lo[1] + SRC_ROWS * by + (int)(threadIdx.x >> 6);
   const int k0 = b.lo[2] + SRC_ZSEG * bz;
   const int k1 = min(hi2, k0 + SRC_ZSEG - 1);
-  // rows past the box repeat the last row and lanes past the box load cell
-  // hi0+1 (the last cell's right neighbour); neither stores anything
   const bool own = i <= hi0 && j <= hi1;
   const int ic = min(i, hi0 + 1), jc = min(j, hi1 + 1);
   const int w = (int)(threadIdx.x >> 6);
-  const int j0 = b.lo[1] + SRC_ROWS * by;
-  // temperature planes k, k+1, k+2 of rows j0-1 .. j0+8 (slot r+1 = row j0+r)
-  __shared__ double sT[3][SRC_ROWS + 2][64];
-  // the first and last wave also form the rows outside the block
-  const int jh = w == 0 ? j0 - 1 : min(j0 + SRC_ROWS, hi1 + 1);
-  const int rh = w == 0 ? 0 : SRC_ROWS + 1;
-  const bool halo = w == 0 || w == SRC_ROWS - 1;
-  const int sz = g.st[2];
+  const int i0 = b.lo[0] + 64 * bx, j0 = b.lo[1] + SRC_ROWS * by;
+  // temperature tiles of planes k-1 .. k+2 (ring of 4): rows j0-1 ..
+  // j0+SRC_ROWS by columns i0-1 .. i0+64, formed by all threads together
+  // (coalesced rows, the frame included); gamma-1 and n of the block's own
+  // cells beside them
+  constexpr int TW = 66, TR = SRC_ROWS + 2, TN = TW * TR, NT = 64 * SRC_ROWS;
+  __shared__ double sT[4][TR][TW];
+  __shared__ R sG[4][SRC_ROWS][64], sN[4][SRC_ROWS][64];
+  auto tile = [&](int q) {
+    const int sl = q & 3;
+#pragma unroll
+    for (int t0 = 0; t0 < TN; t0 += NT) {
+      const int t = t0 + (int)threadIdx.x;
+      if (t < TN) {
+        const int r = t / TW, cl = t - r * TW;
+        const int x = min(i0 - 1 + cl, hi0 + 1), y = min(j0 - 1 + r, hi1 + 1);
+        const SrcW<R> sw = src_w<R>(A, ix3(g, x, y, q));
+        sT[sl][r][cl] = sw.T;
+        if (r >= 1 && r <= SRC_ROWS && cl >= 1 && cl <= 64) {
+          sG[sl][r - 1][cl - 1] = sw.gi;
+          sN[sl][r - 1][cl - 1] = sw.ns;
+        }
+      }
+    }
+  };
   const double dxi = A.gdx[0][ic + 1], dxp = A.gdx[0][ic + 2], dxm = A.gdx[0][ic];
   const double dyj = A.gdx[1][jc + 1], dyp = A.gdx[1][jc + 2], dym = A.gdx[1][jc];
   const bool ghost_ij = i == 0 || j == 0 || i > g.n[0] || j > g.n[1];
-  SrcW<R> wm = src_w<R>(A, ix3(g, ic, jc, k0 - 1));
-  SrcW<R> wc = src_w<R>(A, ix3(g, ic, jc, k0));
-  SrcW<R> wq = src_w<R>(A, ix3(g, ic, jc, k0 + 1));
-  sT[k0 % 3][w + 1][lane] = wc.T;
-  sT[(k0 + 1) % 3][w + 1][lane] = wq.T;
-  if (halo) {
-    sT[k0 % 3][rh][lane] = src_T(A, ix3(g, ic, jh, k0));
-    sT[(k0 + 1) % 3][rh][lane] = src_T(A, ix3(g, ic, jh, k0 + 1));
-  }
+  tile(k0 - 1);
+  tile(k0);
+  tile(k0 + 1);
   for (int k = k0; k <= k1; ++k) {
     const int c = ix3(g, ic, jc, k);
-    // slot of plane k+2 was last read for plane k-1, before this barrier
+    // the slot of plane k+2 was last read for plane k-2, before this barrier
     __syncthreads();
-    SrcW<R> wn{};
-    if (k < k1) {
-      wn = src_w<R>(A, c + 2 * sz);
-      sT[(k + 2) % 3][w + 1][lane] = wn.T;
-      if (halo) sT[(k + 2) % 3][rh][lane] = src_T(A, ix3(g, ic, jh, k + 2));
-    }
-    const double T = wc.T;
-    const double Txu = __shfl_up(T, 1, 64), Txd = __shfl_down(T, 1, 64);
-    double Te = 0.0;
-    if (lane == 0 || lane == 63) Te = src_T(A, lane == 0 ? c - 1 : c + 1);
-    const double Txm = lane == 0 ? Te : Txu, Txp = lane == 63 ? Te : Txd;
-    // plane k was written two iterations (two barriers) ago
-    const double Typ = sT[k % 3][w + 2][lane], Tym = sT[k % 3][w][lane];
-    const double Tzp = wq.T, Tzm = wm.T;
+    if (k < k1) tile(k + 2);
+    // planes k-1, k, k+1 were written at least one barrier ago
+    const int s0 = k & 3, sm = (k - 1) & 3, sp = (k + 1) & 3;
+    const double T = sT[s0][w + 1][lane + 1];
+    const double Txm = sT[s0][w + 1][lane], Txp = sT[s0][w + 1][lane + 2];
+    const double Tym = sT[s0][w][lane + 1], Typ = sT[s0][w + 2][lane + 1];
+    const double Tzm = sT[sm][w + 1][lane + 1], Tzp = sT[sp][w + 1][lane + 1];
+    SrcW<R> wc;
+    wc.T = T;
+    wc.gi = sG[s0][w][lane];
+    wc.ns = sN[s0][w][lane];
     const bool ghost = stage == 1 && (ghost_ij || k == 0 || k > g.n[2]);
     if (own && ghost) A.Pout[c] = A.Tp[c];
     const bool valid = own && !ghost;
@@ -482,7 +486,6 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
       u = dmin(u, A.pmax);
       A.Pout[c] = u;
     }
-    wm = wc; wc = wq; wq = wn;
   }
 }
 template __global__ void k_src_col<double>(SrcArgs, Geo, Box, double, int);
