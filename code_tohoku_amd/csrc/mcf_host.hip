@@ -541,7 +541,7 @@ static int do_source(mcf_ctx *x, double dt) {
   int l15[2] = {1, 5}, l5[1] = {5};
   CHK(fillU(x, 7, 1, 1, l15, 2));
   CHK(fillOT(x, 7, 1, OT_SPECIES));
-  Box bc = mkbox(1, nx, 1, ny, 1, nz), bg = mkbox(0, nx + 1, 0, ny + 1, 0, nz + 1);
+  Box bc = mkbox(1, nx, 1, ny, 1, nz);
   double *P = x->U[F_P], *Q = x->Ub[F_P];
   for (int stage = 0; stage < 2; ++stage) {
     if (stage == 0) {
@@ -552,10 +552,13 @@ static int do_source(mcf_ctx *x, double dt) {
       x->U[F_P] = Q; x->Ub[F_P] = P;
       CHK(fillU(x, 7, 1, 1, l5, 1));
       x->U[F_P] = P; x->Ub[F_P] = Q;
-      // corrector: T from the predictor, result written over P
+      // corrector: T from the predictor, result written over P; P's first
+      // ghost layer takes the predictor's
       A.Tp = Q; A.Pn = P; A.Pout = P;
+      const long a0 = nx + 2, a1 = ny + 2;
+      KLAUNCH(k_shell_copy, 2 * (a0 * a1 + a0 * nz + (long)ny * nz), 2, Q, P, x->g);
     }
-    const Box &b = stage == 0 ? bc : bg;
+    const Box &b = bc;
     KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
     {
       dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + SRC_ROWS - 1) / SRC_ROWS), (unsigned)((b.cnt[2] + SRC_ZSEG - 1) / SRC_ZSEG));

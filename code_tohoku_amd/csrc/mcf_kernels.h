@@ -23,12 +23,13 @@ struct Box {
 // ordered with the first two grid axes fastest: tiles that share halo rows
 // (neighbours along x for the x sweep, along the sweep axis for y/z) then
 // run on the same XCD at about the same time.  The renumbering is a
-// bijection; grids whose tile count is not a multiple of 8 keep the
-// dispatch order.
+// bijection: with nt = 8q + r tiles, XCD x < r takes q+1 of them, the
+// others q.
 __device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
   const unsigned gx = gridDim.x, gy = gridDim.y, nt = gx * gy * gridDim.z;
   unsigned t = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  if ((nt & 7u) == 0u) t = (t & 7u) * (nt >> 3) + (t >> 3);
+  const unsigned q = nt >> 3, r = nt & 7u, x = t & 7u, s = t >> 3;
+  t = x < r ? x * (q + 1u) + s : r * (q + 1u) + (x - r) * q + s;
   bx = (int)(t % gx);
   by = (int)((t / gx) % gy);
   bz = (int)(t / (gx * gy));
@@ -151,6 +152,7 @@ __global__ void k_cc4(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);
 constexpr int SRC_ZSEG = 16; // planes per block of k_src_col
 constexpr int SRC_ROWS = 4;  // x rows (waves) per block of k_src_col
+__global__ void k_shell_copy(const double *__restrict__ src, double *__restrict__ dst, Geo g);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
 __global__ void k_src_imc(SrcArgs A, double dt, int stage, int which);
 __global__ void k_zero_u32x2(unsigned *p);

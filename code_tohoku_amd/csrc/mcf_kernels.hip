@@ -314,8 +314,9 @@ __device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, doubl
 // Stage 0 (predictor, dt/2): T from the pressure P (= A.Pn), result to
 // A.Pout.  Stage 1 (corrector, dt): T from the predictor (A.Tp), P_old =
 // A.Pn, result written over A.Pn (= A.Pout; each cell reads only its own
-// P_old there); its threads on the first ghost layer copy the predictor's
-// ghost values across, so the ghosts end as the reference leaves them.
+// P_old there).  Both stages cover the interior; the corrector's first
+// ghost layer takes the predictor's ghost values (k_shell_copy), as the
+// reference leaves them.
 // The backward-Euler cells (dt > 0.2*t_cool) are not solved here: they are
 // queued with their conduction increment for k_src_stiff (the surrogate) or
 // k_src_imc, so the bisection runs on dense waves.
@@ -404,7 +405,6 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
   };
   const double dxi = A.gdx[0][ic + 1], dxp = A.gdx[0][ic + 2], dxm = A.gdx[0][ic];
   const double dyj = A.gdx[1][jc + 1], dyp = A.gdx[1][jc + 2], dym = A.gdx[1][jc];
-  const bool ghost_ij = i == 0 || j == 0 || i > g.n[0] || j > g.n[1];
   tile(k0 - 1);
   tile(k0);
   tile(k0 + 1);
@@ -423,9 +423,7 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
     wc.T = T;
     wc.gi = sG[s0][w][lane];
     wc.ns = sN[s0][w][lane];
-    const bool ghost = stage == 1 && (ghost_ij || k == 0 || k > g.n[2]);
-    if (own && ghost) A.Pout[c] = A.Tp[c];
-    const bool valid = own && !ghost;
+    const bool valid = own;
     const double rho = A.rho[c];
     const double Pn = A.Pn[c];
     const double dzk = A.gdx[2][k + 1], dzp = A.gdx[2][k + 2], dzm = A.gdx[2][k];
@@ -490,6 +488,27 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
 }
 template __global__ void k_src_col<double>(SrcArgs, Geo, Box, double, int);
 template __global__ void k_src_col<float>(SrcArgs, Geo, Box, double, int);
+
+// Copy of the first ghost layer around the interior, src -> dst: the faces
+// k = 0 and n2+1 with their edges and corners, then j = 0 and n1+1 over
+// k = 1..n2, then i = 0 and n0+1 over the interior (j, k).
+__global__ void k_shell_copy(const double *__restrict__ src, double *__restrict__ dst, Geo g) {
+  const long a0 = g.n[0] + 2, a1 = g.n[1] + 2, n1 = g.n[1], n2 = g.n[2];
+  const long fz = a0 * a1, fy = a0 * n2, fx = n1 * n2;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t < 2 * fz) {
+    i = (int)(t % a0); j = (int)((t / a0) % a1); k = t < fz ? 0 : g.n[2] + 1;
+  } else if ((t -= 2 * fz) < 2 * fy) {
+    i = (int)(t % a0); k = 1 + (int)((t / a0) % n2); j = t < fy ? 0 : g.n[1] + 1;
+  } else if ((t -= 2 * fy) < 2 * fx) {
+    j = 1 + (int)(t % n1); k = 1 + (int)((t / n1) % n2); i = t < fx ? 0 : g.n[0] + 1;
+  } else {
+    return;
+  }
+  const int c = ix3(g, i, j, k);
+  dst[c] = src[c];
+}
 
 // The queued backward-Euler cells of one SOURCE stage, 64 per wave: the
 // surrogate predicts, Newton steps refine, the residual test accepts; the
