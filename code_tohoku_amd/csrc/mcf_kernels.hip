@@ -221,8 +221,9 @@ __device__ __forceinline__ double cct_edge(const CtArgs &A, const Geo &g, int i,
   R W0[4], W1[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    // the weight of a candidate pair whose two flags are both set is 1e13
-    int both = (int)(0.51 * (double)f0[m] + 0.51 * (double)f1[m]);
+    // the weight of a candidate pair whose two flags are both set is 1e13:
+    // int(0.51 f0 + 0.51 f1) of flags in {0, 1} is 1 exactly when both are
+    const int both = f0[m] & f1[m];
     W0[m] = (R)f0[m] + (R)(1.0e13 - 1.0) * (R)both;
     W1[m] = (R)f1[m] + (R)(1.0e13 - 1.0) * (R)both;
   }
