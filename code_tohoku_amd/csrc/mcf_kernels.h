@@ -151,7 +151,7 @@ __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);
 constexpr int SRC_ZSEG = 16; // planes per block of k_src_col
-constexpr int SRC_ROWS = 4;  // x rows (waves) per block of k_src_col
+constexpr int SRC_ROWS = 8;  // x rows (waves) per block of k_src_col
 __global__ void k_shell_copy(const double *__restrict__ src, double *__restrict__ dst, Geo g);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
 __global__ void k_src_imc(SrcArgs A, double dt, int stage, int which);

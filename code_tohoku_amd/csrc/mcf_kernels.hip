@@ -253,7 +253,7 @@ __device__ __forceinline__ double cct_edge(const CtArgs &A, const Geo &g, int i,
 // edges of orientation o exist where the position along its edge direction
 // d3 = (o+2)%3 is at most n[d3].
 template <class R>
-__global__ void k_cct3(CtArgs A, Geo g, Box b, double dt_d) {
+__global__ __launch_bounds__(256) void k_cct3(CtArgs A, Geo g, Box b, double dt_d) {
   int i, j, k;
   if (!box_ijk(b, i, j, k)) return;
   // the three EMFs are stored after all are formed, so the loads and limiter
