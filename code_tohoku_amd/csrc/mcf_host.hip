@@ -71,6 +71,7 @@ struct mcf_ctx {
   __bf16 *mlp_pk = nullptr;
   unsigned long long *cool_cnt = nullptr;
   bool mlp_ready = false;
+  int src_pair = 0; // which of the two SOURCE queue-counter pairs the next stage counts in
   long adam_t = 0;
   struct Rec { const char *name; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
@@ -536,7 +537,6 @@ static int do_source(mcf_ctx *x, double dt) {
   for (int d = 0; d < 3; ++d) A.gdx[d] = x->gdx[d];
   A.pmin = x->pmin; A.pmax = x->pmax;
   A.cnt = x->cool_cnt;
-  A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2);
   A.queue = reinterpret_cast<int *>(x->srcq[1]);
   A.fqueue = reinterpret_cast<int *>(x->srcq[2]);
   A.qdc = x->srcq[0];
@@ -568,12 +568,16 @@ static int do_source(mcf_ctx *x, double dt) {
       KLAUNCH(k_shell_copy, 2 * (a0 * a1 + a0 * nz + (long)ny * nz), 2, Q, P, x->g);
     }
     const Box &b = bc;
-    KLAUNCH_G(k_zero_u32x2, dim3(1), dim3(64), 0, A.qn);
+    // queue counters: cool_cnt[2] and cool_cnt[3] (two u32 each), zero at
+    // init; the stage counts in one pair and its k_src_col clears the other
+    A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2 + x->src_pair);
+    A.qz = reinterpret_cast<unsigned *>(x->cool_cnt + 3 - x->src_pair);
     {
       dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + SRC_ROWS - 1) / SRC_ROWS), (unsigned)((b.cnt[2] + SRC_ZSEG - 1) / SRC_ZSEG));
       if (x->mixed) KLAUNCH_N("k_src_col", k_src_col<float>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
       else KLAUNCH_N("k_src_col", k_src_col<double>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
     }
+    x->src_pair ^= 1;
     if (A.sur) {
       KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, stage);
       KLAUNCH_G(k_src_imc, dim3(256), dim3(256), 0, A, dt, stage, 1);

@@ -125,6 +125,7 @@ struct SrcArgs {
   const float *mw;
   unsigned long long *cnt;
   unsigned *qn;   // {queued cells, rejected cells}
+  unsigned *qz;   // the other stage's pair of counters, cleared by k_src_col
   int *queue, *fqueue;
   double *qdc;
 };

@@ -56,10 +56,20 @@ __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long tot = (long)na * nb * (2 * nol);
   if (t >= tot) return;
-  int a = (int)(t % na);
-  long r = t / na;
-  int b = (int)(r % nb);
-  int layer = (int)(r / nb); // 0..2*nol-1
+  // along x the ghost layers of a row are adjacent in memory: layers fastest
+  // there; along y and z the rows (index a) are
+  int a, b, layer; // layer 0..2*nol-1
+  if (d == 0) {
+    layer = (int)(t % (2 * nol));
+    long r = t / (2 * nol);
+    a = (int)(r % na);
+    b = (int)(r / na);
+  } else {
+    a = (int)(t % na);
+    long r = t / na;
+    b = (int)(r % nb);
+    layer = (int)(r / nb);
+  }
   int sd = g.st[d];
   int base = a * g.st[da] + b * g.st[db] + sd; // position 0 along d
   int gpos, spos;
@@ -69,17 +79,20 @@ __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol) {
 }
 
 // x-ghost rules of BC_MPI / BC_MPI_OT at an inflow (BC=4) end
-// (M3:818-884, 948-1092).  One thread per ghost point and side; the fields
-// are processed in list order, as the reference's K loop does.
+// (M3:818-884, 948-1092).  One thread per ghost point and side, the ghost
+// layers of a row on adjacent threads (adjacent in memory); the fields are
+// processed in list order, as the reference's K loop does.  A thread reads
+// only its own ghost point and interior points, so the order of the points
+// does not matter.
 __global__ void k_fill_xrule(XFill X, Geo g, int nol) {
   int ny4 = g.n[1] + 4, nz4 = g.n[2] + 4, nx = g.n[0];
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long tot = (long)ny4 * nz4 * (2 * nol);
   if (t >= tot) return;
-  int j = (int)(t % ny4) - 1;
-  long r = t / ny4;
-  int k = (int)(r % nz4) - 1;
-  int layer = (int)(r / nz4);
+  int layer = (int)(t % (2 * nol));
+  long r = t / (2 * nol);
+  int j = (int)(r % ny4) - 1;
+  int k = (int)(r / ny4) - 1;
   int side = layer < nol ? 0 : 1;
   int i = side == 0 ? 1 - nol + layer : nx + 1 + (layer - nol);
   int c = ix3(g, i, j, k);
@@ -370,6 +383,11 @@ template <class R>
 __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage) {
   int bx, by, bz;
   xcd_tile(bx, by, bz);
+  // the queue counters alternate between two pairs: this stage counts in
+  // A.qn (cleared by the previous stage) and clears A.qz for the next one,
+  // whose consumers of it (k_src_stiff, k_src_imc) finished before this
+  // launch
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 2) A.qz[threadIdx.x] = 0u;
   const int lane = threadIdx.x & 63;
   const int hi0 = b.lo[0] + b.cnt[0] - 1, hi1 = b.lo[1] + b.cnt[1] - 1, hi2 = b.lo[2] + b.cnt[2] - 1;
   const int i = b.lo[0] + 64 * bx + lane;
