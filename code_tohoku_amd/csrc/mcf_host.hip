@@ -237,6 +237,18 @@ static int fill_periodic(mcf_ctx *x, double *const *a, int nf, int d, int nol) {
   return MCF_OK;
 }
 
+// The y and z periodic passes of one fill in one launch (k_fill_yz); dm[f]
+// says which of the two passes field f takes (bit 0: y, bit 1: z).
+static int fill_yz(mcf_ctx *x, double *const *a, const int *dm, int nf, int nol) {
+  FillList L{};
+  L.nf = nf;
+  for (int f = 0; f < nf; ++f) { L.a[f] = a[f]; L.dm[f] = dm[f]; }
+  const long m0 = x->g.n[0] + 4, m1 = x->g.n[1] + 4, m2 = x->g.n[2] + 4;
+  const long nt = 2L * nol * m0 * m2 + 2L * nol * m0 * (m1 - 2 * nol);
+  KLAUNCH(k_fill_yz, nt, 2 * nf, L, x->g, nol);
+  return MCF_OK;
+}
+
 static bool x_inflow(const mcf_ctx *x) { return x->cfg.bcx1 == 4 || x->cfg.bcx2 == 4; }
 
 static int launch_xfill(mcf_ctx *x, XFill &X, int nol) {
@@ -277,12 +289,31 @@ static int fillU(mcf_ctx *x, int dmask, int nol, int mode, const int *list, int 
       CHK(launch_xfill(x, X, nol));
     }
   }
+  if ((dmask & 6) == 6) {
+    int dm[16];
+    for (int t = 0; t < nk; ++t) dm[t] = 3;
+    return fill_yz(x, a, dm, nk, nol);
+  }
   for (int d = 1; d <= 2; ++d)
     if (dmask & (1 << d)) CHK(fill_periodic(x, a, nk, d, nol));
   return MCF_OK;
 }
 
 enum { OT_DXLAG = 1, OT_BCC, OT_BLG, OT_VFC, OT_SPECIES };
+
+// the fields a BC_MPI_OT piece fills along axis d
+static int ot_fields(mcf_ctx *x, int what, int d, double **a) {
+  int nf = 0;
+  if (what == OT_DXLAG) a[nf++] = x->dxlagM;
+  if (what == OT_BCC) for (int c = 0; c < 3; ++c) a[nf++] = x->bcc[c];
+  if (what == OT_BLG) { a[nf++] = x->blg[0]; a[nf++] = x->blg[1]; }
+  if (what == OT_VFC) {
+    int c1 = d == 0 ? 1 : 0, c2 = d == 2 ? 1 : 2; // x: Vfc 2,3; y: 1,3; z: 1,2
+    a[nf++] = x->vfc[c1]; a[nf++] = x->vfc[c2];
+  }
+  if (what == OT_SPECIES) for (int s = 0; s < 8; ++s) a[nf++] = x->sp[s];
+  return nf;
+}
 
 // BC_MPI_OT(nol, .) pieces (M3:930-1254)
 static int fillOT(mcf_ctx *x, int dmask, int nol, int what) {
@@ -291,15 +322,22 @@ static int fillOT(mcf_ctx *x, int dmask, int nol, int what) {
   for (int d = 0; d < 3; ++d) {
     if (!(dmask & (1 << d))) continue;
     double *a[16];
-    int nf = 0;
-    if (what == OT_DXLAG) a[nf++] = x->dxlagM;
-    if (what == OT_BCC) for (int c = 0; c < 3; ++c) a[nf++] = x->bcc[c];
-    if (what == OT_BLG) { a[nf++] = x->blg[0]; a[nf++] = x->blg[1]; }
-    if (what == OT_VFC) {
-      int c1 = d == 0 ? 1 : 0, c2 = d == 2 ? 1 : 2; // x: Vfc 2,3; y: 1,3; z: 1,2
-      a[nf++] = x->vfc[c1]; a[nf++] = x->vfc[c2];
+    if (d == 1 && (dmask & 4)) {
+      // y and z in one launch: the union of the two passes' fields
+      double *u[16];
+      int dm[16], nu = 0;
+      for (int e = 1; e <= 2; ++e) {
+        const int nf = ot_fields(x, what, e, a);
+        for (int t = 0; t < nf; ++t) {
+          int q = 0;
+          while (q < nu && u[q] != a[t]) ++q;
+          if (q == nu) { u[nu] = a[t]; dm[nu] = 0; ++nu; }
+          dm[q] |= e;
+        }
+      }
+      return fill_yz(x, u, dm, nu, nol);
     }
-    if (what == OT_SPECIES) for (int s = 0; s < 8; ++s) a[nf++] = x->sp[s];
+    const int nf = ot_fields(x, what, d, a);
     if (d != 0 || !x_inflow(x)) { CHK(fill_periodic(x, a, nf, d, nol)); continue; }
     XFill X{};
     X.nf = nf;

@@ -38,6 +38,7 @@ __device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
 struct FillList {
   double *a[16];
   int nf;
+  int dm[16]; // k_fill_yz: the passes of each field (bit 0: y, bit 1: z)
 };
 
 enum XRule { XR_PERIODIC = 0, XR_CONST, XR_COPY, XR_FACE, XR_MOM, XR_ENERGY, XR_SPECIES };
@@ -145,6 +146,7 @@ struct DisArgs {
 };
 
 __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol);
+__global__ void k_fill_yz(FillList L, Geo g, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_cct3(CtArgs A, Geo g, Box b, double dt);

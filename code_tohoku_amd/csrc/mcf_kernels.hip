@@ -78,6 +78,46 @@ __global__ void k_fill_periodic(FillList L, Geo g, int d, int nol) {
   for (int f = 0; f < L.nf; ++f) L.a[f][base + gpos * sd] = L.a[f][base + spos * sd];
 }
 
+// The periodic y pass and then the periodic z pass of a fill in one launch.
+// Done in sequence, the passes leave at a ghost point (i, j, k) the value of
+// (i, jw, kw), where jw = j -+ n1 if the point's field has a y pass and j is
+// one of its nol ghost layers (else jw = j), and kw likewise along z; no
+// such source point is written by either pass, so every point can copy its
+// source directly.  Points: the y ghost layers over all (i, k), then the z
+// ghost layers over all i and the j outside the y ghost layers.
+__global__ void k_fill_yz(FillList L, Geo g, int nol) {
+  const int m0 = g.n[0] + 4, m1 = g.n[1] + 4, m2 = g.n[2] + 4, n1 = g.n[1], n2 = g.n[2];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ra = 2L * nol * m0 * m2, rb = 2L * nol * m0 * (m1 - 2 * nol);
+  int i, j, k; // memory positions (index + 1)
+  if (t < ra) {
+    i = (int)(t % m0);
+    long r = t / m0;
+    k = (int)(r % m2);
+    const int layer = (int)(r / m2);
+    j = layer < nol ? 2 - nol + layer : n1 + 2 + (layer - nol);
+  } else if ((t -= ra) < rb) {
+    i = (int)(t % m0);
+    long r = t / m0;
+    const int q = (int)(r % (m1 - 2 * nol));
+    const int layer = (int)(r / (m1 - 2 * nol));
+    j = q < 2 - nol ? q : (q < 2 - nol + n1 ? q + nol : q + 2 * nol);
+    k = layer < nol ? 2 - nol + layer : n2 + 2 + (layer - nol);
+  } else {
+    return;
+  }
+  // ghost layers within nol: memory positions 2-nol..1 and n+2..n+1+nol
+  const bool yg = (j >= 2 - nol && j <= 1) || (j >= n1 + 2 && j <= n1 + 1 + nol);
+  const bool zg = (k >= 2 - nol && k <= 1) || (k >= n2 + 2 && k <= n2 + 1 + nol);
+  const int jw = j <= 1 ? j + n1 : j - n1, kw = k <= 1 ? k + n2 : k - n2;
+  const int c = i + j * g.st[1] + k * g.st[2];
+  for (int f = 0; f < L.nf; ++f) {
+    const bool wy = yg && (L.dm[f] & 1), wz = zg && (L.dm[f] & 2);
+    if (!wy && !wz) continue;
+    L.a[f][c] = L.a[f][i + (wy ? jw : j) * g.st[1] + (wz ? kw : k) * g.st[2]];
+  }
+}
+
 // x-ghost rules of BC_MPI / BC_MPI_OT at an inflow (BC=4) end
 // (M3:818-884, 948-1092).  One thread per ghost point and side, the ghost
 // layers of a row on adjacent threads (adjacent in memory); the fields are
