@@ -611,7 +611,8 @@ static int do_source(mcf_ctx *x, double dt) {
     A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2 + x->src_pair);
     A.qz = reinterpret_cast<unsigned *>(x->cool_cnt + 3 - x->src_pair);
     {
-      dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + SRC_ROWS - 1) / SRC_ROWS), (unsigned)((b.cnt[2] + SRC_ZSEG - 1) / SRC_ZSEG));
+      const int zseg = x->mixed ? SRC_ZSEG_F32 : SRC_ZSEG_F64;
+      dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + SRC_ROWS - 1) / SRC_ROWS), (unsigned)((b.cnt[2] + zseg - 1) / zseg));
       if (x->mixed) KLAUNCH_N("k_src_col", k_src_col<float>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
       else KLAUNCH_N("k_src_col", k_src_col<double>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
     }

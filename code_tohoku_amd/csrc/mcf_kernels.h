@@ -153,7 +153,8 @@ template <class R> __global__ void k_cct3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);
-constexpr int SRC_ZSEG = 16; // planes per block of k_src_col
+// planes per block of k_src_col: FP32 rates (mixed) 64, FP64 32
+constexpr int SRC_ZSEG_F32 = 64, SRC_ZSEG_F64 = 32;
 constexpr int SRC_ROWS = 8;  // x rows (waves) per block of k_src_col
 __global__ void k_shell_copy(const double *__restrict__ src, double *__restrict__ dst, Geo g);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);

@@ -376,7 +376,7 @@ __device__ __forceinline__ double qflux(double Tc, double Tcs, double dxi, doubl
 // k_src_imc, so the bisection runs on dense waves.
 //
 // The cells are visited as a march along z: a block is SRC_ROWS x rows of 64
-// cells (one wave per row) over SRC_ZSEG planes.  Per plane the whole block
+// cells (one wave per row) over SRC_ZSEG_F32 / _F64 planes.  Per plane the whole block
 // forms the temperatures of its rows plus a one-cell frame in x and y
 // (coalesced rows) into a 4-plane LDS ring, so each cell reads its six
 // neighbours from LDS and every temperature is formed once per block (one
@@ -432,8 +432,9 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
   const int hi0 = b.lo[0] + b.cnt[0] - 1, hi1 = b.lo[1] + b.cnt[1] - 1, hi2 = b.lo[2] + b.cnt[2] - 1;
   const int i = b.lo[0] + 64 * bx + lane;
   const int j = b.lo[1] + SRC_ROWS * by + (int)(threadIdx.x >> 6);
-  const int k0 = b.lo[2] + SRC_ZSEG * bz;
-  const int k1 = min(hi2, k0 + SRC_ZSEG - 1);
+  constexpr int ZSEG = Ar<R>::strict ? SRC_ZSEG_F64 : SRC_ZSEG_F32;
+  const int k0 = b.lo[2] + ZSEG * bz;
+  const int k1 = min(hi2, k0 + ZSEG - 1);
   const bool own = i <= hi0 && j <= hi1;
   const int ic = min(i, hi0 + 1), jc = min(j, hi1 + 1);
   const int w = (int)(threadIdx.x >> 6);
