@@ -443,11 +443,12 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     if (S.inflow[0] && S.inflow[1]) {
       // inflow on both sides: the whole sweep in one kernel (the REMAP
       // stage's ghosts are the fill rules, applied in the kernel)
-      dim3 gs((unsigned)((nx + XSCELLS - 1) / XSCELLS), (unsigned)((x->g.n[1] + 3) / 4), (unsigned)x->g.n[2]);
+      // two rows (waves) per block: 2% faster than four, 6% faster than eight
+      dim3 gs((unsigned)((nx + XSCELLS - 1) / XSCELLS), (unsigned)((x->g.n[1] + 1) / 2), (unsigned)x->g.n[2]);
       // reads rho, v(3), p, Bcc(3), face B_x, species(8); writes rho, v(3), p, species(8)
       const double bys = 8.0 * (double)nx * x->g.n[1] * x->g.n[2] * 30;
-      if (x->mixed) KLAUNCH_N("k_sweep_x", k_sweep_x<float>, gs, dim3(256), bys, A, x->g, S, dt);
-      else KLAUNCH_N("k_sweep_x", k_sweep_x<double>, gs, dim3(256), bys, A, x->g, S, dt);
+      if (x->mixed) KLAUNCH_N("k_sweep_x", k_sweep_x<float>, gs, dim3(128), bys, A, x->g, S, dt);
+      else KLAUNCH_N("k_sweep_x", k_sweep_x<double>, gs, dim3(128), bys, A, x->g, S, dt);
       for (int f = 0; f < 5; ++f) swapU(x, f);
       swapSp(x);
       return MCF_OK;

@@ -1255,7 +1255,7 @@ __global__ __launch_bounds__(256, Ar<R>::strict ? 3 : 4) void k_sweep_x(SweepIO 
   xcd_tile(bx, by, bz);
   const int lane = threadIdx.x & 63;
   const int n = g.n[0];
-  const int j = 1 + 4 * by + (int)(threadIdx.x >> 6), k = 1 + bz;
+  const int j = 1 + 2 * by + (int)(threadIdx.x >> 6), k = 1 + bz;
   if (j > g.n[1]) return; // whole waves
   const int c0 = 1 + XSCELLS * bx;
   const int p = c0 - 6 + lane;
