@@ -121,6 +121,31 @@ def test_multistep_vs_oracle_long_columns():
     assert worst(errs) <= MULTI_TOL, errs
 
 
+@pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
+def test_multistep_vs_oracle_long_z(precision):
+    """z longer than one column segment (128 cells) and than SOURCE's
+    z segments (64 planes mixed, 32 FP64), not a multiple of either: the
+    oracle's result (FP64) and agreement to FP32 round-off (mixed)."""
+    p = small(24, 16, 136)
+    st = ics.random_state(p, seed=14)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2), precision=precision)
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    if precision == mcf.PREC_FP64:
+        dts_g = g.run(2)
+        np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+        errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+        assert worst(errs) <= MULTI_TOL, errs
+    else:
+        for dt in dts_o:
+            g.step(dt)
+        got = interior(g.download(), p)
+        ref = interior(o.state(), p)
+        for f in range(8):
+            assert np.abs(got[f] - ref[f]).max() <= 1e-5 * np.abs(ref[f]).max(), f
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 def test_multistep_vs_oracle_long_rows(periodic):
     """x longer than one wave of the fused x sweep (56 cells), not a multiple
