@@ -657,6 +657,9 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   if ((cfg->bcx1 != 3 && cfg->bcx1 != 4) || (cfg->bcx2 != 3 && cfg->bcx2 != 4)) {
     g_err = "x boundaries must be 3 (periodic) or 4 (inflow)"; return MCF_E_ARG;
   }
+  // an inflow end opposite a periodic end is no configuration of the
+  // reference (INPUT3D.DAT: 4,4) and its parity is not established
+  if (cfg->bcx1 != cfg->bcx2) { g_err = "x boundaries must be both inflow (4) or both periodic (3)"; return MCF_E_ARG; }
   // DISSIP works on 10-cell layers at the x ends (M3:3152-3170)
   if (n[0] / nsp[0] < 10) { g_err = "x block must hold at least 10 cells (DISSIP layers)"; return MCF_E_ARG; }
   if (cfg->precision != MCF_PREC_FP64 && cfg->precision != MCF_PREC_MIXED) { g_err = "unknown precision mode"; return MCF_E_ARG; }

@@ -162,6 +162,32 @@ def test_multistep_vs_oracle_long_rows(periodic):
     assert worst(errs) <= MULTI_TOL, errs
 
 
+def test_nonuniform_inflow_x_uses_stage_kernels():
+    """x inflow with two spacings (16 cells over 10 pc, 8 over 10 pc): the x
+    sweep runs as the three stage kernels, so REMAP's x ghosts come from the
+    mode-2 fill rules (momentum from the filled density, energy from
+    density, Bcc_x and Blg); FP64 result against the oracle."""
+    p = Input3D(np1x=16, np2x=8, np1y=8, np2y=8, np1z=8, np2z=8,
+                ql1x=10.0, ql2x=10.0, ql1y=5.0, ql2y=5.0, ql1z=5.0, ql2z=5.0)
+    st = ics.random_state(p, seed=41)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2))
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
+def test_mixed_x_boundaries_rejected():
+    """An inflow end opposite a periodic end is refused at set-up."""
+    p = small(24, 16, 16)
+    p.bcx2 = 3
+    with pytest.raises(mcf.MCFError, match="both inflow"):
+        mcf.Domain(p, nsplit=(2, 2, 2))
+
+
 def test_nonuniform_periodic_x_uses_stage_kernels():
     """Periodic x with two spacings (16 cells over 10 pc, 8 over 10 pc) runs
     the x sweep as the three stage kernels; FP64 result against the oracle."""
