@@ -227,16 +227,6 @@ static long boxsize(const Box &b) { return (long)b.cnt[0] * b.cnt[1] * b.cnt[2];
 // ----------------------------------------------------------------------------
 // ghost fills
 // ----------------------------------------------------------------------------
-static int fill_periodic(mcf_ctx *x, double *const *a, int nf, int d, int nol) {
-  FillList L{};
-  L.nf = nf;
-  for (int f = 0; f < nf; ++f) L.a[f] = a[f];
-  int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
-  long nt = (long)(x->g.n[da] + 4) * (x->g.n[db] + 4) * 2 * nol;
-  KLAUNCH(k_fill_periodic, nt, 2 * nf, L, x->g, d, nol);
-  return MCF_OK;
-}
-
 // The y and z periodic passes of one fill in one launch (k_fill_yz); dm[f]
 // says which of the two passes field f takes (bit 0: y, bit 1: z).
 static int fill_yz(mcf_ctx *x, double *const *a, const int *dm, int nf, int nol) {
@@ -261,44 +251,6 @@ static int launch_xfill(mcf_ctx *x, XFill &X, int nol) {
   return MCF_OK;
 }
 
-// BC_MPI(nol, mode) for the 1-based U components in list (M3:800-926)
-static int fillU(mcf_ctx *x, int dmask, int nol, int mode, const int *list, int nk) {
-  double *a[16];
-  for (int t = 0; t < nk; ++t) a[t] = x->U[list[t] - 1];
-  if (dmask & 1) {
-    if (!x_inflow(x)) {
-      CHK(fill_periodic(x, a, nk, 0, nol));
-    } else {
-      XFill X{};
-      X.nf = nk;
-      int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
-      for (int t = 0; t < nk; ++t) {
-        int K = list[t];
-        X.a[t] = a[t];
-        for (int s = 0; s < 2; ++s) {
-          int side = s + 1;
-          if (bc[s] != 4) { X.rule[t][s] = XR_PERIODIC; continue; }
-          if (K == 1) { X.rule[t][s] = XR_CONST; X.val[t][s] = x->dtf; }
-          else if (K == 6) X.rule[t][s] = XR_FACE;
-          else if (K == 5 && mode == 2) {
-            X.rule[t][s] = XR_ENERGY; X.val[t][s] = x->bbrv[5][1][side]; X.val2[s] = x->bbrv[2][1][side];
-          } else if (K == 2 && mode == 2) { X.rule[t][s] = XR_MOM; X.val[t][s] = x->bbrv[2][1][side]; }
-          else { X.rule[t][s] = XR_CONST; X.val[t][s] = x->bbrv[K][mode][side]; }
-        }
-      }
-      CHK(launch_xfill(x, X, nol));
-    }
-  }
-  if ((dmask & 6) == 6) {
-    int dm[16];
-    for (int t = 0; t < nk; ++t) dm[t] = 3;
-    return fill_yz(x, a, dm, nk, nol);
-  }
-  for (int d = 1; d <= 2; ++d)
-    if (dmask & (1 << d)) CHK(fill_periodic(x, a, nk, d, nol));
-  return MCF_OK;
-}
-
 enum { OT_DXLAG = 1, OT_BCC, OT_BLG, OT_VFC, OT_SPECIES };
 
 // the fields a BC_MPI_OT piece fills along axis d
@@ -315,52 +267,112 @@ static int ot_fields(mcf_ctx *x, int what, int d, double **a) {
   return nf;
 }
 
-// BC_MPI_OT(nol, .) pieces (M3:930-1254)
-static int fillOT(mcf_ctx *x, int dmask, int nol, int what) {
-  int nx = x->g.n[0];
-  int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
-  for (int d = 0; d < 3; ++d) {
-    if (!(dmask & (1 << d))) continue;
-    double *a[16];
-    if (d == 1 && (dmask & 4)) {
-      // y and z in one launch: the union of the two passes' fields
-      double *u[16];
-      int dm[16], nu = 0;
-      for (int e = 1; e <= 2; ++e) {
-        const int nf = ot_fields(x, what, e, a);
-        for (int t = 0; t < nf; ++t) {
-          int q = 0;
-          while (q < nu && u[q] != a[t]) ++q;
-          if (q == nu) { u[nu] = a[t]; dm[nu] = 0; ++nu; }
-          dm[q] |= e;
-        }
-      }
-      return fill_yz(x, u, dm, nu, nol);
+// Fills that run back to back, gathered into one x launch (the x rules or
+// the periodic copy, fields in call order) and one y/z launch.  Equal to the
+// calls in sequence when no x rule reads, at a y or z ghost row, a field an
+// earlier call fills: the rules that read other fields are the species rule
+// (interior rows only) and the mode-2 momentum and energy rules, which are
+// never gathered with another call.
+struct FillPlan {
+  int nol;
+  XFill X{};
+  double *yz[16];
+  int dm[16];
+  int nyz = 0;
+  bool over = false; // more than 16 fields in a pass
+  explicit FillPlan(int n) : nol(n) {}
+  int add_x() {
+    if (X.nf == 16) { over = true; return 15; }
+    return X.nf++;
+  }
+  void add_yz(double *p, int bits) {
+    int q = 0;
+    while (q < nyz && yz[q] != p) ++q;
+    if (q == nyz) {
+      if (nyz == 16) { over = true; return; }
+      yz[nyz] = p; dm[nyz] = 0; ++nyz;
     }
-    const int nf = ot_fields(x, what, d, a);
-    if (d != 0 || !x_inflow(x)) { CHK(fill_periodic(x, a, nf, d, nol)); continue; }
-    XFill X{};
-    X.nf = nf;
-    for (int t = 0; t < nf; ++t) {
-      X.a[t] = a[t];
+    dm[q] |= bits;
+  }
+};
+
+// BC_MPI(nol, mode) for the 1-based U components in list (M3:800-926)
+static void planU(mcf_ctx *x, FillPlan &P, int dmask, int mode, const int *list, int nk) {
+  int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
+  for (int t = 0; t < nk; ++t) {
+    const int K = list[t];
+    double *a = x->U[K - 1];
+    if (dmask & 1) {
+      XFill &X = P.X;
+      const int u = P.add_x();
+      X.a[u] = a;
       for (int s = 0; s < 2; ++s) {
         int side = s + 1;
-        if (bc[s] != 4) { X.rule[t][s] = XR_PERIODIC; continue; }
+        if (bc[s] != 4) { X.rule[u][s] = XR_PERIODIC; continue; }
+        if (K == 1) { X.rule[u][s] = XR_CONST; X.val[u][s] = x->dtf; }
+        else if (K == 6) X.rule[u][s] = XR_FACE;
+        else if (K == 5 && mode == 2) {
+          X.rule[u][s] = XR_ENERGY; X.val[u][s] = x->bbrv[5][1][side]; X.val2[s] = x->bbrv[2][1][side];
+        } else if (K == 2 && mode == 2) { X.rule[u][s] = XR_MOM; X.val[u][s] = x->bbrv[2][1][side]; }
+        else { X.rule[u][s] = XR_CONST; X.val[u][s] = x->bbrv[K][mode][side]; }
+      }
+    }
+    if (dmask & 6) P.add_yz(a, (dmask >> 1) & 3);
+  }
+}
+
+// BC_MPI_OT(nol, .) pieces (M3:930-1254)
+static void planOT(mcf_ctx *x, FillPlan &P, int dmask, int what) {
+  int nx = x->g.n[0];
+  int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
+  double *a[16];
+  if (dmask & 1) {
+    const int nf = ot_fields(x, what, 0, a);
+    XFill &X = P.X;
+    for (int t = 0; t < nf; ++t) {
+      const int u = P.add_x();
+      X.a[u] = a[t];
+      for (int s = 0; s < 2; ++s) {
+        int side = s + 1;
+        if (bc[s] != 4) { X.rule[u][s] = XR_PERIODIC; continue; }
         switch (what) {
-        case OT_DXLAG: X.rule[t][s] = XR_CONST; X.val[t][s] = x->hdx[0][s == 0 ? 1 : nx + 2]; break;
+        case OT_DXLAG: X.rule[u][s] = XR_CONST; X.val[u][s] = x->hdx[0][s == 0 ? 1 : nx + 2]; break;
         case OT_BCC:
-          if (t == 0) { X.rule[t][s] = XR_COPY; X.src[t][s] = s == 0 ? 1 : nx; }
-          else { X.rule[t][s] = XR_CONST; X.val[t][s] = x->bbrv[6 + t][1][side]; }
+          if (t == 0) { X.rule[u][s] = XR_COPY; X.src[u][s] = s == 0 ? 1 : nx; }
+          else { X.rule[u][s] = XR_CONST; X.val[u][s] = x->bbrv[6 + t][1][side]; }
           break;
-        case OT_BLG: X.rule[t][s] = XR_CONST; X.val[t][s] = x->bbrv[7 + t][1][side]; break;
-        case OT_VFC: X.rule[t][s] = XR_COPY; X.src[t][s] = s == 0 ? 1 : nx; break;
-        case OT_SPECIES: X.rule[t][s] = XR_SPECIES; X.val[t][s] = x->bbrv_cm[t]; break;
+        case OT_BLG: X.rule[u][s] = XR_CONST; X.val[u][s] = x->bbrv[7 + t][1][side]; break;
+        case OT_VFC: X.rule[u][s] = XR_COPY; X.src[u][s] = s == 0 ? 1 : nx; break;
+        case OT_SPECIES: X.rule[u][s] = XR_SPECIES; X.val[u][s] = x->bbrv_cm[t]; break;
         }
       }
     }
-    CHK(launch_xfill(x, X, nol));
   }
+  for (int d = 1; d <= 2; ++d) {
+    if (!(dmask & (1 << d))) continue;
+    const int nf = ot_fields(x, what, d, a);
+    for (int t = 0; t < nf; ++t) P.add_yz(a[t], d);
+  }
+}
+
+// the x launch, then the y/z launch
+static int flush(mcf_ctx *x, FillPlan &P) {
+  if (P.over) { g_err = "internal: a fill plan holds more than 16 fields"; return MCF_E_STATE; }
+  if (P.X.nf > 0) CHK(launch_xfill(x, P.X, P.nol));
+  if (P.nyz > 0) CHK(fill_yz(x, P.yz, P.dm, P.nyz, P.nol));
   return MCF_OK;
+}
+
+static int fillU(mcf_ctx *x, int dmask, int nol, int mode, const int *list, int nk) {
+  FillPlan P(nol);
+  planU(x, P, dmask, mode, list, nk);
+  return flush(x, P);
+}
+
+static int fillOT(mcf_ctx *x, int dmask, int nol, int what) {
+  FillPlan P(nol);
+  planOT(x, P, dmask, what);
+  return flush(x, P);
 }
 
 // ----------------------------------------------------------------------------
@@ -423,8 +435,12 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // neighbours by shuffles; the ghost cells both stages read are filled
     // first (rho, v(3), p, face B_x, species), as the two stages' fills do
     int l6[6] = {1, 2, 3, 4, 5, 6};
-    CHK(fillU(x, 1, 2, 1, l6, 6));
-    CHK(fillOT(x, 1, 2, OT_SPECIES));
+    {
+      FillPlan P(2);
+      planU(x, P, 1, 1, l6, 6);
+      planOT(x, P, 1, OT_SPECIES);
+      CHK(flush(x, P));
+    }
     A = sweep_io(x, 0);
     XSide S{};
     const int bc[2] = {x->cfg.bcx1, x->cfg.bcx2};
@@ -551,14 +567,22 @@ static int do_ct(mcf_ctx *x, double dt) {
   CtArgs A = ct_args(x);
   int n[3] = {x->g.n[0], x->g.n[1], x->g.n[2]};
   int lx[3] = {2, 7, 8}, ly[3] = {3, 6, 8}, lz[3] = {4, 6, 7}, l5[1] = {5}, l1[1] = {1};
-  CHK(fillU(x, 1, 2, 1, lx, 3));
-  CHK(fillU(x, 2, 2, 1, ly, 3));
-  CHK(fillU(x, 4, 2, 1, lz, 3));
+  {
+    FillPlan P(2);
+    planU(x, P, 1, 1, lx, 3);
+    planU(x, P, 2, 1, ly, 3);
+    planU(x, P, 4, 1, lz, 3);
+    CHK(flush(x, P));
+  }
   Box b1 = mkbox(1, n[0] + 1, 1, n[1] + 1, 1, n[2] + 1);
   KLAUNCH512(k_cc1, boxsize(b1), 6, A, x->g, b1);
-  CHK(fillOT(x, 7, 2, OT_VFC));
-  CHK(fillU(x, 7, 2, 1, l5, 1));
-  CHK(fillU(x, 7, 2, 1, l1, 1));
+  {
+    FillPlan P(2);
+    planOT(x, P, 7, OT_VFC);
+    planU(x, P, 7, 1, l5, 1);
+    planU(x, P, 7, 1, l1, 1);
+    CHK(flush(x, P));
+  }
   // the three edge orientations in one launch over (1..n+1)^3: reads rho, p,
   // Vfc(3), face B(3), writes EMF(3)
   if (x->mixed) KLAUNCH_N("k_cct3", k_cct3<float>, dim3(nblocks(boxsize(b1))), dim3(TPB), 8.0 * boxsize(b1) * 11, A, x->g, b1, dt);
@@ -587,8 +611,12 @@ static int do_source(mcf_ctx *x, double dt) {
   }
   int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
   int l15[2] = {1, 5}, l5[1] = {5};
-  CHK(fillU(x, 7, 1, 1, l15, 2));
-  CHK(fillOT(x, 7, 1, OT_SPECIES));
+  {
+    FillPlan P(1);
+    planU(x, P, 7, 1, l15, 2);
+    planOT(x, P, 7, OT_SPECIES);
+    CHK(flush(x, P));
+  }
   Box bc = mkbox(1, nx, 1, ny, 1, nz);
   double *P = x->U[F_P], *Q = x->Ub[F_P];
   for (int stage = 0; stage < 2; ++stage) {

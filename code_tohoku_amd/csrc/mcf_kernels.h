@@ -145,7 +145,6 @@ struct DisArgs {
   double *divv;
 };
 
-__global__ void k_fill_periodic(FillList L, Geo g, int d, int nol);
 __global__ void k_fill_yz(FillList L, Geo g, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);

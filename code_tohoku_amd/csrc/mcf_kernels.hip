@@ -47,37 +47,6 @@ __device__ __forceinline__ double gammi1_of(const double *const *sp, int c) {
 // ---------------------------------------------------------------------------
 // ghost fills
 // ---------------------------------------------------------------------------
-// Periodic copy of the nol ghost layers on both sides of axis d, over the
-// full transverse index range, for up to 16 arrays.
-__global__ void k_fill_periodic(FillList L, Geo g, int d, int nol) {
-  int n = g.n[d];
-  int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
-  int na = g.n[da] + 4, nb = g.n[db] + 4;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long tot = (long)na * nb * (2 * nol);
-  if (t >= tot) return;
-  // along x the ghost layers of a row are adjacent in memory: layers fastest
-  // there; along y and z the rows (index a) are
-  int a, b, layer; // layer 0..2*nol-1
-  if (d == 0) {
-    layer = (int)(t % (2 * nol));
-    long r = t / (2 * nol);
-    a = (int)(r % na);
-    b = (int)(r / na);
-  } else {
-    a = (int)(t % na);
-    long r = t / na;
-    b = (int)(r % nb);
-    layer = (int)(r / nb);
-  }
-  int sd = g.st[d];
-  int base = a * g.st[da] + b * g.st[db] + sd; // position 0 along d
-  int gpos, spos;
-  if (layer < nol) { gpos = 1 - nol + layer; spos = gpos + n; }
-  else { gpos = n + 1 + (layer - nol); spos = gpos - n; }
-  for (int f = 0; f < L.nf; ++f) L.a[f][base + gpos * sd] = L.a[f][base + spos * sd];
-}
-
 // The periodic y pass and then the periodic z pass of a fill in one launch.
 // Done in sequence, the passes leave at a ghost point (i, j, k) the value of
 // (i, jw, kw), where jw = j -+ n1 if the point's field has a y pass and j is
@@ -118,7 +87,8 @@ __global__ void k_fill_yz(FillList L, Geo g, int nol) {
   }
 }
 
-// x-ghost rules of BC_MPI / BC_MPI_OT at an inflow (BC=4) end
+// x-ghost rules of BC_MPI / BC_MPI_OT, the periodic copy (XR_PERIODIC) or
+// an inflow (BC=4) end's rules
 // (M3:818-884, 948-1092).  One thread per ghost point and side, the ghost
 // layers of a row on adjacent threads (adjacent in memory); the fields are
 // processed in list order, as the reference's K loop does.  A thread reads
