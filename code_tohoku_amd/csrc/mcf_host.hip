@@ -430,6 +430,7 @@ static dim3 sweep_grid(const mcf_ctx *x, int d) {
 // MHD (M3:1264-1324): RHS -> MOC -> REMAP -> primitives
 static int do_sweep(mcf_ctx *x, int d, double dt) {
   SweepIO A = sweep_io(x, d);
+  bool dxlag_fill = false;
   if (d == 0 && x->fuse[0]) {
     // x: RHS and MOC in one kernel, waves of 64 cells along x exchanging
     // neighbours by shuffles; the ghost cells both stages read are filled
@@ -475,7 +476,7 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     else KLAUNCH_N("k_rhsmoc_x", k_rhsmoc_x<double>, gr, dim3(256), by, A, x->g, S, dt);
     for (int f = 0; f < 5; ++f) swapU(x, f);
     swapSp(x); swapBlg(x);
-    CHK(fillOT(x, 1, 2, OT_DXLAG));
+    dxlag_fill = true; // gathered with REMAP's fills below
   } else if (x->fuse[d]) {
     // y/z: RHS and MOC in one column march over the periodic axis; it reads
     // only interior cells (periodic images), so no fills precede it
@@ -511,27 +512,49 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // RHS (fills M3:1346-1357): reads rho, v(3), p, Bcc(3), species; writes
     // rho, v_n, e, Blg(2), species, interface velocity / position, widths
     int lr[3] = {1, A.VN + 1, 5};
-    CHK(fillU(x, 1 << d, 2, 1, lr, 3));
-    CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
+    {
+      FillPlan P(2);
+      planU(x, P, 1 << d, 1, lr, 3);
+      planOT(x, P, 1 << d, OT_SPECIES);
+      CHK(flush(x, P));
+    }
     A = sweep_io(x, d);
     SWEEP_LAUNCH(k_rhs, d, true, 33, A, x->g, dt);
     swapU(x, 0); swapU(x, A.VN); swapSp(x); swapBlg(x);
     // MOC (fills M3:1950-1976): reads rho, v_t(2), Blg(2), p, e, widths, Bn;
     // writes momenta(3), E, Blg(2)
     int lm[4] = {1, A.VT1 + 1, A.VT2 + 1, A.BN + 1};
-    CHK(fillU(x, 1 << d, 2, 1, lm, 4));
-    CHK(fillOT(x, 1 << d, 2, OT_DXLAG));
-    CHK(fillOT(x, 1 << d, 2, OT_BLG));
+    {
+      FillPlan P(2);
+      planU(x, P, 1 << d, 1, lm, 4);
+      planOT(x, P, 1 << d, OT_DXLAG);
+      planOT(x, P, 1 << d, OT_BLG);
+      CHK(flush(x, P));
+    }
     A = sweep_io(x, d);
     SWEEP_LAUNCH(k_moc, d, true, 20, A, x->g, dt);
     swapU(x, A.VN); swapU(x, A.VT1); swapU(x, A.VT2); swapU(x, 4); swapBlg(x);
   }
   // REMAP (fills M3:1571-1594): reads rho, m(3), E, Blg(2), species,
   // widths, interface velocity, Bcc_n; writes rho, v(3), p, species
-  CHK(fillOT(x, 1 << d, 2, OT_BLG));
   int l5[5] = {1, 2, 3, 4, 5};
-  CHK(fillU(x, 1 << d, 2, 2, l5, 5));
-  CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
+  {
+    // mode 2's x rules read rho, Bcc_x and Blg on every row, so with x
+    // inflow these fills run call by call; otherwise they are gathered
+    FillPlan P(2);
+    if (dxlag_fill) planOT(x, P, 1, OT_DXLAG);
+    if (d == 0 && x_inflow(x)) {
+      CHK(flush(x, P));
+      CHK(fillOT(x, 1 << d, 2, OT_BLG));
+      CHK(fillU(x, 1 << d, 2, 2, l5, 5));
+      CHK(fillOT(x, 1 << d, 2, OT_SPECIES));
+    } else {
+      planOT(x, P, 1 << d, OT_BLG);
+      planU(x, P, 1 << d, 2, l5, 5);
+      planOT(x, P, 1 << d, OT_SPECIES);
+      CHK(flush(x, P));
+    }
+  }
   A = sweep_io(x, d);
   if (d == 0 && x->fuse[0]) {
     dim3 gr((unsigned)((x->g.n[0] + XRCELLS - 1) / XRCELLS), (unsigned)((x->g.n[1] + 3) / 4), (unsigned)x->g.n[2]);
