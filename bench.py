@@ -194,7 +194,7 @@ def main():
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build runs the whole domain on a single MI355X "
+        print(f"bench.py measures one GPU only: the library runs the whole domain on a single MI355X "
               f"and has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     if args.pmc_child:
