@@ -711,6 +711,10 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   // an inflow end opposite a periodic end is no configuration of the
   // reference (INPUT3D.DAT: 4,4) and its parity is not established
   if (cfg->bcx1 != cfg->bcx2) { g_err = "x boundaries must be both inflow (4) or both periodic (3)"; return MCF_E_ARG; }
+  // y and z are always periodic in the reference (BC_MPI, M3:888-926)
+  if (cfg->bcy1 != 3 || cfg->bcy2 != 3 || cfg->bcz1 != 3 || cfg->bcz2 != 3) {
+    g_err = "y and z boundaries must be 3 (periodic)"; return MCF_E_ARG;
+  }
   // DISSIP works on 10-cell layers at the x ends (M3:3152-3170)
   if (n[0] / nsp[0] < 10) { g_err = "x block must hold at least 10 cells (DISSIP layers)"; return MCF_E_ARG; }
   if (cfg->precision != MCF_PREC_FP64 && cfg->precision != MCF_PREC_MIXED) { g_err = "unknown precision mode"; return MCF_E_ARG; }
@@ -752,7 +756,17 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   x->divv = p; p += ndivv;
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
   x->red = p; p += 16 + 2 * DT_BLOCKS;
-  HIPCHK(hipMemsetAsync(x->pool, 0, total * sizeof(double), x->stream));
+  // from here on a failure frees everything allocated so far
+#define INITCHK(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) {                                                                        \
+      g_err = std::string(#call) + ": " + hipGetErrorString(e_);                                  \
+      (void)mcf_finalize(x);                                                                       \
+      return e_ == hipErrorOutOfMemory ? MCF_E_NOMEM : MCF_E_HIP;                                  \
+    }                                                                                              \
+  } while (0)
+  INITCHK(hipMemsetAsync(x->pool, 0, total * sizeof(double), x->stream));
   for (int d = 0; d < 3; ++d) {
     bool uni = n[d] >= 8;
     for (int k = 0; k < n[d] + 4 && uni; ++k) uni = std::memcmp(&x->hdx[d][k], &x->hdx[d][0], sizeof(double)) == 0;
@@ -761,16 +775,17 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
     x->fuse[d] = uni;
   }
   for (int d = 0; d < 3; ++d) {
-    HIPCHK(hipMemcpyAsync(x->gx[d], x->hx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(x->gdx[d], x->hdx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    INITCHK(hipMemcpyAsync(x->gx[d], x->hx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    INITCHK(hipMemcpyAsync(x->gdx[d], x->hdx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
   }
-  HIPCHK(hipHostMalloc((void **)&x->hred, 2 * sizeof(double), hipHostMallocDefault));
-  HIPCHK(hipMalloc(&x->mlp, 4 * MLP_NP * sizeof(float)));
-  HIPCHK(hipMalloc(&x->mlp_pk, MLP_NPK * sizeof(__bf16)));
-  HIPCHK(hipMalloc(&x->cool_cnt, 4 * sizeof(unsigned long long)));
-  HIPCHK(hipMemsetAsync(x->mlp, 0, 4 * MLP_NP * sizeof(float), x->stream));
-  HIPCHK(hipMemsetAsync(x->cool_cnt, 0, 4 * sizeof(unsigned long long), x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
+  INITCHK(hipHostMalloc((void **)&x->hred, 2 * sizeof(double), hipHostMallocDefault));
+  INITCHK(hipMalloc(&x->mlp, 4 * MLP_NP * sizeof(float)));
+  INITCHK(hipMalloc(&x->mlp_pk, MLP_NPK * sizeof(__bf16)));
+  INITCHK(hipMalloc(&x->cool_cnt, 4 * sizeof(unsigned long long)));
+  INITCHK(hipMemsetAsync(x->mlp, 0, 4 * MLP_NP * sizeof(float), x->stream));
+  INITCHK(hipMemsetAsync(x->cool_cnt, 0, 4 * sizeof(unsigned long long), x->stream));
+  INITCHK(hipStreamSynchronize(x->stream));
+#undef INITCHK
   *out = x;
   return MCF_OK;
 }
@@ -860,6 +875,13 @@ extern "C" int mcf_timestep(mcf_ctx *x, double tsave, double *dt_out, int *tag) 
   if (!(std::isfinite(dt)) || tCFL < 0.0 || tLMT < 0.0) {
     g_err = "non-finite or negative time step (err at Couran/Stblty)";
     return MCF_E_NONFINITE;
+  }
+  // EVOLVE stops before asking for a step once time reaches tfinal or the
+  // next save time (M3:567-568); a step of zero or negative length
+  // means the caller did not
+  if (dt <= 0.0) {
+    g_err = "no time step left: time has reached tfinal or tsave";
+    return MCF_E_STATE;
   }
   *dt_out = dt;
   if (tag) *tag = st;

@@ -14,11 +14,20 @@
  * Single block: every MPI halo exchange becomes an in-place ghost fill with
  * the reference's pass order (x over all j,k; then y over all i,k; then z
  * over whole planes) and its physical x-boundary rules (BC=4 inflow).
+ *
+ * Threads (OpenMP): the pencil and plane loops of every stage are split over
+ * threads.  Each pencil, cell, face or edge is computed by one thread with
+ * exactly the serial expressions (per-thread pencil work arrays), and the
+ * dt minima are combined with the same comparisons, so the result does not
+ * depend on the thread count (tests/test_oracle_golden.py checks 1 and 4).
  */
 #include "mcf_oracle.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define KB 8.63359            /* M3:11 */
 #define KCOND 1.6384e-2       /* M3:11 */
@@ -48,8 +57,24 @@ struct mcfo {
   double *Tn, *Pn, *Qx, *Qy, *Qz;
   double *ppre; /* U(:,:,:,5) right after the RHS ghost fill of a sweep */
   int nsp[3];   /* emulated MPI block split */
-  double *pw;   /* pencil work pool */
+  double *pw;   /* pencil work pool, one slice of 64 arrays per thread */
+  int nthr;     /* threads the pool has slices for */
 };
+
+static int nthreads_max(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+static int thread_id(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
 
 #define U(k) (o->f[(k)])
 #define SPC(s) (o->f[9 + (s)])
@@ -158,7 +183,8 @@ mcfo *mcfo_create(const mcfo_params *p) {
                     &o->vfc[2], &o->dnc, &o->emf[0], &o->emf[1], &o->emf[2], &o->dq[0], &o->dq[1],
                     &o->dq[2], &o->dq[3], &o->Tn, &o->Pn, &o->Qx, &o->Qy, &o->Qz, &o->ppre};
   for (size_t t = 0; t < sizeof(tmp) / sizeof(tmp[0]); t++) *tmp[t] = (double *)calloc(o->ncell, sizeof(double));
-  o->pw = (double *)calloc((size_t)64 * (o->nmax + 8), sizeof(double));
+  o->nthr = nthreads_max();
+  o->pw = (double *)calloc((size_t)o->nthr * 64 * (o->nmax + 8), sizeof(double));
   int nspl[3] = {p->nsplitx, p->nsplity, p->nsplitz};
   for (int d = 0; d < 3; d++) {
     o->nsp[d] = nspl[d] < 1 ? 1 : nspl[d];
@@ -167,6 +193,16 @@ mcfo *mcfo_create(const mcfo_params *p) {
   o->ifevo = 1;
   setup_constants(o);
   return o;
+}
+
+int mcfo_set_threads(mcfo *o, int nthreads) {
+  if (nthreads < 1) nthreads = nthreads_max();
+  double *pw = (double *)calloc((size_t)nthreads * 64 * (o->nmax + 8), sizeof(double));
+  if (!pw) return -1;
+  free(o->pw);
+  o->pw = pw;
+  o->nthr = nthreads;
+  return nthreads;
 }
 
 void mcfo_destroy(mcfo *o) {
@@ -482,9 +518,11 @@ static long pencil_base(const mcfo *o, int d, int L, int M) {
   return ix3(o, M, L, 0);
 }
 
-/* pencil work arrays: slot t gives an array valid on -1..nmax+2, returned
- * pointing at index 0 */
-static double *pwk(mcfo *o, int t) { return o->pw + (size_t)t * (o->nmax + 8) + 2; }
+/* pencil work arrays of the calling thread: slot t gives an array valid on
+ * -1..nmax+2, returned pointing at index 0 */
+static double *pwk(mcfo *o, int t) {
+  return o->pw + ((size_t)thread_id() * 64 + (size_t)t) * (o->nmax + 8) + 2;
+}
 
 static void rhs(mcfo *o, int d, double dt) {
   sweepdef w = sweep_def(o, d);
@@ -493,10 +531,13 @@ static void rhs(mcfo *o, int d, double dt) {
   fill_ot(o, 1 << d, 2, 5);
   memcpy(o->ppre, U(5), sizeof(double) * (size_t)o->ncell);
   const double *dxe = o->gdx[d] + 1;
-  double *g1 = pwk(o, 0), *gVN = pwk(o, 1), *g5 = pwk(o, 2), *gB1 = pwk(o, 3), *gB2 = pwk(o, 4);
-  double *Pa = pwk(o, 5), *Va = pwk(o, 6), *dm = pwk(o, 7);
   double *U1 = U(1), *UVN = U(w.VN), *U5 = U(5), *Bt1 = BCC(w.BT1), *Bt2 = BCC(w.BT2);
   double dtv = dt, fac = o->p.facdep;
+#pragma omp parallel num_threads(o->nthr)
+  {
+  double *g1 = pwk(o, 0), *gVN = pwk(o, 1), *g5 = pwk(o, 2), *gB1 = pwk(o, 3), *gB2 = pwk(o, 4);
+  double *Pa = pwk(o, 5), *Va = pwk(o, 6), *dm = pwk(o, 7);
+#pragma omp for collapse(2) schedule(static)
   for (int L = 1; L <= w.ncl; L++)
     for (int M = 1; M <= w.ncm; M++) {
       long b = pencil_base(o, d, L, M), s = w.s;
@@ -554,6 +595,7 @@ static void rhs(mcfo *o, int d, double dt) {
       }
       for (int i = 1; i <= w.n + 1; i++) o->vfc1[b + i * s] = Va[i - 1];
     }
+  }
 }
 
 /* One MOC interface (M3:1997-2056) from the slopes at positions i and i+1
@@ -618,6 +660,7 @@ static void moc(mcfo *o, int d, double dt) {
   sweepdef w = sweep_def(o, d);
   int list[4] = {1, w.VT1, w.VT2, w.BN};
   fillU(o, 1 << d, 2, 1, list, 4);
+#pragma omp parallel for collapse(2) schedule(static) num_threads(o->nthr)
   for (int L = 1; L <= w.ncl; L++)
     for (int M = 1; M <= w.ncm; M++) {
       long b = pencil_base(o, d, L, M), s = w.s;
@@ -627,14 +670,17 @@ static void moc(mcfo *o, int d, double dt) {
   fill_ot(o, 1 << d, 2, 3);
   const double *xe = o->gx[d] + 1;
   int nc = w.n / o->nsp[d];
+  double *U1 = U(1), *U5 = U(5), *UT1 = U(w.VT1), *UT2 = U(w.VT2), *UVN = U(w.VN), *UBn = U(w.BN);
+  double *Q[5] = {U1, UT1, UT2, o->blg[0], o->blg[1]};
+  double *Bl1 = o->blg[0], *Bl2 = o->blg[1];
+#pragma omp parallel num_threads(o->nthr)
+  {
   double *dxl = pwk(o, 0);
   double *g[5], *h[5];
   for (int q = 0; q < 5; q++) { g[q] = pwk(o, 1 + q); h[q] = pwk(o, 6 + q); }
   double *VayL = pwk(o, 11), *BayL = pwk(o, 12), *VazL = pwk(o, 13), *BazL = pwk(o, 14);
   double *VayR = pwk(o, 15), *BayR = pwk(o, 16), *VazR = pwk(o, 17), *BazR = pwk(o, 18);
-  double *U1 = U(1), *U5 = U(5), *UT1 = U(w.VT1), *UT2 = U(w.VT2), *UVN = U(w.VN), *UBn = U(w.BN);
-  double *Q[5] = {U1, UT1, UT2, o->blg[0], o->blg[1]};
-  double *Bl1 = o->blg[0], *Bl2 = o->blg[1];
+#pragma omp for collapse(2) schedule(static)
   for (int L = 1; L <= w.ncl; L++)
     for (int M = 1; M <= w.ncm; M++) {
       long b = pencil_base(o, d, L, M), s = w.s;
@@ -670,6 +716,7 @@ static void moc(mcfo *o, int d, double dt) {
         Bl2[c] = Bl2[c] + dt / dxl[i] * (Bnap * VazL[i] - Bnam * VazR[i - 1]);
       }
     }
+  }
 }
 
 static void remap(mcfo *o, int d, double dt) {
@@ -679,15 +726,18 @@ static void remap(mcfo *o, int d, double dt) {
   fillU(o, 1 << d, 2, 2, list, 5);
   fill_ot(o, 1 << d, 2, 5);
   const double *xe = o->gx[d] + 1;
-  double *dxl = pwk(o, 0);
-  double *grd[15], *F[15];
-  for (int q = 0; q < 15; q++) { grd[q] = pwk(o, 1 + q); F[q] = pwk(o, 16 + q); }
   double *Q[15];
   for (int q = 0; q < 5; q++) Q[q] = U(q + 1);
   Q[5] = o->blg[0];
   Q[6] = o->blg[1];
   for (int sp = 0; sp < 8; sp++) Q[7 + sp] = SPC(sp);
   double fac = o->p.facdep;
+#pragma omp parallel num_threads(o->nthr)
+  {
+  double *dxl = pwk(o, 0);
+  double *grd[15], *F[15];
+  for (int q = 0; q < 15; q++) { grd[q] = pwk(o, 1 + q); F[q] = pwk(o, 16 + q); }
+#pragma omp for collapse(2) schedule(static)
   for (int L = 1; L <= w.ncl; L++)
     for (int M = 1; M <= w.ncm; M++) {
       long b = pencil_base(o, d, L, M), s = w.s;
@@ -709,11 +759,13 @@ static void remap(mcfo *o, int d, double dt) {
         for (int q = 0; q < 15; q++) Q[q][c] = (Q[q][c] * dxl[i] + F[q][i] - F[q][i - 1]) / (xe[i] - xe[i - 1]);
       }
     }
+  }
 }
 
 static void mhd_sweep(mcfo *o, int d, double dt) {
   /* MHD (M3:1264-1324) */
   int n = o->n[d];
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = -1; k <= o->n[2] + 2; k++)
     for (int j = -1; j <= o->n[1] + 2; j++)
       for (int i = -1; i <= o->n[0] + 2; i++) {
@@ -726,6 +778,7 @@ static void mhd_sweep(mcfo *o, int d, double dt) {
   remap(o, d, dt);
   int iBcc = d + 1;
   double rfloor = 1.27 * o->rmin;
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= o->n[2]; k++)
     for (int j = 1; j <= o->n[1]; j++)
       for (int i = 1; i <= o->n[0]; i++) {
@@ -757,16 +810,19 @@ static void cc1(mcfo *o) {
   fillU(o, 2, 2, 1, ly, 3);
   fillU(o, 4, 2, 1, lz, 3);
   int nx = o->n[0], ny = o->n[1], nz = o->n[2];
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx + 1; i++)
         o->vfc[0][ix3(o, i, j, k)] = (dxg(o, 0, i - 1) * U(2)[ix3(o, i, j, k)] + dxg(o, 0, i) * U(2)[ix3(o, i - 1, j, k)]) /
                                      (dxg(o, 0, i - 1) + dxg(o, 0, i));
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny + 1; j++)
       for (int i = 1; i <= nx; i++)
         o->vfc[1][ix3(o, i, j, k)] = (dxg(o, 1, j - 1) * U(3)[ix3(o, i, j, k)] + dxg(o, 1, j) * U(3)[ix3(o, i, j - 1, k)]) /
                                      (dxg(o, 1, j - 1) + dxg(o, 1, j));
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz + 1; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++)
@@ -797,9 +853,12 @@ static void cct(mcfo *o, int orient, double dt) {
   int n1 = o->n[d1], n2 = o->n[d2], n3 = o->n[d3];
   double *Vx = o->vfc[d1], *Vy = o->vfc[d2], *Bx = U(6 + d1), *By = U(6 + d2);
   double *dQ1 = o->dq[0], *dQ2 = o->dq[1], *dQ3 = o->dq[2], *dQ4 = o->dq[3];
-  double *grd = pwk(o, 0), *grd2 = pwk(o, 1);
   double facm = o->p.facdep * 0.5;
+#pragma omp parallel num_threads(o->nthr)
+  {
+  double *grd = pwk(o, 0), *grd2 = pwk(o, 1);
   /* slopes along d1 of Vfc_d2 and B_d2 (dQ 2, 4), at d2 = 1..n2+1, d3 = 1..n3 */
+#pragma omp for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2 + 1; j++) {
       long base = lix(o, orient, 0, j, k);
@@ -812,6 +871,7 @@ static void cct(mcfo *o, int orient, double dt) {
       }
     }
   /* slopes along d2 of Vfc_d1 and B_d1 (dQ 1, 3), at d1 = 1..n1+1, d3 = 1..n3 */
+#pragma omp for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int i = 1; i <= n1 + 1; i++) {
       long base = lix(o, orient, i, 0, k);
@@ -823,8 +883,10 @@ static void cct(mcfo *o, int orient, double dt) {
         dQ3[c] = grd2[j];
       }
     }
+  }
   const double *dxe = o->gdx[d1] + 1, *dye = o->gdx[d2] + 1;
   double *emf = o->emf[(d1 + 2) % 3]; /* Ez for orient 0, Ex for 1, Ey for 2 */
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2 + 1; j++)
       for (int i = 1; i <= n1 + 1; i++) {
@@ -900,18 +962,21 @@ static void cc2(mcfo *o, int orient) {
   if (orient == 0) {
     int l1[1] = {1};
     fillU(o, 7, 2, 1, l1, 1);
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny + 1; j++)
         for (int i = 1; i <= nx + 1; i++)
           o->dnc[ix3(o, i, j, k)] = 0.25 * (r[ix3(o, i, j, k)] + r[ix3(o, i - 1, j, k)] + r[ix3(o, i, j - 1, k)] +
                                             r[ix3(o, i - 1, j - 1, k)]);
   } else if (orient == 1) {
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
     for (int k = 1; k <= nz + 1; k++)
       for (int j = 1; j <= ny + 1; j++)
         for (int i = 1; i <= nx; i++)
           o->dnc[ix3(o, i, j, k)] = 0.25 * (r[ix3(o, i, j, k)] + r[ix3(o, i, j - 1, k)] + r[ix3(o, i, j, k - 1)] +
                                             r[ix3(o, i, j - 1, k - 1)]);
   } else {
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
     for (int k = 1; k <= nz + 1; k++)
       for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx + 1; i++)
@@ -923,6 +988,7 @@ static void cc2(mcfo *o, int orient) {
 static void cc3(mcfo *o, double dt) {
   int nx = o->n[0], ny = o->n[1], nz = o->n[2];
   double *Ex = o->emf[0], *Ey = o->emf[1], *Ez = o->emf[2];
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx + 1; i++) {
@@ -930,6 +996,7 @@ static void cc3(mcfo *o, double dt) {
         U(6)[c] = U(6)[c] + dt / dxg(o, 1, j) * (Ez[ix3(o, i, j + 1, k)] - Ez[c]) -
                   dt / dxg(o, 2, k) * (Ey[ix3(o, i, j, k + 1)] - Ey[c]);
       }
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny + 1; j++)
       for (int i = 1; i <= nx; i++) {
@@ -937,6 +1004,7 @@ static void cc3(mcfo *o, double dt) {
         U(7)[c] = U(7)[c] + dt / dxg(o, 2, k) * (Ex[ix3(o, i, j, k + 1)] - Ex[c]) -
                   dt / dxg(o, 0, i) * (Ez[ix3(o, i + 1, j, k)] - Ez[c]);
       }
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz + 1; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -948,6 +1016,7 @@ static void cc3(mcfo *o, double dt) {
 
 void mcfo_cc4(mcfo *o) {
   int nx = o->n[0], ny = o->n[1], nz = o->n[2];
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -975,12 +1044,14 @@ void mcfo_ct(mcfo *o, double dt) {
 
 static void temp_and_flux(mcfo *o) {
   int nx = o->n[0], ny = o->n[1], nz = o->n[2];
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 0; k <= nz + 1; k++)
     for (int j = 0; j <= ny + 1; j++)
       for (int i = 0; i <= nx + 1; i++) {
         long c = ix3(o, i, j, k);
         o->Tn[c] = U(5)[c] / (KB * (SPC(SP)[c] + SPC(SH)[c] + SPC(SH2)[c] + SPC(SHE)[c] + SPC(SHEP)[c]));
       }
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 0; k <= nz; k++)
     for (int j = 0; j <= ny; j++)
       for (int i = 0; i <= nx; i++) {
@@ -1005,6 +1076,7 @@ void mcfo_source(mcfo *o, double dt) {
   fillU(o, 7, 1, 1, l15, 2);
   fill_ot(o, 7, 1, 5);
   temp_and_flux(o);
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
   for (int k = 0; k <= nz + 1; k++)
     for (int j = 0; j <= ny + 1; j++)
       for (int i = 0; i <= nx + 1; i++) o->Pn[ix3(o, i, j, k)] = U(5)[ix3(o, i, j, k)];
@@ -1013,6 +1085,7 @@ void mcfo_source(mcfo *o, double dt) {
       fillU(o, 7, 1, 1, l5, 1);
       temp_and_flux(o);
     }
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
@@ -1049,9 +1122,14 @@ void mcfo_source(mcfo *o, double dt) {
   }
 }
 
+/* The minima of Couran and Stblty are formed per z plane (in parallel) and
+ * then over the planes in order; min is exact, so this equals the serial
+ * running minimum for any finite values. */
 double mcfo_couran(mcfo *o) {
-  double tCFL = o->p.tfinal;
-  for (int k = 1; k <= o->n[2]; k++)
+  double *part = (double *)malloc(sizeof(double) * (size_t)o->n[2]);
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
+  for (int k = 1; k <= o->n[2]; k++) {
+    double tCFL = o->p.tfinal;
     for (int j = 1; j <= o->n[1]; j++)
       for (int i = 1; i <= o->n[0]; i++) {
         long c = ix3(o, i, j, k);
@@ -1062,12 +1140,19 @@ double mcfo_couran(mcfo *o) {
         t = dxg(o, 1, j) / (sqrt(c2) + fabs(U(3)[c])); tCFL = DMIN(tCFL, t);
         t = dxg(o, 2, k) / (sqrt(c2) + fabs(U(4)[c])); tCFL = DMIN(tCFL, t);
       }
+    part[k - 1] = tCFL;
+  }
+  double tCFL = o->p.tfinal;
+  for (int k = 0; k < o->n[2]; k++) tCFL = DMIN(tCFL, part[k]);
+  free(part);
   return tCFL;
 }
 
 double mcfo_stblty(mcfo *o) {
-  double tLMT = o->p.tfinal;
-  for (int k = 1; k <= o->n[2]; k++)
+  double *part = (double *)malloc(sizeof(double) * (size_t)o->n[2]);
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
+  for (int k = 1; k <= o->n[2]; k++) {
+    double tLMT = o->p.tfinal;
     for (int j = 1; j <= o->n[1]; j++)
       for (int i = 1; i <= o->n[0]; i++) {
         long c = ix3(o, i, j, k);
@@ -1084,6 +1169,11 @@ double mcfo_stblty(mcfo *o) {
         tLMT = DMIN(tLMT, tauC);
         tLMT = DMIN(tLMT, alpha);
       }
+    part[k - 1] = tLMT;
+  }
+  double tLMT = o->p.tfinal;
+  for (int k = 0; k < o->n[2]; k++) tLMT = DMIN(tLMT, part[k]);
+  free(part);
   return tLMT;
 }
 

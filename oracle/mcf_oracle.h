@@ -55,6 +55,10 @@ typedef struct mcfo mcfo;
 void    mcfo_default_params(mcfo_params *p);
 mcfo   *mcfo_create(const mcfo_params *p);
 void    mcfo_destroy(mcfo *o);
+/* Threads this instance splits its loops over (default: OpenMP's maximum;
+ * nthreads < 1 restores it).  Results do not depend on it.  Returns the
+ * count, or -1 if the work arrays cannot be allocated. */
+int     mcfo_set_threads(mcfo *o, int nthreads);
 double *mcfo_field(mcfo *o, int id);
 double *mcfo_grid(mcfo *o, int axis, int spacing); /* x/dx arrays, index -1..n+2 */
 void    mcfo_get_time(const mcfo *o, double *time, int *ifevo);

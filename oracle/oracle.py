@@ -50,6 +50,8 @@ def lib():
         L.mcfo_create.restype = P
         L.mcfo_create.argtypes = [C.POINTER(Params)]
         L.mcfo_destroy.argtypes = [P]
+        L.mcfo_set_threads.argtypes = [P, C.c_int]
+        L.mcfo_set_threads.restype = C.c_int
         L.mcfo_field.restype = pd
         L.mcfo_field.argtypes = [P, C.c_int]
         L.mcfo_grid.restype = pd
@@ -95,11 +97,15 @@ class Oracle:
     """One single-block CPU instance of the reference step."""
 
     def __init__(self, p, state: np.ndarray | None = None, time: float = 0.0, ifevo: int = 1,
-                 nsplit=(2, 2, 2)):
+                 nsplit=(2, 2, 2), threads: int = 0):
         self.p = p
         self.n = p.n
         self._params = to_params(p, nsplit)
         self.h = lib().mcfo_create(C.byref(self._params))
+        # threads < 1: OpenMP's default (OMP_NUM_THREADS or all cores)
+        self.threads = lib().mcfo_set_threads(self.h, threads)
+        if self.threads < 1:
+            raise MemoryError("oracle work arrays")
         nx, ny, nz = self.n
         self.shape = (nz + 4, ny + 4, nx + 4)
         if state is not None:

@@ -45,3 +45,20 @@ def test_block_split_matters():
     o.run(6)
     errs = rel_linf(interior(o.state(), p), interior(expand(arr["out6"], p), p))
     assert worst(errs) > 1e-12
+
+
+@pytest.mark.parametrize("name,steps", [("inflow_48x8x8", 6), ("periodic_24x8x8", 6)])
+def test_oracle_thread_count_independent(name, steps):
+    """The oracle splits its pencil and plane loops over OpenMP threads: one
+    thread and four threads must give the fixture bit for bit (each cell is
+    computed by one thread with the serial expressions; the dt minima are
+    exact)."""
+    p, nsplit, arr = load_case(name)
+    outs = []
+    for th in (1, 4):
+        o = oracle.Oracle(p, expand(arr["input"], p), nsplit=nsplit, threads=th)
+        assert o.threads == th
+        dts, _ = o.run(steps)
+        np.testing.assert_array_equal(dts, arr[f"dts{steps}"])
+        outs.append(o.state())
+    np.testing.assert_array_equal(outs[0], outs[1])
