@@ -14,13 +14,19 @@ The headline ``value`` is the mixed-precision path (FP32 reconstruction,
 Riemann and EMF; FP64 state, pressure recovery and face B) with the MLP
 cooling surrogate; the plain FP64 path is timed on the same state in the
 same run (``fp64``).  The timed steps carry no profiling events; ten more
-steps with every launch bracketed by HIP events give the per-kernel table
-and the roofline of the dominant kernel; two child ``rocprofv3 --pmc`` runs
-(FETCH_SIZE, then WRITE_SIZE, calibrated by a plain copy kernel) give its
-HBM traffic per launch.  The CPU restatement (oracle/) is timed on a bounded
-sample of the same problem as the CPU baseline.
+steps with every launch bracketed by HIP events give the per-kernel and
+per-stage tables.  ``roofline`` is the stage (dt, SOURCE call, x/y/z sweep,
+CT) that takes the most time per step: its compulsory HBM bytes (each
+persistent array it must read or write, once per interior cell: 30 doubles
+per cell for a sweep, 14 for CT, 8 for a SOURCE call, 13 for dt) over its
+measured time; two child ``rocprofv3 --pmc`` runs (FETCH_SIZE, then
+WRITE_SIZE, calibrated by a plain copy kernel) give the HBM bytes its
+kernels moved.  ``stiff`` times the same problem at 1e4 times the density,
+where SOURCE's backward-Euler branch (IMC or the surrogate) does the work.
+The CPU restatement (oracle/, OpenMP on the host's cores) is timed on a
+bounded sample of the same problem as the CPU baseline.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pmc]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pmc] [--no-stiff]
 """
 import argparse
 import csv
@@ -33,11 +39,12 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
-ALG_BYTES_PER_ZONE = 920.0      # whole FP64 step, 5 active species (compulsory traffic)
 COPY_WORDS = 1 << 27            # calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -74,8 +81,9 @@ def surrogate_weights():
 def timed(d, steps, warmup, profile_steps=10):
     """Warm-up, then the timed steps (no profiling events in them), then
     profile_steps more steps with every launch bracketed by HIP events for
-    the per-kernel table: returns (seconds, dts, {kernel: (launches, ms,
-    bytes)}, profiled steps)."""
+    the per-kernel and per-stage tables: returns (seconds, dts, {kernel:
+    (launches, ms, touched bytes)}, {stage: (runs, ms, compulsory bytes)},
+    profiled steps)."""
     d.run(warmup, tsave=1e30)
     d.sync()
     t0 = time.perf_counter()
@@ -83,25 +91,110 @@ def timed(d, steps, warmup, profile_steps=10):
     d.sync()
     el = time.perf_counter() - t0
     assert len(dts) == steps
-    rep, np_ = {}, 0
+    rep, stg, np_ = {}, {}, 0
     if profile_steps > 0:
         d.profile(True)
         np_ = len(d.run(profile_steps, tsave=1e30))
         d.sync()
         rep = d.profile_report()
+        stg = d.profile_stages()
         d.profile(False)
-    return el, dts, rep, np_
+    return el, dts, rep, stg, np_
 
 
-def report_kernels(tag, rep, steps):
+def report_kernels(tag, rep, stg, steps):
     tot = sum(v[1] for v in rep.values())
     log(f"[bench] {tag}: kernel time {tot / steps:.3f} ms/step")
-    for k, (cnt, ms, by) in sorted(rep.items(), key=lambda kv: -kv[1][1])[:14]:
+    for k, (runs, ms, by) in sorted(stg.items(), key=lambda kv: -kv[1][1]):
+        if runs:
+            log(f"  stage {k:10s} runs/step {runs / steps:4.1f}  ms/run {ms / runs:8.4f}  ms/step {ms / steps:8.4f}"
+                f"  {by / (ms * 1e-3) / 1e9 if ms > 0 else 0:8.1f} GB/s compulsory")
+    for k, (cnt, ms, by) in sorted(rep.items(), key=lambda kv: -kv[1][1])[:16]:
         log(f"  {k:18s} launches {cnt:6d}  avg {1e3 * ms / cnt:9.2f} us  total/step {ms / steps:8.3f} ms"
-            f"  {by / (ms * 1e-3) / 1e9:8.1f} GB/s")
+            f"  {by / (ms * 1e-3) / 1e9:8.1f} GB/s touched")
+
+
+def kernel_stage(name):
+    """The step stage a kernel belongs to (None for the ghost fills, which
+    several stages launch).  Sweep kernels carry their axis as the first
+    template argument (k_rhsmoc_col<1> is the y sweep's)."""
+    if name in ("k_sweep_x", "k_rhsmoc_x", "k_remap_x") or name.endswith("<0>"):
+        return "sweep_x"
+    if name.endswith("<1>"):
+        return "sweep_y"
+    if name.endswith("<2>"):
+        return "sweep_z"
+    if name in ("k_cc1", "k_cct3", "k_cc3", "k_cc4"):
+        return "ct"
+    if name in ("k_src_col", "k_shell_copy", "k_src_stiff", "k_src_imc"):
+        return "source"
+    if name in ("k_dtmin", "k_dtfinal"):
+        return "dt"
+    if name in ("k_dissip_div", "k_dissip_upd"):
+        return "dissip"
+    return None
+
+
+def stage_rows(stg, rep, steps, traffic=None):
+    """Per-stage table: compulsory bytes (each persistent array the stage
+    must read or write, once per interior cell) over the stage's measured
+    time, and, where the counter runs measured them, the HBM bytes its own
+    kernels moved per run (ghost fills excluded)."""
+    rows = {}
+    for name, (runs, ms, by) in stg.items():
+        if runs == 0 or by == 0:
+            continue
+        ach = by / (ms * 1e-3) / 1e9
+        kern = {k: round(1e3 * v[1] / v[0], 2) for k, v in rep.items() if kernel_stage(k) == name}
+        row = {"runs_per_step": runs / steps, "ms_per_run": ms / runs, "ms_per_step": ms / steps,
+               "compulsory_bytes_per_run": by / runs, "achieved_GBs": round(ach, 1),
+               "frac": round(ach / HBM_PEAK_GBS, 4), "kernels_avg_us": kern}
+        if traffic is not None and traffic[0] is not None:
+            per, _ = traffic
+            tb = [per[k]["bytes_per_step"] for k in kern if k in per]
+            if tb:
+                row["traffic_bytes_per_run"] = sum(tb) / (runs / steps)
+        rows[name] = row
+    return rows
+
+
+def roofline(rows, traffic):
+    """The roofline of the stage that takes the most time per step."""
+    name, r = max(rows.items(), key=lambda kv: kv[1]["ms_per_step"])
+    out = {"bound": "hbm", "stage": name, "kernels_avg_us": r["kernels_avg_us"], "achieved": r["achieved_GBs"],
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac"],
+           "traffic": r.get("traffic_bytes_per_run"),
+           "alg_bytes_per_run": r["compulsory_bytes_per_run"], "ms_per_run": r["ms_per_run"],
+           "accounting": "compulsory bytes of the stage: a sweep reads rho, v(3), p, 8 species, Bcc(3), face B_n and "
+                         "writes rho, v(3), p, 8 species (30 doubles per cell); CT 8 + 6; SOURCE 7 + 1 per call; "
+                         "dt 13"}
+    if traffic is not None and traffic[0] is None:
+        out["traffic_note"] = traffic[1]
+    elif traffic is not None:
+        out["traffic_detail"] = traffic[1]
+    return out
+
+
+def kernel_table(rep, traffic, steps, top=10):
+    """Per-kernel time, bytes of the arrays each kernel touches (a
+    diagnostic: it counts the sweep's own intermediates) and, where the
+    counter runs measured it, HBM traffic per launch."""
+    per = traffic[0] if traffic is not None else None
+    rows = []
+    for name, (cnt, ms, by) in sorted(rep.items(), key=lambda kv: -kv[1][1])[:top]:
+        row = {"kernel": name, "stage": kernel_stage(name), "launches_per_step": cnt / steps,
+               "avg_launch_us": round(1e3 * ms / cnt, 2), "ms_per_step": round(ms / steps, 4),
+               "touched_bytes_per_launch": by / cnt, "touched_GBs": round(by / (ms * 1e-3) / 1e9, 1)}
+        if per is not None and name in per:
+            row["traffic_bytes_per_launch"] = per[name]["bytes"]
+            row["traffic_GBs"] = round(per[name]["bytes"] / (ms / cnt * 1e-3) / 1e9, 1)
+        rows.append(row)
+    return rows
 
 
 def cpu_baseline(n: int, steps: int):
+    """The oracle (C restatement, OpenMP over all the threads OpenMP is given)
+    on `steps` FP64 steps of the same problem."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from code_tohoku_amd import ics
@@ -109,33 +202,88 @@ def cpu_baseline(n: int, steps: int):
     p = ics.converging_flow_input(n)
     st = ics.converging_flow_state(p)
     o = oracle.Oracle(p, st, nsplit=(max(1, n // 64),) * 3)
+    del st
     t0 = time.perf_counter()
     dts, _ = o.run(steps)
     el = time.perf_counter() - t0
-    return {"value": n ** 3 * len(dts) / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
-            "sample": f"CPU restatement (oracle/, C, 1 thread), {len(dts)} FP64 steps of the same converging-flow "
-                      f"problem at {n}^3, {el:.1f} s"}
+    return {"value": n ** 3 * len(dts) / el, "unit": "zone-updates/s", "cores": o.threads, "kind": "port",
+            "sample": f"CPU restatement (oracle/, C + OpenMP, {o.threads} threads), {len(dts)} FP64 steps of the "
+                      f"same converging-flow problem at {n}^3, {el:.1f} s"}
+
+
+def stiff_state(n):
+    """The converging flow at 1e4 times the density and pressure (n = 1e4
+    cm^-3, T = 6000 K): the cooling time is below Stblty's dt floor, so SOURCE
+    takes its backward-Euler branch in most cells."""
+    p, st = build_state(n)
+    st[0] *= 1.0e4
+    st[4] *= 1.0e4
+    st[8:16] *= 1.0e4
+    return p, st
+
+
+def stiff_compare(n, steps, warmup, w):
+    """ms/step of the stiff problem: FP64 + IMC, mixed + IMC, mixed +
+    surrogate, with the backward-Euler cell counts of the timed steps."""
+    p, st = stiff_state(n)
+    nsplit = (max(1, n // 64),) * 3
+    out = {"workload": f"converging_flow_{n}^3 at 1e4 x density (n = 1e4 cm^-3, T = 6000 K)", "steps": steps}
+    for tag, mixed, sur in (("fp64_imc", False, False), ("mixed_imc", True, False), ("mixed_surrogate", True, True)):
+        d = make_domain(p, st, nsplit, mixed, sur, w)
+        d.run(warmup, tsave=1e30)
+        d.sync()
+        d.cool_stats(reset=True)
+        t0 = time.perf_counter()
+        dts = d.run(steps, tsave=1e30)
+        d.sync()
+        el = time.perf_counter() - t0
+        nst, nfb = d.cool_stats()
+        d.close()
+        out[tag] = {"ms_per_step": 1e3 * el / steps, "zone_updates_per_s": n ** 3 * steps / el,
+                    "backward_euler_cells": nst, "imc_fallbacks": nfb,
+                    "fallback_fraction": (nfb / nst) if (sur and nst) else None, "dt_mean": float(np.mean(dts))}
+        log(f"[bench] stiff {tag}: {out[tag]}")
+    out["surrogate_speedup_vs_mixed_imc"] = out["mixed_imc"]["ms_per_step"] / out["mixed_surrogate"]["ms_per_step"]
+    out["mixed_surrogate_speedup_vs_fp64"] = out["fp64_imc"]["ms_per_step"] / out["mixed_surrogate"]["ms_per_step"]
+    return out
 
 
 # ---------------------------------------------------------------------------
 # HBM traffic of the dominant kernel: child rocprofv3 --pmc runs
 # ---------------------------------------------------------------------------
+PMC_STEPS = 2
+
+
 def pmc_child(args):
-    """Run under rocprofv3 --pmc: one warm-up and one step of the headline
+    """Run under rocprofv3 --pmc: upload, PMC_STEPS steps of the headline
     path, then the calibration copy."""
     p, st = build_state(args.n)
     w, _ = surrogate_weights()
     d = make_domain(p, st, (max(1, args.n // 64),) * 3, True, True, w)
     del st
-    d.run(2, tsave=1e30)
+    d.run(PMC_STEPS, tsave=1e30)
     d.sync()
     d.copy_probe(COPY_WORDS, 1)
     d.close()
 
 
+def short_name(kernel_name):
+    """rocprofv3's kernel name -> the library's profile name: namespace and
+    argument list dropped, a leading integer template argument (the sweep
+    axis) kept: "void mcf::k_rhsmoc_col<1, float>(...)" -> "k_rhsmoc_col<1>"."""
+    base = kernel_name.split("(")[0].replace("void ", "").replace("mcf::", "").strip()
+    if "<" in base:
+        head, args_ = base.split("<", 1)
+        first = args_.rstrip(">").split(",")[0].strip()
+        base = f"{head}<{first}>" if first.isdigit() else head
+    return base
+
+
 def pmc_traffic(args):
-    """{kernel: HBM bytes per launch} of the headline path from FETCH_SIZE and
-    WRITE_SIZE (one rocprofv3 run each, calibrated by k_copy), or a reason."""
+    """{kernel: HBM bytes} of the headline path's steps from FETCH_SIZE and
+    WRITE_SIZE (one rocprofv3 run each, calibrated by k_copy), or a reason.
+    Only dispatches from the first dt reduction on are counted (the upload's
+    CC(4) is not part of a step)."""
     prof = shutil.which("rocprofv3")
     if prof is None:
         return None, "rocprofv3 not found"
@@ -151,20 +299,26 @@ def pmc_traffic(args):
             files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
             if r.returncode != 0 and not files:
                 return None, f"rocprofv3 pass {counter} failed (rc {r.returncode})"
-            per = {}
+            disp = {}
             for f in files:
                 for row in csv.DictReader(open(f)):
                     if row.get("Counter_Name") != counter:
                         continue
-                    base = row["Kernel_Name"].split("(")[0].replace("void ", "").replace("mcf::", "").split("<")[0]
-                    d = per.setdefault(base, {})
-                    d[row["Dispatch_Id"]] = d.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+                    did = int(row["Dispatch_Id"])
+                    name, v = disp.get(did, (short_name(row["Kernel_Name"]), 0.0))
+                    disp[did] = (name, v + float(row["Counter_Value"]))
+            first = min((i for i, (nm, _) in disp.items() if nm == "k_dtmin"), default=None)
+            per = {}
+            for did, (nm, v) in disp.items():
+                if nm != "k_copy" and (first is None or did < first):
+                    continue
+                per.setdefault(nm, []).append(v)
             res[counter] = per
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     try:
-        copy_f = list(res["FETCH_SIZE"]["k_copy"].values())[-1]
-        copy_w = list(res["WRITE_SIZE"]["k_copy"].values())[-1]
+        copy_f = res["FETCH_SIZE"]["k_copy"][-1]
+        copy_w = res["WRITE_SIZE"]["k_copy"][-1]
     except (KeyError, IndexError):
         return None, "counters missing for the calibration copy"
     nbytes = 8.0 * COPY_WORDS
@@ -174,10 +328,11 @@ def pmc_traffic(args):
         kw = res["WRITE_SIZE"].get(k)
         if not kw or k == "k_copy":
             continue
-        fetch = sum(kf.values()) / len(kf) * 1024.0 * sf
-        write = sum(kw.values()) / len(kw) * 1024.0 * sw
-        out[k] = {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "launches": len(kf)}
-    return out, {"fetch_scale": sf, "write_scale": sw,
+        fetch = sum(kf) / len(kf) * 1024.0 * sf
+        write = sum(kw) / len(kw) * 1024.0 * sw
+        out[k] = {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "launches": len(kf),
+                  "bytes_per_step": (fetch + write) * len(kf) / PMC_STEPS}
+    return out, {"fetch_scale": sf, "write_scale": sw, "steps": PMC_STEPS,
                  "calibration": f"k_copy, {int(nbytes)} B read and {int(nbytes)} B written"}
 
 
@@ -187,10 +342,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pmc", action="store_true")
+    ap.add_argument("--no-stiff", action="store_true")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -214,9 +370,9 @@ def main():
     d = make_domain(p, st, nsplit, True, True, w)
     log(f"[bench] mixed+surrogate: warmup {args.warmup}, timed {args.steps} steps")
     d.cool_stats(reset=True)
-    el, dts, rep, nprof = timed(d, args.steps, args.warmup)
+    el, dts, rep, stg, nprof = timed(d, args.steps, args.warmup)
     nstiff, nfall = d.cool_stats()
-    report_kernels("mixed+surrogate", rep, nprof)
+    report_kernels("mixed+surrogate", rep, stg, nprof)
     from code_tohoku_amd import surrogate
     solve = surrogate.evaluate(d, n=1 << 20)
     log(f"[bench] surrogate on 2^20 stiff cells: {solve}")
@@ -226,8 +382,8 @@ def main():
     f = make_domain(p, st, nsplit, False, False, None)
     del st
     log(f"[bench] fp64: warmup {args.warmup}, timed {args.steps} steps")
-    el64, dts64, rep64, nprof64 = timed(f, args.steps, args.warmup)
-    report_kernels("fp64", rep64, nprof64)
+    el64, dts64, rep64, stg64, nprof64 = timed(f, args.steps, args.warmup)
+    report_kernels("fp64", rep64, stg64, nprof64)
     f.close()
 
     ms_step = 1e3 * el / args.steps
@@ -236,40 +392,10 @@ def main():
     log(f"[bench] mixed+surrogate {ms_step:.3f} ms/step {value:.4e} zone-updates/s; "
         f"fp64 {1e3 * el64 / args.steps:.3f} ms/step {value64:.4e}; ratio {value / value64:.3f}")
 
-    def roof(rep, traffic):
-        name, (cnt, ms, by) = max(rep.items(), key=lambda kv: kv[1][1])
-        achieved = by / (ms * 1e-3) / 1e9
-        r = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(1e3 * ms / cnt, 2),
-             "bytes_per_launch": by / cnt}
-        if traffic is not None:
-            per, info = traffic
-            if per is not None and name in per:
-                t = per[name]
-                r["traffic"] = t["bytes"]
-                r["traffic_GBs"] = round(t["bytes"] / (ms / cnt * 1e-3) / 1e9, 1)
-                r["traffic_detail"] = {"fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"], **info}
-            elif per is None:
-                r["traffic_note"] = info
-        return r
-
-    def kernel_table(rep, traffic, steps, top=8):
-        """Per-kernel achieved bandwidth (algorithmic bytes) and, where the
-        counter runs measured it, HBM traffic per launch."""
-        per = traffic[0] if traffic is not None else None
-        rows = []
-        for name, (cnt, ms, by) in sorted(rep.items(), key=lambda kv: -kv[1][1])[:top]:
-            row = {"kernel": name, "launches_per_step": cnt / steps, "avg_launch_us": round(1e3 * ms / cnt, 2),
-                   "ms_per_step": round(ms / steps, 4), "alg_bytes_per_launch": by / cnt,
-                   "achieved_GBs": round(by / (ms * 1e-3) / 1e9, 1),
-                   "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-            if per is not None and name in per:
-                row["traffic_bytes_per_launch"] = per[name]["bytes"]
-                row["traffic_GBs"] = round(per[name]["bytes"] / (ms / cnt * 1e-3) / 1e9, 1)
-            rows.append(row)
-        return rows
-
-    step_gbs = ALG_BYTES_PER_ZONE * n ** 3 / (ms_step * 1e-3) / 1e9
+    rows = stage_rows(stg, rep, nprof, pmc)
+    rows64 = stage_rows(stg64, rep64, nprof64)
+    step_bytes = sum(r["compulsory_bytes_per_run"] * r["runs_per_step"] for r in rows.values())
+    step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
     out = {"metric": "MHD zone-updates/s at 256^3 (mixed+surrogate vs FP64) + % HBM roofline",
            "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -277,20 +403,24 @@ def main():
            "config": {"workload": f"converging_flow_{n}^3_ki_cooling", "grid": [n, n, n],
                       "block_split": list(nsplit), "precision": "mixed", "cooling": "surrogate",
                       "surrogate_weights": wsrc},
-           "roofline": roof(rep, pmc),
+           "roofline": roofline(rows, pmc),
+           "stages": rows,
            "kernels": kernel_table(rep, pmc, nprof),
            "speedup_vs_fp64": value / value64,
-           "fp64": {"value": value64, "ms_per_step": 1e3 * el64 / args.steps, "roofline": roof(rep64, None),
-                    "kernels": kernel_table(rep64, None, nprof64),
+           "fp64": {"value": value64, "ms_per_step": 1e3 * el64 / args.steps, "roofline": roofline(rows64, None),
+                    "stages": rows64, "kernels": kernel_table(rep64, None, nprof64),
                     "dt_mean": float(sum(dts64) / len(dts64))},
            "surrogate": {"backward_euler_cells": nstiff, "imc_fallbacks": nfall,
                          "fallback_fraction": (nfall / nstiff) if nstiff else None,
-                         "note": "cells of the run's steps (warm-up, timed, profiled) that took SOURCE's backward-Euler branch",
+                         "note": "cells of the run's steps (warm-up, timed, profiled) that took SOURCE's "
+                                 "backward-Euler branch; none at n = 1 (Stblty keeps dt below 0.2 t_cool)",
                          "stiff_solve": solve},
-           "step_roofline": {"alg_bytes_per_zone": ALG_BYTES_PER_ZONE, "achieved": round(step_gbs, 1),
+           "step_roofline": {"compulsory_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
                              "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
            "dt_mean": float(sum(dts) / len(dts)),
            "copy_probe": {"bytes": 16 * (1 << 28), "ms": cms, "GBs": round(16 * (1 << 28) / (cms * 1e-3) / 1e9, 1)}}
+    if not args.no_stiff:
+        out["stiff"] = stiff_compare(n, 10, 2, w)
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle {args.cpu_steps} steps at {args.cpu_n}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)

@@ -55,6 +55,7 @@ struct mcf_ctx {
   double bbrv[11][3][3]{}; // BBRV(k, mode, side), 1-based
   double bbrv_cm[8]{};
   double dtf = 0, pmin = 0, pmax = 0, rmin = 0, spmin[8]{};
+  float *dtfd = nullptr; // DTF field on the device (mcf_set_inflow_density), null = uniform dtf
   int nsp[3]{1, 1, 1};
   double time = 0.0;
   int ifevo = 1;
@@ -73,9 +74,24 @@ struct mcf_ctx {
   bool mlp_ready = false;
   int src_pair = 0; // which of the two SOURCE queue-counter pairs the next stage counts in
   long adam_t = 0;
-  struct Rec { const char *name; hipEvent_t a, b; double bytes; };
+  struct Rec { const char *name; const char *stage; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
+  // stage of the step the launches belong to (mcf_profile_stages) and how
+  // many times each stage ran while profiling
+  const char *stage = "other";
+  std::map<std::string, int> stage_runs;
+};
+
+// Marks the launches made while it lives as one run of a stage of the step.
+struct StageScope {
+  mcf_ctx *x;
+  const char *prev;
+  StageScope(mcf_ctx *c, const char *name) : x(c), prev(c->stage) {
+    x->stage = name;
+    if (x->prof && std::strcmp(prev, name) != 0) x->stage_runs[name] += 1;
+  }
+  ~StageScope() { x->stage = prev; }
 };
 
 static hipEvent_t prof_event(mcf_ctx *x) {
@@ -193,7 +209,7 @@ static long boxsize(const Box &b) { return (long)b.cnt[0] * b.cnt[1] * b.cnt[2];
     HIPCHK(hipGetLastError());                                                                     \
     if (x->prof) {                                                                                 \
       HIPCHK(hipEventRecord(eb_, x->stream));                                                      \
-      x->recs.push_back({name, ea_, eb_, (double)(bytes)});                                        \
+      x->recs.push_back({name, x->stage, ea_, eb_, (double)(bytes)});                              \
     }                                                                                              \
   } while (0)
 #define KLAUNCH_G(kern, grid, block, bytes, ...) KLAUNCH_N(#kern, kern, grid, block, bytes, __VA_ARGS__)
@@ -246,6 +262,19 @@ static int launch_xfill(mcf_ctx *x, XFill &X, int nol) {
   X.bcc1 = x->bcc[0];
   X.blg1 = x->blg[0];
   X.blg2 = x->blg[1];
+  if (x->dtfd) {
+    // the boundary ranks' dx(0) and L = dx(0)*Ncellx*NSPLTx (M3:821-822, 857-858)
+    const int cx = x->g.n[0] / x->nsp[0], nsx = x->nsp[0];
+    X.dtf.f = x->dtfd;
+    X.dtf.xe = x->gx[0];
+    X.dtf.t = x->time;
+    for (int s = 0; s < 2; ++s) {
+      const double dx0 = x->hdx[0][(s == 0 ? 0 : (nsx - 1) * cx) + 1];
+      X.dtf.dx0[s] = dx0;
+      X.dtf.L[s] = dx0 * (double)cx * (double)nsx;
+      X.dtf.v[s] = x->bbrv[2][1][s + 1];
+    }
+  }
   long nt = (long)(x->g.n[1] + 4) * (x->g.n[2] + 4) * 2 * nol;
   KLAUNCH(k_fill_xrule, nt, 2 * X.nf, X, x->g, nol);
   return MCF_OK;
@@ -309,7 +338,7 @@ static void planU(mcf_ctx *x, FillPlan &P, int dmask, int mode, const int *list,
       for (int s = 0; s < 2; ++s) {
         int side = s + 1;
         if (bc[s] != 4) { X.rule[u][s] = XR_PERIODIC; continue; }
-        if (K == 1) { X.rule[u][s] = XR_CONST; X.val[u][s] = x->dtf; }
+        if (K == 1) { X.rule[u][s] = x->dtfd ? XR_DTF : XR_CONST; X.val[u][s] = x->dtf; }
         else if (K == 6) X.rule[u][s] = XR_FACE;
         else if (K == 5 && mode == 2) {
           X.rule[u][s] = XR_ENERGY; X.val[u][s] = x->bbrv[5][1][side]; X.val2[s] = x->bbrv[2][1][side];
@@ -411,24 +440,29 @@ static dim3 sweep_grid(const mcf_ctx *x, int d) {
 // Stage kernel launch: strict FP64 (R = double) or mixed (R = float).
 // RHS and MOC use 9 waves on the x sweep and 10 on y/z (halo positions),
 // REMAP 9 waves everywhere.
+// Profile names carry the sweep axis as the template argument rocprofv3
+// shows first ("k_rhs<1>" for k_rhs<1, double> and k_rhs<1, float>).
 #define SWEEP_LAUNCH(kern, d, wide, narr, ...)                                                      \
   do {                                                                                             \
     dim3 gr_ = sweep_grid(x, d);                                                                   \
     const unsigned bt_ = (d != 0 && (wide)) ? 640u : 576u;                                         \
     const double by_ = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * (narr);                   \
+    static const char *const nm_[3] = {#kern "<0>", #kern "<1>", #kern "<2>"};                     \
     if (x->mixed) {                                                                                \
-      if (d == 0) KLAUNCH_N(#kern, (kern<0, float>), gr_, dim3(bt_), by_, __VA_ARGS__);                   \
-      else if (d == 1) KLAUNCH_N(#kern, (kern<1, float>), gr_, dim3(bt_), by_, __VA_ARGS__);              \
-      else KLAUNCH_N(#kern, (kern<2, float>), gr_, dim3(bt_), by_, __VA_ARGS__);                          \
+      if (d == 0) KLAUNCH_N(nm_[0], (kern<0, float>), gr_, dim3(bt_), by_, __VA_ARGS__);                  \
+      else if (d == 1) KLAUNCH_N(nm_[1], (kern<1, float>), gr_, dim3(bt_), by_, __VA_ARGS__);             \
+      else KLAUNCH_N(nm_[2], (kern<2, float>), gr_, dim3(bt_), by_, __VA_ARGS__);                         \
     } else {                                                                                       \
-      if (d == 0) KLAUNCH_N(#kern, (kern<0, double>), gr_, dim3(bt_), by_, __VA_ARGS__);                  \
-      else if (d == 1) KLAUNCH_N(#kern, (kern<1, double>), gr_, dim3(bt_), by_, __VA_ARGS__);             \
-      else KLAUNCH_N(#kern, (kern<2, double>), gr_, dim3(bt_), by_, __VA_ARGS__);                         \
+      if (d == 0) KLAUNCH_N(nm_[0], (kern<0, double>), gr_, dim3(bt_), by_, __VA_ARGS__);                 \
+      else if (d == 1) KLAUNCH_N(nm_[1], (kern<1, double>), gr_, dim3(bt_), by_, __VA_ARGS__);            \
+      else KLAUNCH_N(nm_[2], (kern<2, double>), gr_, dim3(bt_), by_, __VA_ARGS__);                        \
     }                                                                                              \
   } while (0)
 
 // MHD (M3:1264-1324): RHS -> MOC -> REMAP -> primitives
 static int do_sweep(mcf_ctx *x, int d, double dt) {
+  static const char *const names[3] = {"sweep_x", "sweep_y", "sweep_z"};
+  StageScope scope(x, names[d]);
   SweepIO A = sweep_io(x, d);
   bool dxlag_fill = false;
   if (d == 0 && x->fuse[0]) {
@@ -487,11 +521,11 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     const double cb = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2];
     const double by = cb * 23, byr = cb * 31;
     if (x->mixed) {
-      if (d == 1) KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<1, float>), gr, dim3(256), by, A, x->g, dt);
-      else KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<2, float>), gr, dim3(256), by, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_rhsmoc_col<1>", (k_rhsmoc_col<1, float>), gr, dim3(256), by, A, x->g, dt);
+      else KLAUNCH_N("k_rhsmoc_col<2>", (k_rhsmoc_col<2, float>), gr, dim3(256), by, A, x->g, dt);
     } else {
-      if (d == 1) KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<1, double>), gr, dim3(256), by, A, x->g, dt);
-      else KLAUNCH_N("k_rhsmoc_col", (k_rhsmoc_col<2, double>), gr, dim3(256), by, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_rhsmoc_col<1>", (k_rhsmoc_col<1, double>), gr, dim3(256), by, A, x->g, dt);
+      else KLAUNCH_N("k_rhsmoc_col<2>", (k_rhsmoc_col<2, double>), gr, dim3(256), by, A, x->g, dt);
     }
     for (int f = 0; f < 5; ++f) swapU(x, f);
     swapBlg(x);
@@ -499,11 +533,11 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // fills); it forms the Lagrangian species from the pre-sweep ones
     A = sweep_io(x, d);
     if (x->mixed) {
-      if (d == 1) KLAUNCH_N("k_remap_col", (k_remap_col<1, float>), gr, dim3(256), byr, A, x->g, dt);
-      else KLAUNCH_N("k_remap_col", (k_remap_col<2, float>), gr, dim3(256), byr, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_remap_col<1>", (k_remap_col<1, float>), gr, dim3(256), byr, A, x->g, dt);
+      else KLAUNCH_N("k_remap_col<2>", (k_remap_col<2, float>), gr, dim3(256), byr, A, x->g, dt);
     } else {
-      if (d == 1) KLAUNCH_N("k_remap_col", (k_remap_col<1, double>), gr, dim3(256), byr, A, x->g, dt);
-      else KLAUNCH_N("k_remap_col", (k_remap_col<2, double>), gr, dim3(256), byr, A, x->g, dt);
+      if (d == 1) KLAUNCH_N("k_remap_col<1>", (k_remap_col<1, double>), gr, dim3(256), byr, A, x->g, dt);
+      else KLAUNCH_N("k_remap_col<2>", (k_remap_col<2, double>), gr, dim3(256), byr, A, x->g, dt);
     }
     for (int f = 0; f < 5; ++f) swapU(x, f);
     swapSp(x);
@@ -587,6 +621,7 @@ static int do_cc4(mcf_ctx *x) {
 
 // CT (M3:638-649)
 static int do_ct(mcf_ctx *x, double dt) {
+  StageScope scope(x, "ct");
   CtArgs A = ct_args(x);
   int n[3] = {x->g.n[0], x->g.n[1], x->g.n[2]};
   int lx[3] = {2, 7, 8}, ly[3] = {3, 6, 8}, lz[3] = {4, 6, 7}, l5[1] = {5}, l1[1] = {1};
@@ -617,6 +652,7 @@ static int do_ct(mcf_ctx *x, double dt) {
 // SOURCE (M3:2448-2572)
 static int do_source(mcf_ctx *x, double dt) {
   if (x->cfg.ifthrm != 2) return MCF_OK;
+  StageScope scope(x, "source");
   SrcArgs A{};
   A.rho = x->U[F_RHO];
   for (int s = 0; s < 8; ++s) A.sp[s] = x->sp[s];
@@ -680,6 +716,7 @@ static int do_source(mcf_ctx *x, double dt) {
 }
 
 static int do_dissip(mcf_ctx *x) {
+  StageScope scope(x, "dissip");
   int l[3] = {2, 3, 4};
   CHK(fillU(x, 7, 2, 1, l, 3));
   DisArgs A{};
@@ -800,6 +837,7 @@ extern "C" int mcf_finalize(mcf_ctx *x) {
   if (x->mlp_pk) (void)hipFree(x->mlp_pk);
   if (x->cool_cnt) (void)hipFree(x->cool_cnt);
   if (x->hred) (void)hipHostFree(x->hred);
+  if (x->dtfd) (void)hipFree(x->dtfd);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return MCF_OK;
@@ -812,6 +850,7 @@ extern "C" int mcf_upload(mcf_ctx *x, const double *U8, const double *sp8) {
     HIPCHK(hipMemcpyAsync(x->U[f], U8 + f * x->ncell, bytes, hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(x->sp[f], sp8 + f * x->ncell, bytes, hipMemcpyHostToDevice, x->stream));
   }
+  StageScope scope(x, "upload");
   CHK(do_cc4(x)); // CC(4,0) after the restart read (M3:470)
   HIPCHK(hipStreamSynchronize(x->stream));
   return MCF_OK;
@@ -836,6 +875,36 @@ extern "C" int mcf_download_bcc(mcf_ctx *x, double *bcc3) {
   return MCF_OK;
 }
 
+// DTF (M3:402-431): interior from the caller, x ghosts dinit1 as REAL*4,
+// y and z ghosts the periodic images the reference's halo exchange gives.
+extern "C" int mcf_set_inflow_density(mcf_ctx *x, const float *dtf) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  if (!dtf) {
+    if (x->dtfd) { HIPCHK(hipStreamSynchronize(x->stream)); HIPCHK(hipFree(x->dtfd)); }
+    x->dtfd = nullptr;
+    return MCF_OK;
+  }
+  const int nx = x->g.n[0], ny = x->g.n[1], nz = x->g.n[2];
+  std::vector<float> h(x->ncell, (float)x->dtf);
+  for (int k = -1; k <= nz + 2; ++k)
+    for (int j = -1; j <= ny + 2; ++j) {
+      const int jw = j < 1 ? j + ny : (j > ny ? j - ny : j);
+      const int kw = k < 1 ? k + nz : (k > nz ? k - nz : k);
+      for (int i = 1; i <= nx; ++i) {
+        const float v = dtf[(size_t)(i - 1) + (size_t)nx * ((size_t)(jw - 1) + (size_t)ny * (size_t)(kw - 1))];
+        if (!std::isfinite(v) || v <= 0.0f) { g_err = "inflow density must be finite and positive"; return MCF_E_ARG; }
+        h[(size_t)(i + 1) + (size_t)(j + 1) * x->g.st[1] + (size_t)(k + 1) * x->g.st[2]] = v;
+      }
+    }
+  if (!x->dtfd) {
+    hipError_t e = hipMalloc(&x->dtfd, x->ncell * sizeof(float));
+    if (e != hipSuccess) { x->dtfd = nullptr; g_err = std::string("hipMalloc: ") + hipGetErrorString(e); return MCF_E_NOMEM; }
+  }
+  HIPCHK(hipMemcpyAsync(x->dtfd, h.data(), x->ncell * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
 extern "C" int mcf_set_time(mcf_ctx *x, double time, int ifevo) {
   if (!x || ifevo < 1 || ifevo > 6) { g_err = "bad argument"; return MCF_E_ARG; }
   x->time = time;
@@ -851,6 +920,7 @@ extern "C" int mcf_get_time(mcf_ctx *x, double *time, int *ifevo) {
 
 extern "C" int mcf_timestep(mcf_ctx *x, double tsave, double *dt_out, int *tag) {
   if (!x || !dt_out) { g_err = "null argument"; return MCF_E_ARG; }
+  StageScope scope(x, "dt");
   DtArgs A{};
   for (int f = 0; f < 8; ++f) { A.U[f] = x->U[f]; A.sp[f] = x->sp[f]; }
   for (int c = 0; c < 3; ++c) { A.bcc[c] = x->bcc[c]; A.gdx[c] = x->gdx[c]; }
@@ -968,9 +1038,32 @@ extern "C" int mcf_copy_probe(mcf_ctx *x, long nwords, int reps, double *ms) {
   return MCF_OK;
 }
 
+static void profile_clear(mcf_ctx *x) {
+  for (auto &r : x->recs) {
+    x->evpool.push_back(r.a);
+    x->evpool.push_back(r.b);
+  }
+  x->recs.clear();
+  x->stage_runs.clear();
+}
+
 extern "C" int mcf_profile(mcf_ctx *x, int enable) {
   if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  if (enable && !x->prof) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    profile_clear(x);
+  }
   x->prof = enable != 0;
+  return MCF_OK;
+}
+
+static int profile_copy(const std::map<std::string, mcf_kstat> &acc, mcf_kstat *out, int maxn, int *n) {
+  int i = 0;
+  for (auto &kv : acc) {
+    if (out && i < maxn) out[i] = kv.second;
+    ++i;
+  }
+  *n = i;
   return MCF_OK;
 }
 
@@ -986,17 +1079,47 @@ extern "C" int mcf_profile_report(mcf_ctx *x, mcf_kstat *out, int maxn, int *n) 
     k.launches += 1;
     k.ms += ms;
     k.bytes += r.bytes;
-    x->evpool.push_back(r.a);
-    x->evpool.push_back(r.b);
   }
-  x->recs.clear();
-  int i = 0;
+  return profile_copy(acc, out, maxn, n);
+}
+
+// Compulsory HBM bytes of one run of a stage: every persistent array the
+// stage must read or write, once per interior cell (ghost re-reads and the
+// stage's own temporaries excluded).  A sweep reads rho, v(3), p, the 8
+// species, Bcc(3) and the normal face B and writes rho, v(3), p and the
+// species (17 + 13 doubles); CT reads rho, v(3), p and face B(3) and writes
+// face B(3) and Bcc(3) (8 + 6); a SOURCE call reads rho, p and the five
+// active species and writes p (7 + 1); the dt reduction reads rho, v(3), p,
+// Bcc(3) and the five active species (13); DISSIP reads and writes v(3) in
+// the ten cells next to each x end.
+static double stage_bytes(const mcf_ctx *x, const std::string &st) {
+  const double cells = (double)x->g.n[0] * x->g.n[1] * x->g.n[2];
+  if (st.rfind("sweep_", 0) == 0) return 8.0 * 30.0 * cells;
+  if (st == "ct") return 8.0 * 14.0 * cells;
+  if (st == "source") return 8.0 * 8.0 * cells;
+  if (st == "dt") return 8.0 * 13.0 * cells;
+  if (st == "dissip") return 8.0 * 6.0 * 20.0 * x->g.n[1] * x->g.n[2];
+  return 0.0;
+}
+
+extern "C" int mcf_profile_stages(mcf_ctx *x, mcf_kstat *out, int maxn, int *n) {
+  if (!x || !n) { g_err = "null argument"; return MCF_E_ARG; }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  std::map<std::string, mcf_kstat> acc;
+  for (auto &r : x->recs) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    mcf_kstat &k = acc[r.stage];
+    std::snprintf(k.name, sizeof(k.name), "%s", r.stage);
+    k.ms += ms;
+  }
   for (auto &kv : acc) {
-    if (out && i < maxn) out[i] = kv.second;
-    ++i;
+    auto it = x->stage_runs.find(kv.first);
+    const int runs = it == x->stage_runs.end() ? 0 : it->second;
+    kv.second.launches = runs;
+    kv.second.bytes = stage_bytes(x, kv.first) * runs;
   }
-  *n = i;
-  return MCF_OK;
+  return profile_copy(acc, out, maxn, n);
 }
 
 // ----------------------------------------------------------------------------

@@ -41,7 +41,18 @@ struct FillList {
   int dm[16]; // k_fill_yz: the passes of each field (bit 0: y, bit 1: z)
 };
 
-enum XRule { XR_PERIODIC = 0, XR_CONST, XR_COPY, XR_FACE, XR_MOM, XR_ENERGY, XR_SPECIES };
+enum XRule { XR_PERIODIC = 0, XR_CONST, XR_COPY, XR_FACE, XR_MOM, XR_ENERGY, XR_SPECIES, XR_DTF };
+
+// Inflow density read from the DTF field shifted by -v t and wrapped in x
+// (XR_DTF): the field (float, the (-1:N+2)^3 box), the simulation time, the
+// inflow velocity, dx(0) and L = dx(0)*Ncellx*NSPLTx of the boundary rank on
+// each side, and the global cell edges x(i) at xe[i+1].
+struct DtfShift {
+  const float *f;
+  const double *xe;
+  double t;
+  double v[2], dx0[2], L[2];
+};
 
 // x-direction fill with the inflow (BC=4) rules of BC_MPI / BC_MPI_OT
 struct XFill {
@@ -52,6 +63,7 @@ struct XFill {
   int nf;
   const double *rho, *bcc1, *blg1, *blg2;
   double val2[2];
+  DtfShift dtf;
 };
 
 // Fused sweep stages (mcf_sweep.hip): inputs are the current buffers,

@@ -127,6 +127,24 @@ __global__ void k_fill_xrule(XFill X, Geo g, int nol) {
     case XR_SPECIES:
       if (inner) a[c] = X.val[f][side] * X.rho[c];
       break;
+    case XR_DTF: { // M3:818-826 (left), 854-861 (right), in the reference's operation order
+      const DtfShift &D = X.dtf;
+      const double dx0 = D.dx0[side], L = D.L[side];
+      double xp;
+      int nn;
+      if (side == 0) {
+        xp = D.xe[1] - 0.5 * dx0 + (double)i * dx0 - D.t * D.v[0];
+        nn = (int)(fabs(xp) / L) + 1;
+        xp = xp + L * (double)nn;
+      } else {
+        xp = 0.5 * (D.xe[i] + D.xe[i + 1]) - D.t * D.v[1];
+        nn = (int)(fabs(xp) / L);
+        xp = xp - L * (double)nn;
+      }
+      const int ii = (int)(xp / dx0) + 1;
+      a[c] = (double)D.f[ix3(g, ii, j, k)];
+      break;
+    }
     }
   }
 }

@@ -78,7 +78,12 @@ __device__ __forceinline__ R vslope(double qm, double q0, double qp, R dxm, R dx
     const float d2 = mm ? dx0 + dxm : dx0 + 0.5f * dxm + 0.5f * dxp;
     const float r2 = A::div(n2, d2);
     const float m = rmin(fabsf(r1), fabsf(r2));
-    const float gm = (r1 * r2 > 0.0f) ? copysignf(m, r1) : 0.0f;
+    // minmod keeps the smaller slope when the two have the same sign; the
+    // signs are compared directly because the FP32 product of two slopes of
+    // a tiny field (a species near its 1e-20 floor: r ~ 1e-22) underflows
+    // to zero and would zero the slope where the FP64 product does not
+    const bool same = (r1 > 0.0f && r2 > 0.0f) || (r1 < 0.0f && r2 < 0.0f);
+    const float gm = same ? copysignf(m, r1) : 0.0f;
     return mm ? gm : r2;
   }
 }

@@ -1,13 +1,19 @@
 ! mcf_evolve.f90 -- Fortran driver: the reference's EVOLVE loop on one GPU.
 !
 ! Follows 3DMHD-CfCA/MCF3DSGincfca3.f90 ("M3"): INPUT3D.DAT (M3:134-151),
-! tsave.DAT / time.DAT (M3:309-312, 520-530), the restart files 000{rank}.dat
-! of an NPE-rank run (M3:454-465, 749-757), the EVOLVE outer/inner loops with
-! the tsave/tfinal stops and the dt print (M3:554-670), SAVEU's FP32 snapshot,
-! FP64 restart, time.DAT, tsave.DAT and info{nunit}.DAT (M3:684-757).  The
-! whole domain lives on the GPU; each physics call of the inner loop is one
-! call through the C ABI (mcf_timestep for Couran+Stblty+reduction, mcf_step
-! for SOURCE / MHD x3 / CT / SOURCE / DISSIP).
+! the cell-centre file cdnt.DAT (M3:300-307), tsave.DAT / tsave2D.DAT /
+! time.DAT (M3:309-312, 520-530), the restart files 000{rank}.dat of an
+! NPE-rank run (M3:454-465, 749-757), the EVOLVE outer/inner loops with the
+! tsave/tfinal stops, the 2D-slice cadence and the dt print (M3:554-670),
+! the wall-clock guard that ends a run after 11.7 hours (M3:663-670), SAVEU's
+! FP32 snapshot, FP64 restart, time.DAT, tsave.DAT and info{nunit}.DAT
+! (M3:684-757) and SAVEU2D's k = 1 slices 2D{nunit2D}{rank}.dat with
+! tsave2D.DAT (M3:763-794).  The whole domain lives on the GPU; each physics
+! call of the inner loop is one call through the C ABI (mcf_timestep for
+! Couran+Stblty+reduction, mcf_step for SOURCE / MHD x3 / CT / SOURCE /
+! DISSIP).
+!
+! Environment: MCF_WALL_HOURS overrides the 11.7-hour wall-clock limit.
 !
 ! usage: mcf_evolve [rundir [NPE]]   (NPE: rank count of the reproduced run,
 !                                     default 8; fixes the block split)
@@ -18,14 +24,15 @@ program mcf_evolve
 
   type(mcf_config) :: cfg
   type(c_ptr) :: ctx
-  real(c_double), allocatable, target :: U(:,:,:,:), SP(:,:,:,:)
+  real(c_double), allocatable, target :: U(:,:,:,:), SP(:,:,:,:), BC(:,:,:,:)
   character(len=512) :: rundir, arg
-  integer :: npe, nsx, nsy, nsz, nx, ny, nz, nunit, st, itime, in10, in20, ifevo
+  integer :: npe, nsx, nsy, nsz, nx, ny, nz, nunit, nunit2D, st, itime, in10, in20, ifevo
   integer(c_int) :: rc, tag
-  real(c_double) :: time, dt, tsave, tfinal
-  real(c_double), parameter :: dtsave = 0.25d0
-  real(c_double) :: t(1000)
-  integer :: i
+  real(c_double) :: time, dt, tsave, tfinal, tsave2D
+  real(c_double), parameter :: dtsave = 0.25d0, dtsave2D = 0.025d0
+  real(c_double) :: t(1000), time_cpu(3), wall_limit
+  integer(8) :: clk0, clk1, clkrate
+  integer :: i, ios
 
   rundir = '.'
   npe = 8
@@ -41,11 +48,21 @@ program mcf_evolve
   cfg%nsplitx = nsx; cfg%nsplity = nsy; cfg%nsplitz = nsz
   nx = cfg%np1x + cfg%np2x; ny = cfg%np1y + cfg%np2y; nz = cfg%np1z + cfg%np2z
   tfinal = cfg%tfinal
+  call write_cdnt(trim(rundir), cfg, nx, ny, nz)
+  wall_limit = 11.7d0
+  call get_environment_variable('MCF_WALL_HOURS', arg, status=ios)
+  if (ios == 0) read(arg, *) wall_limit
 
   open(2, file=trim(rundir)//'/tsave.DAT')
   read(2, *) time
   read(2, *) nunit
   close(2)
+  nunit2D = 0
+  open(2, file=trim(rundir)//'/tsave2D.DAT', status='old', iostat=ios)
+  if (ios == 0) then
+    read(2, *) nunit2D
+    close(2)
+  end if
   t = 0.d0
   open(3, file=trim(rundir)//'/time.DAT')
   do i = 1, nunit
@@ -53,7 +70,7 @@ program mcf_evolve
   end do
 10 close(3)
 
-  allocate(U(-1:nx+2, -1:ny+2, -1:nz+2, 8), SP(-1:nx+2, -1:ny+2, -1:nz+2, 8))
+  allocate(U(-1:nx+2, -1:ny+2, -1:nz+2, 8), SP(-1:nx+2, -1:ny+2, -1:nz+2, 8), BC(-1:nx+2, -1:ny+2, -1:nz+2, 3))
   U = 0.d0; SP = 0.d0
   if (nunit == 1) then
     call shock_tube_ic(cfg, U, SP, nx, ny, nz)
@@ -69,12 +86,17 @@ program mcf_evolve
   st = 1
   dt = 0.d0
   itime = 1 + int((time + 1.d-8)/dtsave)
+  time_cpu = 0.d0
+  call system_clock(count_rate=clkrate)
   do in10 = 1, cfg%maxstp
+    call system_clock(clk0)
     tsave = dtsave*dble(itime)
     if (time >= tfinal) goto 9000
     if (time >= tsave) goto 7777
     call saveu(0)
     do in20 = 1, cfg%nitera
+      tsave2D = dtsave2D*nunit2D
+      if (time >= tsave2D) call saveu2d()
       if (time >= tfinal) goto 9000
       if (time >= tsave) goto 7777
       call mcf_check(mcf_timestep(ctx, tsave, dt, tag), 'mcf_timestep')
@@ -86,6 +108,11 @@ program mcf_evolve
     itime = itime - 1
 7777 continue
     itime = itime + 1
+    ! wall-clock guard (M3:660-670): hours of this outer iteration, summed
+    call system_clock(clk1)
+    time_cpu(2) = dble(clk1 - clk0)/dble(clkrate)/3.6d3
+    time_cpu(3) = time_cpu(3) + time_cpu(2)
+    if (time_cpu(3) + time_cpu(2) > wall_limit) goto 9000
   end do
 9000 continue
   call saveu(1)
@@ -120,6 +147,19 @@ contains
     close(2)
     if (msig == 1) close(5)
   end subroutine saveu
+
+  ! SAVEU2D (M3:763-794): each rank's k = 1 plane as FP32 records, Bcc in
+  ! place of the face field, then the slice counter and time
+  subroutine saveu2d()
+    call mcf_check(mcf_download(ctx, U, SP), 'mcf_download')
+    call mcf_check(mcf_download_bcc(ctx, BC), 'mcf_download_bcc')
+    call write_rank_slices(trim(rundir), U, SP, BC, nx, ny, nz, nsx, nsy, nsz, nunit2D)
+    nunit2D = nunit2D + 1
+    open(3, file=trim(rundir)//'/tsave2D.DAT')
+    write(3, *) nunit2D
+    write(3, *) time
+    close(3)
+  end subroutine saveu2d
 
 end program mcf_evolve
 
@@ -281,3 +321,82 @@ subroutine write_rank_files(dir, U, SP, nx, ny, nz, sx, sy, sz, nunit)
     close(8)
   end do
 end subroutine write_rank_files
+
+
+! cdnt.DAT (M3:300-307): the global cell centres along x, y and z, one
+! D25.17 value per line, from the grid INITIA builds (M3:255-280)
+subroutine write_cdnt(dir, cfg, nx, ny, nz)
+  use mcf_abi_mod
+  character(*), intent(in) :: dir
+  type(mcf_config), intent(in) :: cfg
+  integer, intent(in) :: nx, ny, nz
+  double precision :: x_i(-1:nx+2), y_i(-1:ny+2), z_i(-1:nz+2), d(-1:max(nx, ny, nz)+2)
+  integer :: i
+  do i = -1, nx + 2
+    if (i <= cfg%np1x) then
+      d(i) = cfg%ql1x/dble(cfg%np1x)
+    else
+      d(i) = cfg%ql2x/dble(cfg%np2x)
+    end if
+  end do
+  x_i(-1) = -d(0)
+  do i = 0, nx + 2
+    x_i(i) = x_i(i-1) + d(i)
+  end do
+  d = cfg%ql1y/dble(cfg%np1y)
+  y_i(-1) = -d(0)
+  do i = 0, ny + 2
+    y_i(i) = y_i(i-1) + d(i)
+  end do
+  d = cfg%ql1z/dble(cfg%np1z)
+  z_i(-1) = -d(0)
+  do i = 0, nz + 2
+    z_i(i) = z_i(i-1) + d(i)
+  end do
+  open(4, file=dir//'/cdnt.DAT')
+  write(4, '(D25.17)') (0.5d0*(x_i(i-1) + x_i(i)), i = 1, nx)
+  write(4, '(D25.17)') (0.5d0*(y_i(i-1) + y_i(i)), i = 1, ny)
+  write(4, '(D25.17)') (0.5d0*(z_i(i-1) + z_i(i)), i = 1, nz)
+  close(4)
+end subroutine write_cdnt
+
+
+! Per-rank 2D slices 2D{nunit2D}{rank}.dat (M3:763-783): the rank's local
+! k = 1 plane, one FP32 record of 17 values per cell (rho, v, p, Bcc,
+! species, Phi = 0) for i = 1..Ncellx, per j = 1..Ncelly
+subroutine write_rank_slices(dir, U, SP, BC, nx, ny, nz, sx, sy, sz, nunit2D)
+  character(*), intent(in) :: dir
+  integer, intent(in) :: nx, ny, nz, sx, sy, sz, nunit2D
+  double precision, intent(in) :: U(-1:nx+2, -1:ny+2, -1:nz+2, 8), SP(-1:nx+2, -1:ny+2, -1:nz+2, 8)
+  double precision, intent(in) :: BC(-1:nx+2, -1:ny+2, -1:nz+2, 3)
+  integer :: r, ist, jst, kst, cx, cy, cz, i, j, m, gi, gj, gk
+  real(4), allocatable :: rec(:,:)
+  character(len=3) :: rs, ns
+  cx = nx/sx; cy = ny/sy; cz = nz/sz
+  allocate(rec(17, cx))
+  write(ns, '(I3.3)') nunit2D
+  do r = 0, sx*sy*sz - 1
+    ist = mod(r, sx); kst = r/(sx*sy); jst = r/sx - sy*kst
+    write(rs, '(I3.3)') r
+    open(11, file=dir//'/2D'//ns//rs//'.dat', form='unformatted')
+    gk = kst*cz + 1
+    do j = 1, cy
+      gj = jst*cy + j
+      do i = 1, cx
+        gi = ist*cx + i
+        do m = 1, 5
+          rec(m, i) = real(U(gi, gj, gk, m), 4)
+        end do
+        do m = 1, 3
+          rec(5 + m, i) = real(BC(gi, gj, gk, m), 4)
+        end do
+        do m = 1, 8
+          rec(8 + m, i) = real(SP(gi, gj, gk, m), 4)
+        end do
+        rec(17, i) = 0.0
+      end do
+      write(11) rec
+    end do
+    close(11)
+  end do
+end subroutine write_rank_slices

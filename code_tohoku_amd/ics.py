@@ -319,6 +319,16 @@ def converging_flow_state(p: Input3D, drho: float = 0.1, idum: int = 1) -> np.nd
     return st
 
 
+def inflow_density_field(st: np.ndarray, p: Input3D) -> np.ndarray:
+    """The DTF field of a state: its interior density as REAL*4, [k, j, i].
+    INITIA's fresh-start path sets the density to this field
+    (MCF3DSGincfca3.f90:437), and the x-inflow ghosts read it shifted by -v t
+    (:818-826, 854-861), so the gas entering the box carries the initial
+    density structure."""
+    nx, ny, nz = p.n
+    return np.ascontiguousarray(st[0, 2:nz + 2, 2:ny + 2, 2:nx + 2], dtype=np.float32)
+
+
 def turbulence_input(n: int = 512) -> Input3D:
     """Decaying MHD turbulence box: 10 pc periodic cube, n_tot = 10 cm^-3
     (the WNM composition ratios of MCF3DSGincfca3.f90:156-157), T = 100 K,

@@ -72,6 +72,7 @@ def lib():
         L.mcf_download.argtypes = [P, pd, pd]
         L.mcf_download_bcc.argtypes = [P, pd]
         L.mcf_set_time.argtypes = [P, D, I]
+        L.mcf_set_inflow_density.argtypes = [P, C.c_void_p]
         L.mcf_get_time.argtypes = [P, pd, pi]
         L.mcf_timestep.argtypes = [P, D, pd, pi]
         L.mcf_source.argtypes = [P, D]
@@ -87,6 +88,7 @@ def lib():
         L.mcf_profile.argtypes = [P, I]
         L.mcf_copy_probe.argtypes = [P, C.c_long, I, pd]
         L.mcf_profile_report.argtypes = [P, C.POINTER(KStat), I, pi]
+        L.mcf_profile_stages.argtypes = [P, C.POINTER(KStat), I, pi]
         pf, pi32, pi64 = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
         L.mcf_mlp_init.argtypes = [P, C.c_uint64]
         L.mcf_mlp_set_weights.argtypes = [P, pf]
@@ -98,9 +100,9 @@ def lib():
         L.mcf_cool_solve.argtypes = [P, pd, pd, pd, pd, pd, C.c_int64, I, pd, pi32, pd]
         L.mcf_cool_stats.argtypes = [P, pi64, I]
         for nm in ("mcf_init", "mcf_finalize", "mcf_upload", "mcf_download", "mcf_download_bcc",
-                   "mcf_set_time", "mcf_get_time", "mcf_timestep", "mcf_source", "mcf_sweep",
+                   "mcf_set_time", "mcf_set_inflow_density", "mcf_get_time", "mcf_timestep", "mcf_source", "mcf_sweep",
                    "mcf_ct", "mcf_dissip", "mcf_step", "mcf_run", "mcf_sync", "mcf_profile",
-                   "mcf_profile_report", "mcf_copy_probe", "mcf_mlp_init", "mcf_mlp_set_weights",
+                   "mcf_profile_report", "mcf_profile_stages", "mcf_copy_probe", "mcf_mlp_init", "mcf_mlp_set_weights",
                    "mcf_mlp_get_weights", "mcf_mlp_train", "mcf_mlp_forward", "mcf_mlp_grad",
                    "mcf_cool_samples", "mcf_cool_solve", "mcf_cool_stats"):
             getattr(L, nm).restype = C.c_int
@@ -173,6 +175,16 @@ class Domain:
         _chk(lib().mcf_download_bcc(self.h, _dp(b)))
         return b
 
+    def set_inflow_density(self, dtf):
+        """x-inflow density field DTF (float32 [k, j, i] over the interior),
+        shifted by -v t into the inflow ghosts; None for the uniform value."""
+        if dtf is None:
+            _chk(lib().mcf_set_inflow_density(self.h, None))
+            return
+        a = np.ascontiguousarray(dtf, dtype=np.float32)
+        assert a.shape == tuple(self.n[::-1])
+        _chk(lib().mcf_set_inflow_density(self.h, a.ctypes.data))
+
     def set_time(self, time: float, ifevo: int = 1):
         _chk(lib().mcf_set_time(self.h, time, ifevo))
 
@@ -229,11 +241,19 @@ class Domain:
         _chk(lib().mcf_profile(self.h, 1 if enable else 0))
 
     def profile_report(self):
-        """{kernel: (launches, ms, algorithmic bytes)} since the last report."""
+        """{kernel: (launches, ms, bytes of the arrays it touches)} since
+        profiling was enabled."""
         buf = (KStat * 128)()
         n = C.c_int()
         _chk(lib().mcf_profile_report(self.h, buf, 128, C.byref(n)))
         return {buf[i].name.decode(): (buf[i].launches, buf[i].ms, buf[i].bytes) for i in range(min(n.value, 128))}
+
+    def profile_stages(self):
+        """{stage: (runs, ms, compulsory bytes)} since profiling was enabled."""
+        buf = (KStat * 32)()
+        n = C.c_int()
+        _chk(lib().mcf_profile_stages(self.h, buf, 32, C.byref(n)))
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].ms, buf[i].bytes) for i in range(min(n.value, 32))}
 
     # --- KI-cooling surrogate -------------------------------------------
     def mlp_init(self, seed: int = 1):

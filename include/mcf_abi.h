@@ -79,6 +79,15 @@ int mcf_download(mcf_ctx *ctx, double *U8, double *species8);
 /* Cell-centred field Bcc(:,:,:,1:3) (contiguous). */
 int mcf_download_bcc(mcf_ctx *ctx, double *bcc3);
 
+/* Inhomogeneous x-inflow density: the REAL*4 field DTF that INITIA reads
+ * from DTF/D{rank}.dat (M3:402-431), given as nx*ny*nz float32 values over
+ * the interior, x fastest (the rank files in global order).  The inflow
+ * ghost density of every fill is then DTF at the ghost's position shifted by
+ * -v t and wrapped periodically in x (M3:818-826, 854-861) instead of the
+ * uniform dinit1 of the shipped reference, which skips the read (M3:404).
+ * NULL restores the uniform value.  Values must be finite and positive. */
+int mcf_set_inflow_density(mcf_ctx *ctx, const float *dtf);
+
 /* Simulation clock and sweep-order counter (time, ifEVO of EVOLVE). */
 int mcf_set_time(mcf_ctx *ctx, double time, int ifevo);
 int mcf_get_time(mcf_ctx *ctx, double *time, int *ifevo);
@@ -103,12 +112,18 @@ int mcf_step(mcf_ctx *ctx, double dt);
 int mcf_run(mcf_ctx *ctx, int nsteps, double tsave, double *dts, int *ndone);
 
 /* Per-kernel timing: while enabled, every kernel launch of the context is
- * bracketed by HIP events on the context's stream.  mcf_profile_report
- * waits for the device, then returns one entry per kernel name with the
- * number of launches, the summed event time and the summed algorithmic
- * bytes (each array the kernel reads or writes counted once per element of
- * its launch).  No reference counterpart (the reference only keeps the
- * wall clock of M3:556,663-666). */
+ * bracketed by HIP events on the context's stream (enabling clears what was
+ * recorded before).  mcf_profile_report waits for the device, then returns
+ * one entry per kernel name with the number of launches, the summed event
+ * time and the summed bytes of the arrays the kernel touches (each array it
+ * reads or writes counted once per element of its launch).
+ * mcf_profile_stages returns one entry per stage of the step ("dt",
+ * "source", "sweep_x", "sweep_y", "sweep_z", "ct", "dissip", "upload"):
+ * launches = runs of the stage, ms = summed event time of all its launches
+ * (ghost fills included), bytes = the stage's compulsory HBM bytes (each
+ * persistent array it must read or write, once per interior cell) times its
+ * runs.  No reference counterpart (the reference only keeps the wall clock
+ * of M3:556,663-666). */
 typedef struct mcf_kstat {
   char name[48];
   int launches;
@@ -117,6 +132,7 @@ typedef struct mcf_kstat {
 } mcf_kstat;
 int mcf_profile(mcf_ctx *ctx, int enable);
 int mcf_profile_report(mcf_ctx *ctx, mcf_kstat *out, int maxn, int *n);
+int mcf_profile_stages(mcf_ctx *ctx, mcf_kstat *out, int maxn, int *n);
 
 /* Streams nwords doubles from one device buffer to another reps times with
  * a plain copy kernel (k_copy) and returns the mean time per copy: the

@@ -5,11 +5,15 @@
 # /opt/conda.  Test infrastructure: the binary goes to oracle/_ref/bin/ and is
 # used only to write golden fixtures in this container.
 #
-# usage: oracle/build_ref.sh [NDX]     (NDX = compile-time block size, default 18)
+# usage: oracle/build_ref.sh [NDX [dtf]]  (NDX = compile-time block size, default 18;
+#        dtf: read the inhomogeneous inflow density field DTF/D{rank}.dat)
 #
 # -O2 without -march: baseline x86-64 has no FMA, so nothing is fused.
 set -euo pipefail
 NDX=${1:-18}
+VARIANT=${2:-}
+SUFFIX=""
+if [ "$VARIANT" = "dtf" ]; then SUFFIX="_dtf"; fi
 REF=/root/reference/3DMHD-CfCA
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref
@@ -22,10 +26,10 @@ fi
 mkdir -p "$OUT/bin"
 TMP=$(mktemp -d "$OUT/tmp_build.XXXXXX")
 trap 'rm -rf "$TMP"' EXIT
-python3 "$HERE/patch_ref.py" "$REF/MCF3DSGincfca3.f90" "$TMP/m3.f90" "$NDX"
+python3 "$HERE/patch_ref.py" "$REF/MCF3DSGincfca3.f90" "$TMP/m3.f90" "$NDX" $VARIANT
 cp "$REF/selfG_MPIfull8.f90" "$TMP/sg8.f90"
 cd "$TMP"
 $FC -O2 -I$MPI/include -c m3.f90 -o m3.o 2>build.log || { cat build.log; exit 1; }
 $FC -O2 -I$MPI/include -c sg8.f90 -o sg8.o 2>>build.log || { cat build.log; exit 1; }
-$FC -O2 -o "$OUT/bin/mcf_ref_ndx$NDX" m3.o sg8.o -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi
-echo "built $OUT/bin/mcf_ref_ndx$NDX"
+$FC -O2 -o "$OUT/bin/mcf_ref_ndx$NDX$SUFFIX" m3.o sg8.o -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi
+echo "built $OUT/bin/mcf_ref_ndx$NDX$SUFFIX"

@@ -56,6 +56,7 @@ struct mcfo {
   double *vfc[3], *dnc, *emf[3], *dq[4];
   double *Tn, *Pn, *Qx, *Qy, *Qz;
   double *ppre; /* U(:,:,:,5) right after the RHS ghost fill of a sweep */
+  float *dtfa;  /* inhomogeneous inflow density field DTF (REAL*4), NULL = uniform */
   int nsp[3];   /* emulated MPI block split */
   double *pw;   /* pencil work pool, one slice of 64 arrays per thread */
   int nthr;     /* threads the pool has slices for */
@@ -195,6 +196,25 @@ mcfo *mcfo_create(const mcfo_params *p) {
   return o;
 }
 
+/* DTF (M3:402-431): the REAL*4 field read from DTF/D{rank}.dat over the
+ * global interior (x fastest), x ghosts left at dinit1, y and z ghosts the
+ * periodic images the reference's halo exchange gives them. */
+int mcfo_set_dtf(mcfo *o, const float *interior) {
+  if (!interior) { free(o->dtfa); o->dtfa = NULL; return 0; }
+  if (!o->dtfa) o->dtfa = (float *)malloc(sizeof(float) * (size_t)o->ncell);
+  if (!o->dtfa) return -1;
+  const int nx = o->n[0], ny = o->n[1], nz = o->n[2];
+  for (long c = 0; c < o->ncell; c++) o->dtfa[c] = (float)o->dtf;
+  for (int k = -1; k <= nz + 2; k++)
+    for (int j = -1; j <= ny + 2; j++) {
+      const int jw = j < 1 ? j + ny : (j > ny ? j - ny : j);
+      const int kw = k < 1 ? k + nz : (k > nz ? k - nz : k);
+      for (int i = 1; i <= nx; i++)
+        o->dtfa[ix3(o, i, j, k)] = interior[(size_t)(i - 1) + (size_t)nx * ((size_t)(jw - 1) + (size_t)ny * (size_t)(kw - 1))];
+    }
+  return 0;
+}
+
 int mcfo_set_threads(mcfo *o, int nthreads) {
   if (nthreads < 1) nthreads = nthreads_max();
   double *pw = (double *)calloc((size_t)nthreads * 64 * (o->nmax + 8), sizeof(double));
@@ -214,6 +234,7 @@ void mcfo_destroy(mcfo *o) {
                    o->Tn, o->Pn, o->Qx, o->Qy, o->Qz, o->ppre};
   for (size_t t = 0; t < sizeof(tmp) / sizeof(tmp[0]); t++) free(tmp[t]);
   free(o->pw);
+  free(o->dtfa);
   free(o);
 }
 
@@ -261,6 +282,31 @@ static void periodic(mcfo *o, double *a, int d, int nol, int left) {
   fill_periodic(o, a, d, nol, left, -1, t1hi, -1, t2hi);
 }
 
+/* Inflow ghost density read from the DTF field shifted by -v t and wrapped
+ * periodically in x (left M3:818-826, right M3:854-861).  The block edge
+ * quantities are those of the rank that owns the boundary: dx(0) of the
+ * first or last x block, L = dx(0)*Ncellx*NSPLTx. */
+static double dtf_ghost(const mcfo *o, int side, int IX, int JY, int KZ) {
+  const int cx = o->n[0] / o->nsp[0], nsx = o->nsp[0];
+  double xp, dx0, L;
+  int nn;
+  if (side == 1) {
+    dx0 = dxg(o, 0, 0);
+    L = dx0 * cx * nsx;
+    xp = xg(o, 0, 0) - 0.5 * dx0 + IX * dx0 - o->time * o->bbrv[2][1][1];
+    nn = (int)(fabs(xp) / L) + 1;
+    xp = xp + L * nn;
+  } else {
+    dx0 = dxg(o, 0, (nsx - 1) * cx);
+    L = dx0 * cx * nsx;
+    xp = 0.5 * (xg(o, 0, IX - 1) + xg(o, 0, IX)) - o->time * o->bbrv[2][1][2];
+    nn = (int)(fabs(xp) / L);
+    xp = xp - L * nn;
+  }
+  const int II = (int)(xp / dx0) + 1;
+  return (double)o->dtfa[ix3(o, II, JY, KZ)];
+}
+
 /* BC_MPI x-side rule for U(:,:,:,K) (M3:818-884). side 1 = left, 2 = right. */
 static void xrule_U(mcfo *o, int K, int side, int nol, int mode) {
   int nx = o->n[0];
@@ -271,7 +317,7 @@ static void xrule_U(mcfo *o, int K, int side, int nol, int mode) {
       for (int i = lo; i <= hi; i++) {
         long c = ix3(o, i, j, k);
         if (K == 1) {
-          U(1)[c] = o->dtf;
+          U(1)[c] = o->dtfa ? dtf_ghost(o, side, i, j, k) : o->dtf;
         } else if (K == 6) {
           U(6)[c] = U(6)[ix3(o, side == 1 ? 1 : nx + 1, j, k)];
         } else if (K == 5 && mode == 2) {

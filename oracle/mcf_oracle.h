@@ -59,6 +59,12 @@ void    mcfo_destroy(mcfo *o);
  * nthreads < 1 restores it).  Results do not depend on it.  Returns the
  * count, or -1 if the work arrays cannot be allocated. */
 int     mcfo_set_threads(mcfo *o, int nthreads);
+/* Inhomogeneous x-inflow density DTF (M3:402-431, 818-826, 854-861):
+ * nx*ny*nz float32 values over the interior, x fastest (the reference's
+ * DTF/D{rank}.dat files in global order).  NULL restores the uniform dinit1
+ * of the shipped reference, which skips the read (M3:404).  Returns 0, or
+ * -1 if the copy cannot be allocated. */
+int     mcfo_set_dtf(mcfo *o, const float *interior);
 double *mcfo_field(mcfo *o, int id);
 double *mcfo_grid(mcfo *o, int axis, int spacing); /* x/dx arrays, index -1..n+2 */
 void    mcfo_get_time(const mcfo *o, double *time, int *ifevo);

@@ -52,6 +52,8 @@ def lib():
         L.mcfo_destroy.argtypes = [P]
         L.mcfo_set_threads.argtypes = [P, C.c_int]
         L.mcfo_set_threads.restype = C.c_int
+        L.mcfo_set_dtf.argtypes = [P, C.c_void_p]
+        L.mcfo_set_dtf.restype = C.c_int
         L.mcfo_field.restype = pd
         L.mcfo_field.argtypes = [P, C.c_int]
         L.mcfo_grid.restype = pd
@@ -117,6 +119,17 @@ class Oracle:
         if getattr(self, "h", None):
             lib().mcfo_destroy(self.h)
             self.h = None
+
+    def set_dtf(self, dtf):
+        """Inflow density field (float32 [k, j, i] over the interior), or
+        None for the uniform inflow density."""
+        if dtf is None:
+            lib().mcfo_set_dtf(self.h, None)
+            return
+        self._dtf = np.ascontiguousarray(dtf, dtype=np.float32)
+        assert self._dtf.shape == tuple(self.n[::-1])
+        if lib().mcfo_set_dtf(self.h, self._dtf.ctypes.data) != 0:
+            raise MemoryError("oracle DTF field")
 
     def field(self, fid: int) -> np.ndarray:
         ptr = lib().mcfo_field(self.h, fid)

@@ -26,21 +26,44 @@ REF_BIN = os.path.join(HERE, "_ref", "bin")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
-def ref_binary(ndx: int) -> str:
-    path = os.path.join(REF_BIN, f"mcf_ref_ndx{ndx}")
+def ref_binary(ndx: int, dtf: bool = False) -> str:
+    path = os.path.join(REF_BIN, f"mcf_ref_ndx{ndx}" + ("_dtf" if dtf else ""))
     if not os.path.exists(path):
-        subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(ndx)], check=True)
+        cmd = ["bash", os.path.join(HERE, "build_ref.sh"), str(ndx)] + (["dtf"] if dtf else [])
+        subprocess.run(cmd, check=True)
     return path
 
 
+def write_dtf_files(run: str, dtf: np.ndarray, n, nsplit):
+    """DTF/D{rank}.dat as INITIA reads them (M3:405-415): per rank, for
+    k = 1..Ncellz and j = 1..Ncelly of its block, one record of the REAL*4
+    density over its x range.  dtf: float32 [k, j, i] over the global
+    interior."""
+    nx, ny, nz = n
+    cx, cy, cz = nx // nsplit[0], ny // nsplit[1], nz // nsplit[2]
+    os.makedirs(os.path.join(run, "DTF"), exist_ok=True)
+    for r in range(nsplit[0] * nsplit[1] * nsplit[2]):
+        ist = r % nsplit[0]
+        kst = r // (nsplit[0] * nsplit[1])
+        jst = r // nsplit[0] - nsplit[1] * kst
+        with open(os.path.join(run, "DTF", f"D{r:03d}.dat"), "wb") as f:
+            for k in range(cz):
+                for j in range(cy):
+                    row = np.ascontiguousarray(dtf[kst * cz + k, jst * cy + j, ist * cx:(ist + 1) * cx], dtype="<f4")
+                    mark = np.array([row.nbytes], dtype="<i4").tobytes()
+                    f.write(mark + row.tobytes() + mark)
+
+
 def run_reference(params: rs.Input3D, state: np.ndarray, nsteps: int, nranks: int = 8,
-                  tag: str = "run", time0: float = 0.0) -> dict:
+                  tag: str = "run", time0: float = 0.0, dtf: np.ndarray | None = None) -> dict:
+    """dtf (float32 [k, j, i], global interior): run the build that reads
+    the inhomogeneous inflow density field, with these values."""
     nsplit = rs.NSPLIT_TABLE[nranks]
     nx, ny, nz = params.n
     cx, cy, cz = nx // nsplit[0], ny // nsplit[1], nz // nsplit[2]
     assert cx * nsplit[0] == nx and cy * nsplit[1] == ny and cz * nsplit[2] == nz
     ndx = max(cx, cy, cz) + 2
-    exe = ref_binary(ndx)
+    exe = ref_binary(ndx, dtf is not None)
     run = os.path.join(HERE, "_ref", "runs", tag)
     if os.path.isdir(run):
         shutil.rmtree(run)
@@ -52,6 +75,8 @@ def run_reference(params: rs.Input3D, state: np.ndarray, nsteps: int, nranks: in
     rs.write_counters(run, time=time0, nunit=2)
     for r, b in enumerate(rs.split_state(state, nsplit)):
         rs.write_restart_rank(os.path.join(run, f"000{r:03d}.dat"), b)
+    if dtf is not None:
+        write_dtf_files(run, dtf, params.n, nsplit)
     cmd = f"ulimit -s unlimited; {MPIEXEC} -n {nranks} {exe}"
     res = subprocess.run(["bash", "-c", cmd], cwd=run, capture_output=True, text=True, timeout=1800)
     if res.returncode != 0:

@@ -14,7 +14,16 @@ Cases:
   inflow_48x8x8    x inflow, 16 ranks (4,2,2), 6 steps
   briowu_64x8x8    config 1 geometry (dx = dy = dz = 1/64): planar Brio-Wu
                    double shock tube, periodic x, no cooling, 200 steps.
-usage: python tests/golden/make_golden.py
+  converging_24    the converging-flow initial state (Kolmogorov density,
+                   x inflow, B at 45 deg, KI cooling; the reference's own WNM
+                   composition and pressure) at 24^3, 8 ranks,
+                   8 steps; also keeps the files the reference writes on the
+                   way: the 2D slices 2D{nunit2D}{rank}.dat (SAVEU2D,
+                   M3:763-794), tsave2D.DAT and cdnt.DAT (M3:300-307).
+  converging_dtf_24  the same with the inflow density read from the DTF
+                   field and shifted by -v t (the build that does not skip
+                   the DTF read, oracle/patch_ref.py), 8 steps.
+usage: python tests/golden/make_golden.py [case ...]     (default: all)
 """
 import json
 import os
@@ -57,15 +66,94 @@ def case(name, p, st, steps, nranks):
     print("wrote", name, {k: v.shape for k, v in out.items()})
 
 
+def read_slices(run_dir, p, nsplit):
+    """The reference's 2D{nunit2D}{rank}.dat files of a run: float32 array
+    [slice, rank, j, i, 17] (each rank's local k = 1 plane, one record of
+    17 values per cell and j)."""
+    nx, ny, nz = p.n
+    cx, cy = nx // nsplit[0], ny // nsplit[1]
+    nr = nsplit[0] * nsplit[1] * nsplit[2]
+    out = []
+    u = 0
+    while os.path.exists(os.path.join(run_dir, f"2D{u:03d}000.dat")):
+        ranks = []
+        for r in range(nr):
+            raw = np.fromfile(os.path.join(run_dir, f"2D{u:03d}{r:03d}.dat"), dtype="<u4")
+            rec = 2 + 17 * cx
+            assert raw.size == cy * rec, (raw.size, cy, rec)
+            body = raw.reshape(cy, rec)[:, 1:-1].copy().view("<f4")
+            ranks.append(body.reshape(cy, cx, 17))
+        out.append(np.stack(ranks))
+        u += 1
+    return np.stack(out)
+
+
+def converging_24():
+    # the reference hard-wires the inflow composition and pressure (the WNM
+    # state of M3:155-157: n = 1.11 cm^-3, p/kb = 8810.8 K cm^-3), which
+    # INPUT3D.DAT cannot change, so the fixture keeps them
+    p = ics.converging_flow_input(24)
+    base = Input3D()
+    for k in ("Hini", "pini", "H2ini", "Heini", "Hepini", "Cini", "COini", "Cpini", "pinit_over_kb"):
+        setattr(p, k, getattr(base, k))
+    st = ics.converging_flow_state(p)
+    k, nranks = 8, 8
+    r = ref_runner.run_reference(p, st, k, nranks=nranks, tag="golden_converging_24")
+    assert len(r["dts"]) == k
+    with open(os.path.join(r["run_dir"], "cdnt.DAT")) as f:
+        cdnt = f.read()
+    with open(os.path.join(r["run_dir"], "tsave2D.DAT")) as f:
+        t2d = f.read()
+    meta = {"params": p.__dict__, "nranks": nranks, "nsplit": NSPLIT_TABLE[nranks], "steps": [k]}
+    np.savez_compressed(os.path.join(HERE, "ref_converging_24.npz"), meta=json.dumps(meta),
+                        input=crop(st, p), **{f"out{k}": crop(r["state"], p), f"dts{k}": np.array(r["dts"]),
+                                             f"time{k}": np.array(r["time"])},
+                        slices=read_slices(r["run_dir"], p, NSPLIT_TABLE[nranks]),
+                        cdnt=np.array(cdnt), tsave2d=np.array(t2d))
+    print("wrote converging_24")
+
+
+def converging_dtf_24():
+    """The same state with the inflow density read from the DTF field (the
+    initial density as REAL*4, as INITIA's nunit = 1 path uses it, M3:437)
+    and shifted by -v t (M3:818-826, 854-861): the build that does not skip
+    the DTF read."""
+    p = ics.converging_flow_input(24)
+    base = Input3D()
+    for k in ("Hini", "pini", "H2ini", "Heini", "Hepini", "Cini", "COini", "Cpini", "pinit_over_kb"):
+        setattr(p, k, getattr(base, k))
+    st = ics.converging_flow_state(p)
+    nx, ny, nz = p.n
+    dtf = st[0, 2:nz + 2, 2:ny + 2, 2:nx + 2].astype(np.float32)
+    k, nranks = 8, 8
+    r = ref_runner.run_reference(p, st, k, nranks=nranks, tag="golden_converging_dtf_24", dtf=dtf)
+    assert len(r["dts"]) == k
+    meta = {"params": p.__dict__, "nranks": nranks, "nsplit": NSPLIT_TABLE[nranks], "steps": [k]}
+    np.savez_compressed(os.path.join(HERE, "ref_converging_dtf_24.npz"), meta=json.dumps(meta),
+                        input=crop(st, p), dtf=dtf,
+                        **{f"out{k}": crop(r["state"], p), f"dts{k}": np.array(r["dts"]),
+                           f"time{k}": np.array(r["time"])})
+    print("wrote converging_dtf_24")
+
+
 def main():
-    p = small_input(24, periodic=False)
-    case("inflow_24x8x8", p, ics.random_state(p, seed=3), (1, 6), 8)
-    p = small_input(24, periodic=True)
-    case("periodic_24x8x8", p, ics.random_state(p, seed=5), (1, 6), 8)
-    p = small_input(48, periodic=False)
-    case("inflow_48x8x8", p, ics.random_state(p, seed=7), (6,), 16)
-    p = ics.brio_wu_input(64, ny=8)
-    case("briowu_64x8x8", p, ics.brio_wu_state(p), (200,), 8)
+    cases = sys.argv[1:]
+    if not cases or "inflow_24x8x8" in cases:
+        p = small_input(24, periodic=False)
+        case("inflow_24x8x8", p, ics.random_state(p, seed=3), (1, 6), 8)
+    if not cases or "periodic_24x8x8" in cases:
+        p = small_input(24, periodic=True)
+        case("periodic_24x8x8", p, ics.random_state(p, seed=5), (1, 6), 8)
+    if not cases or "inflow_48x8x8" in cases:
+        p = small_input(48, periodic=False)
+        case("inflow_48x8x8", p, ics.random_state(p, seed=7), (6,), 16)
+    if not cases or "briowu_64x8x8" in cases:
+        p = ics.brio_wu_input(64, ny=8)
+        case("briowu_64x8x8", p, ics.brio_wu_state(p), (200,), 8)
+    if not cases or "converging_24" in cases:
+        converging_24()
+    if not cases or "converging_dtf_24" in cases:
+        converging_dtf_24()
 
 
 if __name__ == "__main__":

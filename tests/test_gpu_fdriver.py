@@ -1,10 +1,12 @@
 """The Fortran EVOLVE driver (code_tohoku_amd/fdriver) on the GPU: the
-reference's own inputs (INPUT3D.DAT, tsave.DAT, per-rank restart files of an
-8-rank run) in, the reference's restart files out, compared with what the
+reference's own inputs (INPUT3D.DAT, tsave.DAT, tsave2D.DAT, per-rank restart
+files of an 8-rank run) in, the reference's files out -- restart files, 2D
+slices (SAVEU2D), tsave2D.DAT and cdnt.DAT -- compared with what the
 reference wrote for the same run."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from code_tohoku_amd import build
@@ -38,3 +40,46 @@ def test_fortran_driver_matches_reference(tmp_path):
     with open(os.path.join(run, "tsave.DAT")) as f:
         t_end = float(f.readline().replace("D", "E"))
     assert abs(t_end - float(arr["time6"])) <= 1e-13 * t_end
+
+
+def test_fortran_driver_slices_and_cdnt(tmp_path):
+    """8 steps of the converging flow at 24^3 (8 ranks): the restart files
+    at the end, every 2D slice SAVEU2D wrote on the way (each rank's k = 1
+    plane in FP32, Bcc in place of the face field), the slice counter and
+    time in tsave2D.DAT and the cell centres in cdnt.DAT."""
+    p, nsplit, arr = load_case("converging_24")
+    run = str(tmp_path)
+    q = rs.Input3D(**p.__dict__)
+    q.maxstp, q.nitera = 1, 8
+    with open(os.path.join(run, "INPUT3D.DAT"), "w") as f:
+        f.write(q.to_text())
+    rs.write_counters(run, 0.0, 2)
+    for r, b in enumerate(rs.split_state(expand(arr["input"], p), nsplit)):
+        rs.write_restart_rank(os.path.join(run, f"000{r:03d}.dat"), b)
+    res = subprocess.run([build.EVOLVE, run, "8"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    nx, ny, nz = p.n
+    cx, cy, cz = nx // nsplit[0], ny // nsplit[1], nz // nsplit[2]
+    blocks = [rs.read_restart_rank(os.path.join(run, f"000{r:03d}.dat"), cx, cy, cz)[0] for r in range(8)]
+    out = rs.assemble_state(blocks, p.n, nsplit)
+    errs = rel_linf(interior(out, p), interior(expand(arr["out8"], p), p))
+    assert worst(errs) <= 1e-11, errs
+    with open(os.path.join(run, "cdnt.DAT")) as f:
+        assert f.read() == str(arr["cdnt"])
+    with open(os.path.join(run, "tsave2D.DAT")) as f:
+        got2d = f.read().split()
+    want2d = str(arr["tsave2d"]).split()
+    assert int(got2d[0]) == int(want2d[0])
+    assert abs(float(got2d[1]) - float(want2d[1])) <= 1e-13 * float(want2d[1])
+    ref = arr["slices"]
+    for u in range(ref.shape[0]):
+        for r in range(ref.shape[1]):
+            raw = np.fromfile(os.path.join(run, f"2D{u:03d}{r:03d}.dat"), dtype="<u4")
+            rec = 2 + 17 * cx
+            assert raw.size == cy * rec
+            got = raw.reshape(cy, rec)[:, 1:-1].copy().view("<f4").reshape(cy, cx, 17)
+            # the FP64 states agree to 1e-12, so the FP32 values agree to one
+            # rounding: 2 ulp of each field's largest value
+            scale = np.abs(ref[u, :, :, :, :]).max(axis=(0, 1, 2))
+            assert np.all(np.abs(got - ref[u, r]) <= 2.0 ** -22 * scale), (u, r)
+    assert not os.path.exists(os.path.join(run, f"2D{ref.shape[0]:03d}000.dat"))

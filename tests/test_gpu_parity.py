@@ -31,7 +31,7 @@ def small(nx=32, ny=16, nz=16, periodic=False):
 
 
 @pytest.mark.parametrize("name,steps", [("inflow_24x8x8", 1), ("periodic_24x8x8", 1), ("inflow_24x8x8", 6),
-                                        ("periodic_24x8x8", 6), ("inflow_48x8x8", 6)])
+                                        ("periodic_24x8x8", 6), ("inflow_48x8x8", 6), ("converging_24", 8)])
 def test_reference_fixture(name, steps):
     p, nsplit, arr = load_case(name)
     g = mcf.Domain(p, nsplit=nsplit)
@@ -47,6 +47,40 @@ def test_reference_fixture(name, steps):
     assert worst(errs) <= tol, errs
     for a, b in zip(faces(got, p), faces(ref, p)):
         assert np.abs(a - b).max() <= tol * np.abs(b).max()
+
+
+def test_dtf_inflow_fixture():
+    """Inflow density read from the DTF field shifted by -v t (the reference
+    build with the DTF read enabled): 8 steps against the reference."""
+    p, nsplit, arr = load_case("converging_dtf_24")
+    g = mcf.Domain(p, nsplit=nsplit)
+    g.upload(expand(arr["input"], p))
+    g.set_inflow_density(arr["dtf"])
+    dts = g.run(8)
+    np.testing.assert_allclose(dts, arr["dts8"], rtol=1e-13, atol=0)
+    errs = rel_linf(interior(g.download(), p), interior(expand(arr["out8"], p), p))
+    print("dtf fixture worst", worst(errs))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
+def test_dtf_inflow_wrapped_shift_vs_oracle():
+    """The DTF shift at late times: started at t = 7.3 the inflowing gas has
+    moved 76 pc, so the ghost density wraps the 20 pc field several times;
+    two steps at 32^3 (block split 2) against the oracle."""
+    p = ics.converging_flow_input(32)
+    st = ics.converging_flow_state(p)
+    dtf = ics.inflow_density_field(st, p)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2), time=7.3)
+    o.set_dtf(dtf)
+    g = mcf.Domain(p, nsplit=(2, 2, 2))
+    g.upload(st)
+    g.set_time(7.3)
+    g.set_inflow_density(dtf)
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2, tsave=7.5)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    assert worst(errs) <= MULTI_TOL, errs
 
 
 def test_brio_wu_200_steps_bitexact():

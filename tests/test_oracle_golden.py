@@ -7,7 +7,8 @@ routine of the step that the fixtures exercise: SOURCE with KI cooling
 (explicit and implicit branches), RHS/RIEMAN, MOC/CHAREQ with the
 rank-boundary ghost semantics, REMAP, the CT family, DISSIP and the dt
 reduction, in both x-inflow and periodic geometries and for the 8- and
-16-rank block splits.
+16-rank block splits, on seeded random states and on the converging-flow
+initial state (Kolmogorov density, tanh inflow profile, B at 45 deg).
 """
 import numpy as np
 import pytest
@@ -16,7 +17,7 @@ import oracle
 from helpers import expand, faces, interior, load_case, rel_linf, worst
 
 CASES = [("inflow_24x8x8", 1), ("inflow_24x8x8", 6), ("periodic_24x8x8", 1), ("periodic_24x8x8", 6),
-         ("inflow_48x8x8", 6), ("briowu_64x8x8", 200)]
+         ("inflow_48x8x8", 6), ("briowu_64x8x8", 200), ("converging_24", 8)]
 
 
 @pytest.mark.parametrize("name,steps", CASES)
@@ -62,3 +63,22 @@ def test_oracle_thread_count_independent(name, steps):
         np.testing.assert_array_equal(dts, arr[f"dts{steps}"])
         outs.append(o.state())
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_oracle_dtf_inflow_matches_reference():
+    """With the DTF read enabled (oracle/patch_ref.py, option dtf) the
+    inflow ghost density is the DTF field shifted by -v t (M3:818-826,
+    854-861): the oracle given the same field reproduces the reference bit
+    for bit, and the uniform inflow density does not."""
+    p, nsplit, arr = load_case("converging_dtf_24")
+    for use, exact in ((True, True), (False, False)):
+        o = oracle.Oracle(p, expand(arr["input"], p), nsplit=nsplit)
+        if use:
+            o.set_dtf(arr["dtf"])
+        dts, _ = o.run(8)
+        errs = rel_linf(interior(o.state(), p), interior(expand(arr["out8"], p), p))
+        if exact:
+            np.testing.assert_array_equal(dts, arr["dts8"])
+            assert worst(errs) == 0.0, errs
+        else:
+            assert worst(errs) > 1e-6
