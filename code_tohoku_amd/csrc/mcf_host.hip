@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,35 @@ static thread_local std::string g_err;
   } while (0)
 
 enum { F_RHO = 0, F_VX, F_VY, F_VZ, F_P, F_BX, F_BY, F_BZ };
+
+// Device scratch of mcf_cool_solve_noalloc: the fallback counter pair in the
+// first 256 bytes, then the queue of cells the surrogate rejected.  Each GPU
+// holds its own copy until the process ends.  Host code never passes this
+// name to a kernel: the device address is taken once per device with
+// hipGetSymbolAddress (noalloc_scratch) and only that address is used.
+static constexpr size_t NOALLOC_HDR = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_HDR + (size_t)MCF_NOALLOC_MAX_CELLS * sizeof(int)];
+static constexpr int NOALLOC_MAX_DEV = 64;
+static void *g_scratch_ptr[NOALLOC_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on device dev (current), looked up on the
+// first call for that device.
+static int noalloc_scratch(int dev, void **ptr) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) { g_err = "device ordinal out of range for the no-allocation scratch"; return MCF_E_ARG; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return MCF_E_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return MCF_OK;
+}
 
 struct mcf_ctx {
   mcf_config cfg;
@@ -73,6 +103,7 @@ struct mcf_ctx {
   unsigned long long *cool_cnt = nullptr;
   bool mlp_ready = false;
   int src_pair = 0; // which of the two SOURCE queue-counter pairs the next stage counts in
+  void *scratch = nullptr; // this device's address of the no-allocation scratch
   long adam_t = 0;
   struct Rec { const char *name; const char *stage; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
@@ -770,6 +801,10 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   hipError_t e = hipGetDevice(&x->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { g_err = std::string("HIP init: ") + hipGetErrorString(e); delete x; return MCF_E_HIP; }
+  {
+    const int r = noalloc_scratch(x->dev, &x->scratch);
+    if (r != MCF_OK) { (void)hipStreamDestroy(x->stream); delete x; return r; }
+  }
   const int nstate = 19, nbuf = 13, nsweep = 8, nscratch = 9;
   size_t nc = x->ncell;
   size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
@@ -1312,6 +1347,34 @@ extern "C" int mcf_cool_solve(mcf_ctx *x, const double *P0, const double *nn, co
   if (ms) *ms = t;
   (void)hipEventDestroy(ea);
   (void)hipEventDestroy(eb);
+  return MCF_OK;
+}
+
+// mcf_cool_solve's kernels on caller-owned device arrays, with the counter
+// pair and the fallback queue in this device's scratch; no allocation, no
+// synchronisation.  The counters are cleared by a kernel (k_zero_u32x2), so
+// the call can be captured into a HIP graph.
+extern "C" int mcf_cool_solve_noalloc(mcf_ctx *x, const double *P0, const double *nn, const double *dt,
+                                      const double *rho, const double *gammi1, int64_t n, int mode, double *P,
+                                      int32_t *flag) {
+  if (!x || !P0 || !nn || !dt || !rho || !gammi1 || !P || (mode != 0 && mode != 1)) {
+    g_err = "bad argument";
+    return MCF_E_ARG;
+  }
+  if (n <= 0 || n > (int64_t)MCF_NOALLOC_MAX_CELLS) {
+    g_err = "ncell must be in 1..MCF_NOALLOC_MAX_CELLS (2097152) for mcf_cool_solve_noalloc";
+    return MCF_E_ARG;
+  }
+  if (mode == 1 && !x->mlp_ready) { g_err = "no weights loaded"; return MCF_E_STATE; }
+  if (!x->scratch) { g_err = "no-allocation scratch not set up"; return MCF_E_STATE; }
+  unsigned *nfb = reinterpret_cast<unsigned *>(x->scratch);
+  int *fq = reinterpret_cast<int *>(static_cast<unsigned char *>(x->scratch) + NOALLOC_HDR);
+  hipLaunchKernelGGL(k_zero_u32x2, dim3(1), dim3(64), 0, x->stream, nfb);
+  hipLaunchKernelGGL(k_cool_solve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, x->stream, P0, nn, dt, rho,
+                     gammi1, (long)n, mode, x->mlp_pk, x->mlp, P, flag, nfb, fq);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_cool_fallback, dim3(256), dim3(256), 0, x->stream, P0, nn, dt, rho, gammi1, nfb, fq, P);
+  HIPCHK(hipGetLastError());
   return MCF_OK;
 }
 

@@ -99,12 +99,13 @@ def lib():
         L.mcf_cool_samples.argtypes = [P, C.c_int64, C.c_uint64, pf, pf]
         L.mcf_cool_solve.argtypes = [P, pd, pd, pd, pd, pd, C.c_int64, I, pd, pi32, pd]
         L.mcf_cool_stats.argtypes = [P, pi64, I]
+        L.mcf_cool_solve_noalloc.argtypes = [P, P, P, P, P, P, C.c_int64, I, P, P]
         for nm in ("mcf_init", "mcf_finalize", "mcf_upload", "mcf_download", "mcf_download_bcc",
                    "mcf_set_time", "mcf_set_inflow_density", "mcf_get_time", "mcf_timestep", "mcf_source", "mcf_sweep",
                    "mcf_ct", "mcf_dissip", "mcf_step", "mcf_run", "mcf_sync", "mcf_profile",
                    "mcf_profile_report", "mcf_profile_stages", "mcf_copy_probe", "mcf_mlp_init", "mcf_mlp_set_weights",
                    "mcf_mlp_get_weights", "mcf_mlp_train", "mcf_mlp_forward", "mcf_mlp_grad",
-                   "mcf_cool_samples", "mcf_cool_solve", "mcf_cool_stats"):
+                   "mcf_cool_samples", "mcf_cool_solve", "mcf_cool_stats", "mcf_cool_solve_noalloc"):
             getattr(L, nm).restype = C.c_int
         _lib = L
     return _lib

@@ -185,6 +185,22 @@ int mcf_cool_samples(mcf_ctx *ctx, int64_t n, uint64_t seed, float *X, float *Y)
  * *ms = kernel time. */
 int mcf_cool_solve(mcf_ctx *ctx, const double *P0, const double *n, const double *dt, const double *rho,
                    const double *gammi1, int64_t ncell, int mode, double *P, int32_t *flag, double *ms);
+/* Allocation-free variant of mcf_cool_solve.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * All array arguments are DEVICE pointers: P0, n, dt, rho, gammi1 (ncell
+ * doubles each) in, P (ncell doubles) and flag (ncell int32, or NULL) out.
+ * The call only enqueues kernels on the context's stream (mcf_stream) and
+ * returns; it calls no hipMalloc, hipFree or synchronisation.  The queue of
+ * cells the surrogate rejects lives in a library-owned device array of
+ * MCF_NOALLOC_MAX_CELLS entries (8 MiB + 256 B on each GPU the library
+ * runs on, held until the process ends); ncell above that is refused with
+ * MCF_E_ARG.  Every call on one GPU shares that array and the last call's
+ * contents stay in it: calls that overlap on one GPU (two streams, two
+ * contexts) race. */
+#define MCF_NOALLOC_MAX_CELLS 2097152
+int mcf_cool_solve_noalloc(mcf_ctx *ctx, const double *P0, const double *n, const double *dt, const double *rho,
+                           const double *gammi1, int64_t ncell, int mode, double *P, int32_t *flag);
 /* counts[0] = backward-Euler cells solved in SOURCE since the last reset
  * (both cooling modes), counts[1] = of those, the cells the surrogate
  * rejected and IMC solved (surrogate mode). */
