@@ -93,9 +93,13 @@ struct mcf_ctx {
   // per-kernel event timing (mcf_profile)
   bool prof = false;
   bool mixed = false; // MCF_PREC_MIXED: FP32 reconstruction / Riemann / EMF
-  // y/z sweeps run RHS and MOC as one column march (k_rhsmoc_col): needs a
-  // uniform spacing along the axis (ghosts included) and n >= 8
+  // y/z sweeps run as column marches: needs a uniform spacing along the
+  // axis (ghosts included) and n >= 8
   bool fuse[3]{false, false, false};
+  // cells per segment of k_sweep_col: 256 (the whole 256^3 column) measured
+  // 1.66 / 2.32 ms per y sweep (mixed / FP64) against 1.68 / 2.37 with 128,
+  // 1.92 / 2.70 with 96 and 2.48 / 3.57 with 192 (an uneven split)
+  int col_seg = 256;
   // cooling surrogate: FP32 weights, Adam moments, gradient (4 x MLP_NP),
   // packed BF16 operands, counters {backward-Euler cells, IMC fallbacks}
   float *mlp = nullptr;
@@ -542,9 +546,31 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     for (int f = 0; f < 5; ++f) swapU(x, f);
     swapSp(x); swapBlg(x);
     dxlag_fill = true; // gathered with REMAP's fills below
+  } else if (x->fuse[d] && x->g.n[d] <= SWEEP_COL_MAXN) {
+    // y/z: the whole sweep in one column march over the periodic axis; it
+    // reads only interior cells (periodic images), so no fills precede it.
+    // One kernel instead of k_rhsmoc_col + k_remap_col (the MOC results no
+    // longer go through HBM): 1.66 against 1.93 ms per sweep (mixed), 2.32
+    // against 2.75 (FP64), at 1 wave per SIMD (354 / 450 registers)
+    const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
+    const int seg = x->col_seg;
+    dim3 gr((unsigned)((n + seg - 1) / seg), (unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4));
+    // reads rho, v(3), p, Bcc(3), face B_n, species(8); writes rho, v(3), p, species(8)
+    const double by = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * 30;
+    if (x->mixed) {
+      if (d == 1) KLAUNCH_N("k_sweep_col<1>", (k_sweep_col<1, float>), gr, dim3(256), by, A, x->g, dt, seg);
+      else KLAUNCH_N("k_sweep_col<2>", (k_sweep_col<2, float>), gr, dim3(256), by, A, x->g, dt, seg);
+    } else {
+      if (d == 1) KLAUNCH_N("k_sweep_col<1>", (k_sweep_col<1, double>), gr, dim3(256), by, A, x->g, dt, seg);
+      else KLAUNCH_N("k_sweep_col<2>", (k_sweep_col<2, double>), gr, dim3(256), by, A, x->g, dt, seg);
+    }
+    for (int f = 0; f < 5; ++f) swapU(x, f);
+    swapSp(x);
+    return MCF_OK;
   } else if (x->fuse[d]) {
-    // y/z: RHS and MOC in one column march over the periodic axis; it reads
-    // only interior cells (periodic images), so no fills precede it
+    // y/z axes longer than k_sweep_col's LDS copy of the cell edges: RHS and
+    // MOC in one column march over the periodic axis (it reads only interior
+    // cells, periodic images, so no fills precede it), then REMAP
     const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
     dim3 gr((unsigned)((n + FUSE_SEG - 1) / FUSE_SEG), (unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4));
     // algorithmic bytes: 14 arrays read, 9 written (RHS+MOC); 18 read, 13
@@ -846,6 +872,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
     if (d == 0 && n[0] >= 8 && cfg->bcx1 == 4 && cfg->bcx2 == 4) uni = true;
     x->fuse[d] = uni;
   }
+
   for (int d = 0; d < 3; ++d) {
     INITCHK(hipMemcpyAsync(x->gx[d], x->hx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));
     INITCHK(hipMemcpyAsync(x->gdx[d], x->hdx[d].data(), (n[d] + 4) * sizeof(double), hipMemcpyHostToDevice, x->stream));

@@ -180,6 +180,22 @@ def test_multistep_vs_oracle_long_columns():
     assert worst(errs) <= MULTI_TOL, errs
 
 
+def test_axis_longer_than_sweep_col_uses_split_column_kernels():
+    """y longer than the fused column sweep takes (2048 cells): the y sweep
+    runs as k_rhsmoc_col + k_remap_col; two FP64 steps against the oracle."""
+    p = Input3D(np1x=8, np2x=8, np1y=1028, np2y=1028, np1z=4, np2z=4,
+                ql1x=5.0, ql2x=5.0, ql1y=642.5, ql2y=642.5, ql1z=2.5, ql2z=2.5)
+    st = ics.random_state(p, seed=17)
+    o = oracle.Oracle(p, st, nsplit=(1, 8, 2))
+    g = mcf.Domain(p, nsplit=(1, 8, 2))
+    g.upload(st)
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
 @pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
 def test_multistep_vs_oracle_long_z(precision):
     """z longer than one column segment (128 cells) and than SOURCE's
