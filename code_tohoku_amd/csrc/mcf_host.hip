@@ -372,6 +372,11 @@ static void planU(mcf_ctx *x, FillPlan &P, int dmask, int mode, const int *list,
       X.a[u] = a;
       for (int s = 0; s < 2; ++s) {
         int side = s + 1;
+        if (bc[s] == 1) { // outflow: zero-gradient copy; face B_x as the inflow rule
+          if (K == 6) X.rule[u][s] = XR_FACE;
+          else { X.rule[u][s] = XR_COPY; X.src[u][s] = s == 0 ? 1 : x->g.n[0]; }
+          continue;
+        }
         if (bc[s] != 4) { X.rule[u][s] = XR_PERIODIC; continue; }
         if (K == 1) { X.rule[u][s] = x->dtfd ? XR_DTF : XR_CONST; X.val[u][s] = x->dtf; }
         else if (K == 6) X.rule[u][s] = XR_FACE;
@@ -398,6 +403,7 @@ static void planOT(mcf_ctx *x, FillPlan &P, int dmask, int what) {
       X.a[u] = a[t];
       for (int s = 0; s < 2; ++s) {
         int side = s + 1;
+        if (bc[s] == 1) { X.rule[u][s] = XR_COPY; X.src[u][s] = s == 0 ? 1 : nx; continue; }
         if (bc[s] != 4) { X.rule[u][s] = XR_PERIODIC; continue; }
         switch (what) {
         case OT_DXLAG: X.rule[u][s] = XR_CONST; X.val[u][s] = x->hdx[0][s == 0 ? 1 : nx + 2]; break;
@@ -799,12 +805,14 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
     if (nsp[d] < 1 || n[d] % nsp[d]) { g_err = "nsplit must divide the cell count"; return MCF_E_ARG; }
   }
   if (cfg->np1x < 1 || cfg->np2x < 0 || cfg->np1y < 1 || cfg->np1z < 1) { g_err = "bad cell counts"; return MCF_E_ARG; }
-  if ((cfg->bcx1 != 3 && cfg->bcx1 != 4) || (cfg->bcx2 != 3 && cfg->bcx2 != 4)) {
-    g_err = "x boundaries must be 3 (periodic) or 4 (inflow)"; return MCF_E_ARG;
+  if ((cfg->bcx1 != 1 && cfg->bcx1 != 3 && cfg->bcx1 != 4) || (cfg->bcx2 != 1 && cfg->bcx2 != 3 && cfg->bcx2 != 4)) {
+    g_err = "x boundaries must be 1 (outflow), 3 (periodic) or 4 (inflow)"; return MCF_E_ARG;
   }
   // an inflow end opposite a periodic end is no configuration of the
   // reference (INPUT3D.DAT: 4,4) and its parity is not established
-  if (cfg->bcx1 != cfg->bcx2) { g_err = "x boundaries must be both inflow (4) or both periodic (3)"; return MCF_E_ARG; }
+  if (cfg->bcx1 != cfg->bcx2) {
+    g_err = "x boundaries must be alike: both outflow (1), both periodic (3) or both inflow (4)"; return MCF_E_ARG;
+  }
   // y and z are always periodic in the reference (BC_MPI, M3:888-926)
   if (cfg->bcy1 != 3 || cfg->bcy2 != 3 || cfg->bcz1 != 3 || cfg->bcz2 != 3) {
     g_err = "y and z boundaries must be 3 (periodic)"; return MCF_E_ARG;
@@ -870,6 +878,8 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
     for (int k = 0; k < n[d] + 4 && uni; ++k) uni = std::memcmp(&x->hdx[d][k], &x->hdx[d][0], sizeof(double)) == 0;
     // x with inflow on both sides needs no periodic images, so no uniform spacing
     if (d == 0 && n[0] >= 8 && cfg->bcx1 == 4 && cfg->bcx2 == 4) uni = true;
+    // an outflow x sweep runs as the stage kernels with the copy fills between
+    if (d == 0 && cfg->bcx1 == 1) uni = false;
     x->fuse[d] = uni;
   }
 

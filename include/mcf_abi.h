@@ -46,7 +46,9 @@ typedef struct mcf_config {
   double cfl, facdep;
   int maxstp, nitera;
   double tfinal;
-  int bcx1, bcx2, bcy1, bcy2, bcz1, bcz2;   /* 3 periodic, 4 inflow (x only, both ends alike) */
+  /* x: 1 outflow (zero-gradient copy, the 1D code's BC=1), 3 periodic,
+   * 4 inflow; both x ends alike.  y, z: 3 (periodic) only. */
+  int bcx1, bcx2, bcy1, bcy2, bcz1, bcz2;
   int ifchem, ifthrm, ifrad, ifgrv;          /* ifthrm = 2: cooling+conduction */
   double pinit_over_kb;                      /* 8.810807d3 */
   double Hini, pini, H2ini, Heini, Hepini, Cini, COini, Cpini;

@@ -333,15 +333,34 @@ static void xrule_U(mcfo *o, int K, int side, int nol, int mode) {
       }
 }
 
+/* Outflow (free) x boundary, BC = 1: the zero-gradient rule of the 1D code
+ * (1DMHDKIcool/1DMHDwCC3KIcool.f90:352-372), Q(0) = Q(-1) = Q(1) and
+ * Q(N+1) = Q(N+2) = Q(N), for every cell-centred array; the face field
+ * U(:,:,:,6) keeps the divergence-free copy of the inflow rule (left ghosts
+ * take face 1, only N+2 takes face N+1: face N+1 is a boundary face CT
+ * updates).  The 3D reference has no BC = 1; this is the build's
+ * extension of it. */
+static void outflow_copy(mcfo *o, double *a, int side, int nol, int face) {
+  int nx = o->n[0];
+  int lo = (side == 1) ? 1 - nol : nx + 1, hi = (side == 1) ? 0 : nx + nol;
+  if (face && side == 2) lo = nx + 2;
+  int src = (side == 1) ? 1 : (face ? nx + 1 : nx);
+  for (int k = -1; k <= o->n[2] + 2; k++)
+    for (int j = -1; j <= o->n[1] + 2; j++)
+      for (int i = lo; i <= hi; i++) a[ix3(o, i, j, k)] = a[ix3(o, src, j, k)];
+}
+
 /* BC_MPI(N_ol,mode) for the listed U components along the axes in dmask. */
 static void fillU(mcfo *o, int dmask, int nol, int mode, const int *list, int nk) {
   if (dmask & 1) {
     for (int t = 0; t < nk; t++) {
       if (o->p.bcx1 == 4) xrule_U(o, list[t], 1, nol, mode);
+      else if (o->p.bcx1 == 1) outflow_copy(o, U(list[t]), 1, nol, list[t] == 6);
       else periodic(o, U(list[t]), 0, nol, 1);
     }
     for (int t = 0; t < nk; t++) {
       if (o->p.bcx2 == 4) xrule_U(o, list[t], 2, nol, mode);
+      else if (o->p.bcx2 == 1) outflow_copy(o, U(list[t]), 2, nol, list[t] == 6);
       else periodic(o, U(list[t]), 0, nol, 0);
     }
   }
@@ -371,6 +390,21 @@ static void xrule_copy(mcfo *o, double *a, int side, int nol, int src) {
 /* BC_MPI_OT pieces.  what: 1 = dxlagM, 2 = Bcc, 3 = Blg, 4 = Vfc, 5 = species. */
 static void fill_ot(mcfo *o, int dmask, int nol, int what) {
   int nx = o->n[0];
+  if ((dmask & 1) && (o->p.bcx1 == 1 || o->p.bcx2 == 1)) {
+    /* outflow: every BC_MPI_OT array takes the zero-gradient copy */
+    double *arr[8];
+    int na = 0;
+    if (what == 1) arr[na++] = o->dxlagM;
+    if (what == 2) for (int K = 1; K <= 3; K++) arr[na++] = BCC(K);
+    if (what == 3) { arr[na++] = o->blg[0]; arr[na++] = o->blg[1]; }
+    if (what == 4) { arr[na++] = o->vfc[1]; arr[na++] = o->vfc[2]; }
+    if (what == 5) for (int sp = 0; sp < 8; sp++) arr[na++] = SPC(sp);
+    for (int t = 0; t < na; t++) {
+      if (o->p.bcx1 == 1) outflow_copy(o, arr[t], 1, nol, 0); else periodic(o, arr[t], 0, nol, 1);
+      if (o->p.bcx2 == 1) outflow_copy(o, arr[t], 2, nol, 0); else periodic(o, arr[t], 0, nol, 0);
+    }
+    dmask &= ~1;
+  }
   if (dmask & 1) {
     int b1 = (o->p.bcx1 == 4), b2 = (o->p.bcx2 == 4);
     if (what == 1) {

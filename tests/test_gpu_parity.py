@@ -49,6 +49,52 @@ def test_reference_fixture(name, steps):
         assert np.abs(a - b).max() <= tol * np.abs(b).max()
 
 
+@pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
+def test_outflow_brio_wu_vs_oracle(precision):
+    """Outflow x ends (BC = 1): the Brio-Wu tube at 64x8x8 for 60 steps,
+    with the fast waves through both ends, against the oracle (FP64 dt and
+    fields at 1e-12; mixed precision on the oracle's dt sequence at 1e-5)."""
+    p = ics.brio_wu_input(64, ny=8, periodic_x=False)
+    p.bcx1 = p.bcx2 = 1
+    st = ics.brio_wu_state(p)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    dts_o, _ = o.run(60)
+    g = mcf.Domain(p, nsplit=(2, 2, 2), precision=precision)
+    g.upload(st)
+    if precision == mcf.PREC_FP64:
+        dts_g = g.run(60)
+        np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13, atol=0)
+        tol = STEP_TOL
+    else:
+        for dt in dts_o:
+            g.step(dt)
+        tol = 1e-5
+    errs = rel_linf(interior(g.download(), p)[:8], interior(o.state(), p)[:8])
+    print("outflow Brio-Wu worst", worst(errs))
+    assert worst(errs) <= tol, errs
+
+
+def test_outflow_config1_64cubed_planar():
+    """Config 1 (64^3 Brio-Wu, outflow x, periodic y/z, 200 steps) on the
+    GPU: the dt sequence and every row agree with the 64x8x8 run, which the
+    oracle pins (test_outflow_brio_wu_vs_oracle)."""
+    out = {}
+    for ny in (8, 64):
+        p = ics.brio_wu_input(64, ny=ny, periodic_x=False)
+        p.bcx1 = p.bcx2 = 1
+        g = mcf.Domain(p, nsplit=(2, 2, 2))
+        g.upload(ics.brio_wu_state(p))
+        dts = g.run(200, tsave=1e30)
+        assert len(dts) == 200
+        out[ny] = (dts, interior(g.download(), p)[:8])
+        g.close()
+    np.testing.assert_allclose(out[64][0], out[8][0], rtol=1e-12)
+    big, small = out[64][1], out[8][1]
+    for f in range(8):
+        scale = max(np.abs(small[f]).max(), 1e-300)
+        assert np.abs(big[f] - small[f][:1, :1, :]).max() <= 1e-10 * scale, f
+
+
 def test_dtf_inflow_fixture():
     """Inflow density read from the DTF field shifted by -v t (the reference
     build with the DTF read enabled): 8 steps against the reference."""
