@@ -86,6 +86,12 @@ struct mcf_ctx {
   double bbrv_cm[8]{};
   double dtf = 0, pmin = 0, pmax = 0, rmin = 0, spmin[8]{};
   float *dtfd = nullptr; // DTF field on the device (mcf_set_inflow_density), null = uniform dtf
+  // column densities Ntot, NH2, NnC, NCO, tCII from the left / right x end
+  // (col[2q + dir]) and SHIELD's per-cell scratch; allocated when the
+  // chemistry Fcool, the network or SHIELD is on
+  double *col[10]{}, *ctemp = nullptr;
+  bool chem = false;    // MCF_COOL_CHEM: the chemistry-network Fcool
+  bool has_cols = false;
   int nsp[3]{1, 1, 1};
   double time = 0.0;
   int ifevo = 1;
@@ -712,10 +718,53 @@ static int do_ct(mcf_ctx *x, double dt) {
   return do_cc4(x);
 }
 
-// SOURCE (M3:2448-2572)
+static ChemCols chem_cols(const mcf_ctx *x) {
+  ChemCols C{};
+  for (int q = 0; q < 10; ++q) C.c[q] = x->col[q];
+  return C;
+}
+
+// SHIELD (M3:2924-3036): one wave per (j,k) row
+static int do_shield(mcf_ctx *x) {
+  ShieldArgs S{};
+  for (int s = 0; s < 8; ++s) S.sp[s] = x->sp[s];
+  S.p = x->U[F_P];
+  for (int q = 0; q < 10; ++q) S.col[q] = x->col[q];
+  S.temp = x->ctemp;
+  S.gdx0 = x->gdx[0];
+  S.nsx = x->nsp[0];
+  S.cx = x->g.n[0] / x->nsp[0];
+  const long rows = (long)x->g.n[1] * x->g.n[2];
+  const double cells = (double)rows * x->g.n[0];
+  // reads 8 species, p, tCII(2) and the 8 scan inputs twice; writes temp
+  // and 10 columns
+  KLAUNCH_G(k_shield, dim3((unsigned)((rows + 3) / 4)), dim3(256), 8.0 * cells * 38, S, x->g);
+  return MCF_OK;
+}
+
+static int do_thermal(mcf_ctx *x, double dt);
+
+// SOURCE (M3:2448-2780): SHIELD (ifrad = 2), the thermal update (ifthrm =
+// 2), the chemistry network (ifchem = 2)
 static int do_source(mcf_ctx *x, double dt) {
-  if (x->cfg.ifthrm != 2) return MCF_OK;
   StageScope scope(x, "source");
+  if (x->cfg.ifrad == 2) CHK(do_shield(x));
+  if (x->cfg.ifthrm == 2) CHK(do_thermal(x, dt));
+  if (x->cfg.ifchem == 2) {
+    ChemArgs C{};
+    for (int s = 0; s < 8; ++s) { C.sp[s] = x->sp[s]; C.spmin[s] = x->spmin[s]; }
+    C.rho = x->U[F_RHO];
+    C.p = x->U[F_P];
+    C.cols = chem_cols(x);
+    const long n = (long)x->g.n[0] * x->g.n[1] * x->g.n[2];
+    KLAUNCH(k_chem, n, 26, C, x->g, dt);
+  }
+  return MCF_OK;
+}
+
+// The thermal part of SOURCE (M3:2477-2572): two-stage cooling +
+// saturated conduction
+static int do_thermal(mcf_ctx *x, double dt) {
   SrcArgs A{};
   A.rho = x->U[F_RHO];
   for (int s = 0; s < 8; ++s) A.sp[s] = x->sp[s];
@@ -725,6 +774,7 @@ static int do_source(mcf_ctx *x, double dt) {
   A.queue = reinterpret_cast<int *>(x->srcq[1]);
   A.fqueue = reinterpret_cast<int *>(x->srcq[2]);
   A.qdc = x->srcq[0];
+  A.cols = chem_cols(x);
   if (x->cfg.cooling == MCF_COOL_SURROGATE) {
     if (!x->mlp_ready) { g_err = "cooling surrogate selected but no weights loaded (mcf_mlp_set_weights / mcf_mlp_train)"; return MCF_E_STATE; }
     A.sur = 1;
@@ -762,17 +812,22 @@ static int do_source(mcf_ctx *x, double dt) {
     A.qn = reinterpret_cast<unsigned *>(x->cool_cnt + 2 + x->src_pair);
     A.qz = reinterpret_cast<unsigned *>(x->cool_cnt + 3 - x->src_pair);
     {
-      const int zseg = x->mixed ? SRC_ZSEG_F32 : SRC_ZSEG_F64;
+      // the chemistry Fcool always runs in FP64
+      const bool f32 = x->mixed && !x->chem;
+      const int zseg = f32 ? SRC_ZSEG_F32 : SRC_ZSEG_F64;
       dim3 gr((unsigned)((b.cnt[0] + 63) / 64), (unsigned)((b.cnt[1] + SRC_ROWS - 1) / SRC_ROWS), (unsigned)((b.cnt[2] + zseg - 1) / zseg));
-      if (x->mixed) KLAUNCH_N("k_src_col", k_src_col<float>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
-      else KLAUNCH_N("k_src_col", k_src_col<double>, gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
+      if (x->chem) KLAUNCH_N("k_src_col", (k_src_col<double, true>), gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 19, A, x->g, b, dt, stage);
+      else if (f32) KLAUNCH_N("k_src_col", (k_src_col<float, false>), gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
+      else KLAUNCH_N("k_src_col", (k_src_col<double, false>), gr, dim3(64 * SRC_ROWS), 8.0 * (double)boxsize(b) * 9, A, x->g, b, dt, stage);
     }
     x->src_pair ^= 1;
     if (A.sur) {
       KLAUNCH_G(k_src_stiff, dim3(1024), dim3(256), 0, A, dt, stage);
-      KLAUNCH_G(k_src_imc, dim3(256), dim3(256), 0, A, dt, stage, 1);
+      KLAUNCH_N("k_src_imc", k_src_imc<false>, dim3(256), dim3(256), 0, A, dt, stage, 1);
+    } else if (x->chem) {
+      KLAUNCH_N("k_src_imc", k_src_imc<true>, dim3(256), dim3(256), 0, A, dt, stage, 0);
     } else {
-      KLAUNCH_G(k_src_imc, dim3(256), dim3(256), 0, A, dt, stage, 0);
+      KLAUNCH_N("k_src_imc", k_src_imc<false>, dim3(256), dim3(256), 0, A, dt, stage, 0);
     }
   }
   return MCF_OK;
@@ -820,12 +875,18 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   // DISSIP works on 10-cell layers at the x ends (M3:3152-3170)
   if (n[0] / nsp[0] < 10) { g_err = "x block must hold at least 10 cells (DISSIP layers)"; return MCF_E_ARG; }
   if (cfg->precision != MCF_PREC_FP64 && cfg->precision != MCF_PREC_MIXED) { g_err = "unknown precision mode"; return MCF_E_ARG; }
-  if (cfg->cooling != MCF_COOL_KI && cfg->cooling != MCF_COOL_SURROGATE) { g_err = "unknown cooling mode"; return MCF_E_ARG; }
+  if (cfg->cooling != MCF_COOL_KI && cfg->cooling != MCF_COOL_SURROGATE && cfg->cooling != MCF_COOL_CHEM) {
+    g_err = "unknown cooling mode"; return MCF_E_ARG;
+  }
+  if (cfg->ifgrv == 2) { g_err = "self-gravity (ifgrv = 2) is not implemented"; return MCF_E_ARG; }
+  if (cfg->ifrad == 2 && nsp[0] > SHIELD_MAXB) { g_err = "SHIELD takes at most 64 x blocks"; return MCF_E_ARG; }
   long long vol = (long long)(n[0] + 4) * (n[1] + 4) * (n[2] + 4);
   if (vol >= (1LL << 31)) { g_err = "grid too large for 32-bit indexing"; return MCF_E_ARG; }
   mcf_ctx *x = new mcf_ctx();
   x->cfg = *cfg;
   x->mixed = cfg->precision == MCF_PREC_MIXED;
+  x->chem = cfg->cooling == MCF_COOL_CHEM;
+  x->has_cols = x->chem || cfg->ifchem == 2 || cfg->ifrad == 2;
   for (int d = 0; d < 3; ++d) { x->g.n[d] = n[d]; x->nsp[d] = nsp[d]; }
   x->g.st[0] = 1;
   x->g.st[1] = n[0] + 4;
@@ -844,7 +905,8 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   size_t ndivv = 2 * 12 * (size_t)(n[1] + 2) * (n[2] + 2);
   size_t ngrid = 0;
   for (int d = 0; d < 3; ++d) ngrid += 2 * (size_t)(n[d] + 4);
-  size_t total = (nstate + nbuf + nsweep + nscratch) * nc + ndivv + ngrid + 16 + 2 * DT_BLOCKS;
+  const int nchem = x->has_cols ? 11 : 0;
+  size_t total = (nstate + nbuf + nsweep + nscratch + nchem) * nc + ndivv + ngrid + 16 + 2 * DT_BLOCKS;
   e = hipMalloc(&x->pool, total * sizeof(double));
   if (e != hipSuccess) { g_err = std::string("hipMalloc: ") + hipGetErrorString(e); (void)hipStreamDestroy(x->stream); delete x; return MCF_E_NOMEM; }
   double *p = x->pool;
@@ -862,6 +924,10 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   x->divv = p; p += ndivv;
   for (int d = 0; d < 3; ++d) { x->gx[d] = p; p += n[d] + 4; x->gdx[d] = p; p += n[d] + 4; }
   x->red = p; p += 16 + 2 * DT_BLOCKS;
+  if (x->has_cols) {
+    for (int q = 0; q < 10; ++q) { x->col[q] = p; p += nc; }
+    x->ctemp = p; p += nc;
+  }
   // from here on a failure frees everything allocated so far
 #define INITCHK(call)                                                                              \
   do {                                                                                             \
@@ -924,6 +990,22 @@ extern "C" int mcf_upload(mcf_ctx *x, const double *U8, const double *sp8) {
   }
   StageScope scope(x, "upload");
   CHK(do_cc4(x)); // CC(4,0) after the restart read (M3:470)
+  // column densities and CII optical depths zeroed, then 20 SHIELD sweeps
+  // with ifrad = 2 (M3:472-478)
+  if (x->has_cols) {
+    for (int q = 0; q < 10; ++q) HIPCHK(hipMemsetAsync(x->col[q], 0, bytes, x->stream));
+    if (x->cfg.ifrad == 2)
+      for (int l = 0; l < 20; ++l) CHK(do_shield(x));
+  }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
+extern "C" int mcf_download_columns(mcf_ctx *x, double *col10) {
+  if (!x || !col10) { g_err = "null argument"; return MCF_E_ARG; }
+  if (!x->has_cols) { g_err = "no column densities: the chemistry Fcool, network and SHIELD are all off"; return MCF_E_STATE; }
+  for (int q = 0; q < 10; ++q)
+    HIPCHK(hipMemcpyAsync(col10 + q * x->ncell, x->col[q], x->ncell * sizeof(double), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return MCF_OK;
 }
@@ -998,9 +1080,11 @@ extern "C" int mcf_timestep(mcf_ctx *x, double tsave, double *dt_out, int *tag) 
   for (int c = 0; c < 3; ++c) { A.bcc[c] = x->bcc[c]; A.gdx[c] = x->gdx[c]; }
   A.out = x->red;
   A.tfinal = x->cfg.tfinal;
+  A.cols = chem_cols(x);
   long rows = (long)x->g.n[1] * x->g.n[2];
   unsigned nb = (unsigned)(rows < DT_BLOCKS ? rows : DT_BLOCKS);
-  KLAUNCH_G(k_dtmin, dim3(nb), dim3(256), 104.0 * (double)rows * x->g.n[0], A, x->g, x->red + 16);
+  if (x->chem) KLAUNCH_N("k_dtmin", k_dtmin<true>, dim3(nb), dim3(256), 184.0 * (double)rows * x->g.n[0], A, x->g, x->red + 16);
+  else KLAUNCH_N("k_dtmin", k_dtmin<false>, dim3(nb), dim3(256), 104.0 * (double)rows * x->g.n[0], A, x->g, x->red + 16);
   KLAUNCH_G(k_dtfinal, dim3(1), dim3(256), 16.0 * nb, x->red + 16, (int)nb, x->red, x->cfg.tfinal);
   HIPCHK(hipMemcpyAsync(x->hred, x->red, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -1168,7 +1252,15 @@ static double stage_bytes(const mcf_ctx *x, const std::string &st) {
   const double cells = (double)x->g.n[0] * x->g.n[1] * x->g.n[2];
   if (st.rfind("sweep_", 0) == 0) return 8.0 * 30.0 * cells;
   if (st == "ct") return 8.0 * 14.0 * cells;
-  if (st == "source") return 8.0 * 8.0 * cells;
+  if (st == "source") {
+    // thermal stage 8 arrays (+ the 10 columns the chemistry Fcool reads),
+    // SHIELD 11 read + 10 written, network 17 read + 9 written
+    double a = 0.0;
+    if (x->cfg.ifthrm == 2) a += x->chem ? 18.0 : 8.0;
+    if (x->cfg.ifrad == 2) a += 21.0;
+    if (x->cfg.ifchem == 2) a += 26.0;
+    return 8.0 * a * cells;
+  }
   if (st == "dt") return 8.0 * 13.0 * cells;
   if (st == "dissip") return 8.0 * 6.0 * 20.0 * x->g.n[1] * x->g.n[2];
   return 0.0;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "mcf_chem.h"
 #include "mcf_device.h"
 
 namespace mcf {
@@ -11,6 +12,11 @@ struct Geo {
   int n[3];
   int st[3];
 };
+
+// linear index of cell (i,j,k), i, j, k in -1..n+2
+__device__ __forceinline__ int ix3(const Geo &g, int i, int j, int k) {
+  return (i + 1) + (j + 1) * g.st[1] + (k + 1) * g.st[2];
+}
 
 // Range of (i,j,k) a kernel covers: lo .. lo+cnt-1 per axis
 struct Box {
@@ -143,6 +149,7 @@ struct SrcArgs {
   unsigned *qz;   // the other stage's pair of counters, cleared by k_src_col
   int *queue, *fqueue;
   double *qdc;
+  ChemCols cols; // column densities the chemistry Fcool reads
 };
 
 struct DtArgs {
@@ -152,6 +159,29 @@ struct DtArgs {
   const double *gdx[3];
   double *out;
   double tfinal;
+  ChemCols cols; // column densities the chemistry Fcool reads
+};
+
+// SHIELD (k_shield): species, pressure, the ten column arrays, a per-cell
+// scratch for the CII optical-depth rate, dx (dx(i) at [i+1]) and the x
+// block split of the reproduced MPI run
+struct ShieldArgs {
+  const double *sp[8];
+  const double *p;
+  double *col[10];
+  double *temp;
+  const double *gdx0;
+  int nsx, cx;
+};
+constexpr int SHIELD_MAXB = 64; // x blocks k_shield takes
+
+// chemistry network (k_chem)
+struct ChemArgs {
+  double *sp[8];
+  double *rho;
+  const double *p;
+  ChemCols cols;
+  double spmin[8];
 };
 
 struct DisArgs {
@@ -165,19 +195,22 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_cct3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
-template <class R> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);
+// CHEM: the chemistry-network Fcool (R = double only) instead of the KI one
+template <class R, bool CHEM = false> __global__ void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage);
 // planes per block of k_src_col: FP32 rates (mixed) 64, FP64 32
 constexpr int SRC_ZSEG_F32 = 64, SRC_ZSEG_F64 = 32;
 constexpr int SRC_ROWS = 8;  // x rows (waves) per block of k_src_col
 __global__ void k_shell_copy(const double *__restrict__ src, double *__restrict__ dst, Geo g);
 __global__ void k_src_stiff(SrcArgs A, double dt, int stage);
-__global__ void k_src_imc(SrcArgs A, double dt, int stage, int which);
+template <bool CHEM = false> __global__ void k_src_imc(SrcArgs A, double dt, int stage, int which);
 __global__ void k_zero_u32x2(unsigned *p);
-__global__ void k_dtmin(DtArgs A, Geo g, double *part);
+template <bool CHEM = false> __global__ void k_dtmin(DtArgs A, Geo g, double *part);
 __global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal);
 __global__ void k_dissip_div(DisArgs A, Geo g, int e0);
 __global__ void k_dissip_upd(DisArgs A, Geo g, int e0);
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n);
+__global__ void k_shield(ShieldArgs S, Geo g);
+__global__ void k_chem(ChemArgs A, Geo g, double dt);
 
 // cooling surrogate (mcf_cool.hip)
 __global__ void k_cool_samples(long n, unsigned long long seed, float *X, float *Y);

@@ -17,10 +17,6 @@ namespace mcf {
 // ---------------------------------------------------------------------------
 // index helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int ix3(const Geo &g, int i, int j, int k) {
-  return (i + 1) + (j + 1) * g.st[1] + (k + 1) * g.st[2];
-}
-
 // thread -> (i,j,k) of a box, x fastest; false when out of range
 __device__ __forceinline__ bool box_ijk(const Box &b, int &i, int &j, int &k) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -407,8 +403,9 @@ __device__ __forceinline__ float qflux_f(double Tc, double Tcs, double dxi, doub
   return (float)KCOND * fminf(rt, 3.873f) * (float)(Tcs - Tc) * Ar<float>::div(2.0f, (float)(dxi + dxp));
 }
 
-template <class R>
+template <class R, bool CHEM>
 __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box b, double dt, int stage) {
+  static_assert(!CHEM || Ar<R>::strict, "the chemistry Fcool runs in FP64");
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   // the queue counters alternate between two pairs: this stage counts in
@@ -478,7 +475,9 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
     double u, dc;
     bool expl;
     if constexpr (Ar<R>::strict) {
-      const double CooL = fcool_ki(rho, T);
+      double CooL;
+      if constexpr (CHEM) CooL = own ? ch_fcool(ch_fc_setup(A.sp, A.cols, c), T) : 0.0;
+      else CooL = fcool_ki(rho, T);
       const double gi = wc.gi, nsum = wc.ns;
       expl = dt <= 0.2 * Pn / (gi * fabs(CooL));
       u = stage == 0 ? Pn - gi * CooL * dt * 0.5 : Pn - gi * CooL * dt;
@@ -534,8 +533,9 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
     }
   }
 }
-template __global__ void k_src_col<double>(SrcArgs, Geo, Box, double, int);
-template __global__ void k_src_col<float>(SrcArgs, Geo, Box, double, int);
+template __global__ void k_src_col<double, false>(SrcArgs, Geo, Box, double, int);
+template __global__ void k_src_col<float, false>(SrcArgs, Geo, Box, double, int);
+template __global__ void k_src_col<double, true>(SrcArgs, Geo, Box, double, int);
 
 // Copy of the first ghost layer around the interior, src -> dst: the faces
 // k = 0 and n2+1 with their edges and corners, then j = 0 and n1+1 over
@@ -606,6 +606,7 @@ __global__ __launch_bounds__(256) void k_src_stiff(SrcArgs A, double dt, int sta
 // IMC (M3:2898-2921) on a queue of backward-Euler cells: all of them on the
 // plain path (which = 0), the ones the surrogate's residual test rejected
 // (which = 1).
+template <bool CHEM>
 __global__ __launch_bounds__(256) void k_src_imc(SrcArgs A, double dt, int stage, int which) {
   const unsigned nq = A.qn[which];
   const int *q = which ? A.fqueue : A.queue;
@@ -617,10 +618,13 @@ __global__ __launch_bounds__(256) void k_src_imc(SrcArgs A, double dt, int stage
     const double gi = gammi1_of(sp, c);
     const double nsum = sp[0][c] + sp[1][c] + sp[3][c] + sp[4][c] + sp[2][c];
     const double dts = stage == 0 ? dt * 0.5 : dt;
-    src_finish(A, c, imc(A.Pn[c], nsum, dts, rho, gi));
+    if constexpr (CHEM) src_finish(A, c, ch_imc(ch_fc_setup(sp, A.cols, c), A.Pn[c], nsum, dts, gi));
+    else src_finish(A, c, imc(A.Pn[c], nsum, dts, rho, gi));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.cnt[which], (unsigned long long)nq);
 }
+template __global__ void k_src_imc<false>(SrcArgs, double, int, int);
+template __global__ void k_src_imc<true>(SrcArgs, double, int, int);
 
 __global__ void k_zero_u32x2(unsigned *p) {
   if (threadIdx.x < 2) p[threadIdx.x] = 0u;
@@ -638,7 +642,7 @@ __device__ __forceinline__ double wave_min(double v) {
 // Two-pass min: each block strides over (j,k) rows and writes one pair of
 // partial minima; k_dtfinal folds them.  min is exact, so the result does
 // not depend on the order.
-__global__ void k_dtmin(DtArgs A, Geo g, double *part) {
+template <bool CHEM> __global__ void k_dtmin(DtArgs A, Geo g, double *part) {
   __shared__ double s0[16], s1[16];
   const int nx = g.n[0];
   const long rows = (long)g.n[1] * g.n[2];
@@ -668,7 +672,9 @@ __global__ void k_dtmin(DtArgs A, Geo g, double *part) {
       double gi = gammi1_of(sp, c);
       double alpha = gi * KCOND * sqrt(dmin(Tn, 15.0));
       alpha = 0.4 * Nn * KB / alpha * (dl * dl);
-      double CooL = fcool_ki(rho, Tn);
+      double CooL;
+      if constexpr (CHEM) CooL = ch_fcool(ch_fc_setup(sp, A.cols, c), Tn);
+      else CooL = fcool_ki(rho, Tn);
       double tauC = p / gi / fabs(CooL);
       tauC = dmax(0.2 * tauC, 2.5e-4);
       tl = dmin(tl, tauC);
@@ -687,6 +693,9 @@ __global__ void k_dtmin(DtArgs A, Geo g, double *part) {
     part[2 * blockIdx.x + 1] = tl;
   }
 }
+
+template __global__ void k_dtmin<false>(DtArgs, Geo, double *);
+template __global__ void k_dtmin<true>(DtArgs, Geo, double *);
 
 __global__ void k_dtfinal(const double *part, int nb, double *out, double tfinal) {
   __shared__ double s0[16], s1[16];

@@ -23,6 +23,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 COOL_KI = 1
 COOL_SURROGATE = 2
+COOL_CHEM = 3
+COLUMNS = ("Ntot", "NH2", "NnC", "NCO", "tCII")
 MLP_NPARAM = 4481
 
 
@@ -71,6 +73,7 @@ def lib():
         L.mcf_upload.argtypes = [P, pd, pd]
         L.mcf_download.argtypes = [P, pd, pd]
         L.mcf_download_bcc.argtypes = [P, pd]
+        L.mcf_download_columns.argtypes = [P, pd]
         L.mcf_set_time.argtypes = [P, D, I]
         L.mcf_set_inflow_density.argtypes = [P, C.c_void_p]
         L.mcf_get_time.argtypes = [P, pd, pi]
@@ -100,7 +103,7 @@ def lib():
         L.mcf_cool_solve.argtypes = [P, pd, pd, pd, pd, pd, C.c_int64, I, pd, pi32, pd]
         L.mcf_cool_stats.argtypes = [P, pi64, I]
         L.mcf_cool_solve_noalloc.argtypes = [P, P, P, P, P, P, C.c_int64, I, P, P]
-        for nm in ("mcf_init", "mcf_finalize", "mcf_upload", "mcf_download", "mcf_download_bcc",
+        for nm in ("mcf_init", "mcf_finalize", "mcf_upload", "mcf_download", "mcf_download_bcc", "mcf_download_columns",
                    "mcf_set_time", "mcf_set_inflow_density", "mcf_get_time", "mcf_timestep", "mcf_source", "mcf_sweep",
                    "mcf_ct", "mcf_dissip", "mcf_step", "mcf_run", "mcf_sync", "mcf_profile",
                    "mcf_profile_report", "mcf_profile_stages", "mcf_copy_probe", "mcf_mlp_init", "mcf_mlp_set_weights",
@@ -175,6 +178,13 @@ class Domain:
         b = np.empty((3,) + self.shape)
         _chk(lib().mcf_download_bcc(self.h, _dp(b)))
         return b
+
+    def columns(self) -> np.ndarray:
+        """[q, dir, k, j, i]: column densities of COLUMNS seen from the left
+        (dir 0) and right (dir 1) x ends (mcf_download_columns)."""
+        out = np.empty((10,) + self.shape, dtype=np.float64)
+        _chk(lib().mcf_download_columns(self.h, _dp(out)))
+        return out.reshape((5, 2) + self.shape)
 
     def set_inflow_density(self, dtf):
         """x-inflow density field DTF (float32 [k, j, i] over the interior),

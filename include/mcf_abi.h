@@ -33,8 +33,13 @@ enum {
 enum { MCF_PREC_FP64 = 0, MCF_PREC_MIXED = 1 };
 /* MCF_COOL_KI: KI cooling with the reference's explicit/IMC update.
  * MCF_COOL_SURROGATE: the same, with the backward-Euler (IMC) cells solved
- * by the learned surrogate (see the surrogate section below). */
-enum { MCF_COOL_KI = 1, MCF_COOL_SURROGATE = 2 };
+ * by the learned surrogate (see the surrogate section below).
+ * MCF_COOL_CHEM: the reference's own chemistry-network Fcool (M3:2825-2881:
+ * Ly-alpha, CII with fesc/LEVC2, OI, dust recombination and CO line
+ * cooling; photo-electric, cosmic-ray and H2 photo-destruction heating),
+ * which reads the SHIELD column densities; always FP64, also under
+ * MCF_PREC_MIXED. */
+enum { MCF_COOL_KI = 1, MCF_COOL_SURROGATE = 2, MCF_COOL_CHEM = 3 };
 
 /* INPUT3D.DAT (M3:134-151), the WNM state hard-wired by INITIA
  * (M3:155-157) and the options of this implementation. */
@@ -49,7 +54,10 @@ typedef struct mcf_config {
   /* x: 1 outflow (zero-gradient copy, the 1D code's BC=1), 3 periodic,
    * 4 inflow; both x ends alike.  y, z: 3 (periodic) only. */
   int bcx1, bcx2, bcy1, bcy2, bcz1, bcz2;
-  int ifchem, ifthrm, ifrad, ifgrv;          /* ifthrm = 2: cooling+conduction */
+  /* ifthrm = 2: cooling + conduction; ifchem = 2: the chemistry network;
+   * ifrad = 2: SHIELD column densities; ifgrv = 2 (self-gravity) is
+   * rejected */
+  int ifchem, ifthrm, ifrad, ifgrv;
   double pinit_over_kb;                      /* 8.810807d3 */
   double Hini, pini, H2ini, Heini, Hepini, Cini, COini, Cpini;
   /* MPI block split NSPLTx/y/z of the reference run to reproduce (M3:64-72):
@@ -74,8 +82,16 @@ int mcf_finalize(mcf_ctx *ctx);
 /* Restart read (M3:454-465) followed by CC(4) (M3:470): copies the 8 U
  * components (U(:,:,:,1:8), contiguous) and the 8 species arrays
  * (ndH, ndp, ndH2, ndHe, ndHep, ndC, ndCO, ndCp; contiguous, restart order)
- * to the device and rebuilds the cell-centred field Bcc. */
+ * to the device and rebuilds the cell-centred field Bcc.  Then, as INITIA
+ * does (M3:472-478), zeroes the column densities and CII optical depths and
+ * runs SHIELD 20 times when ifrad = 2. */
 int mcf_upload(mcf_ctx *ctx, const double *U8, const double *species8);
+/* The column densities of module chmvar (M3:29): Ntot, NH2, NnC, NCO,
+ * tCII, each seen from the left x end then the right one (10 arrays,
+ * contiguous, (-1:N+2)^3 layout; interior cells hold values).  The
+ * reference keeps them in memory only; MCF_E_STATE when none is carried
+ * (chemistry Fcool, network and SHIELD all off). */
+int mcf_download_columns(mcf_ctx *ctx, double *col10);
 /* SAVEU (M3:684-757) without the file: copies the state back. */
 int mcf_download(mcf_ctx *ctx, double *U8, double *species8);
 /* Cell-centred field Bcc(:,:,:,1:3) (contiguous). */
@@ -98,7 +114,9 @@ int mcf_get_time(mcf_ctx *ctx, double *time, int *ifevo);
  * clips of EVOLVE (M3:567-604): dt = min(tfinal, CFL*tCFL, tLMT), clipped
  * to tfinal and tsave.  *tag = 1 (CFL bound, 'FastSpd') or 2 ('Conduct'). */
 int mcf_timestep(mcf_ctx *ctx, double tsave, double *dt, int *tag);
-/* SOURCE(dt) (M3:2448-2572): two-stage KI cooling + saturated conduction. */
+/* SOURCE(dt) (M3:2448-2780): SHIELD (ifrad = 2), the two-stage cooling +
+ * saturated conduction (ifthrm = 2) with the configured Fcool, then the
+ * chemistry network (ifchem = 2). */
 int mcf_source(mcf_ctx *ctx, double dt);
 /* MHD(x|y|z) (M3:1264-1324): RHS + MOC + REMAP sweep, dir 0/1/2. */
 int mcf_sweep(mcf_ctx *ctx, int dir, double dt);

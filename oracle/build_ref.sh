@@ -5,15 +5,17 @@
 # /opt/conda.  Test infrastructure: the binary goes to oracle/_ref/bin/ and is
 # used only to write golden fixtures in this container.
 #
-# usage: oracle/build_ref.sh [NDX [dtf]]  (NDX = compile-time block size, default 18;
-#        dtf: read the inhomogeneous inflow density field DTF/D{rank}.dat)
+# usage: oracle/build_ref.sh [NDX [dtf] [chem]]  (NDX = compile-time block size, default 18;
+#        dtf: read the inhomogeneous inflow density field DTF/D{rank}.dat;
+#        chem: keep the reference's chemistry-network Fcool)
 #
 # -O2 without -march: baseline x86-64 has no FMA, so nothing is fused.
 set -euo pipefail
 NDX=${1:-18}
-VARIANT=${2:-}
+shift || true
+VARIANT="$*"
 SUFFIX=""
-if [ "$VARIANT" = "dtf" ]; then SUFFIX="_dtf"; fi
+for v in $VARIANT; do SUFFIX="${SUFFIX}_$v"; done
 REF=/root/reference/3DMHD-CfCA
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref

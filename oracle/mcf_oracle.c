@@ -57,6 +57,7 @@ struct mcfo {
   double *Tn, *Pn, *Qx, *Qy, *Qz;
   double *ppre; /* U(:,:,:,5) right after the RHS ghost fill of a sweep */
   float *dtfa;  /* inhomogeneous inflow density field DTF (REAL*4), NULL = uniform */
+  double *col[10]; /* Ntot, NH2, NnC, NCO, tCII seen from the left / right x end (col[2q+dir]) */
   int nsp[3];   /* emulated MPI block split */
   double *pw;   /* pencil work pool, one slice of 64 arrays per thread */
   int nthr;     /* threads the pool has slices for */
@@ -184,6 +185,8 @@ mcfo *mcfo_create(const mcfo_params *p) {
                     &o->vfc[2], &o->dnc, &o->emf[0], &o->emf[1], &o->emf[2], &o->dq[0], &o->dq[1],
                     &o->dq[2], &o->dq[3], &o->Tn, &o->Pn, &o->Qx, &o->Qy, &o->Qz, &o->ppre};
   for (size_t t = 0; t < sizeof(tmp) / sizeof(tmp[0]); t++) *tmp[t] = (double *)calloc(o->ncell, sizeof(double));
+  if (!p->kicool || p->ifchem == 2 || p->ifrad == 2)
+    for (int q = 0; q < 10; q++) o->col[q] = (double *)calloc(o->ncell, sizeof(double));
   o->nthr = nthreads_max();
   o->pw = (double *)calloc((size_t)o->nthr * 64 * (o->nmax + 8), sizeof(double));
   int nspl[3] = {p->nsplitx, p->nsplity, p->nsplitz};
@@ -235,6 +238,7 @@ void mcfo_destroy(mcfo *o) {
   for (size_t t = 0; t < sizeof(tmp) / sizeof(tmp[0]); t++) free(tmp[t]);
   free(o->pw);
   free(o->dtfa);
+  for (int q = 0; q < 10; q++) free(o->col[q]);
   free(o);
 }
 
@@ -533,6 +537,385 @@ static double imc(double P, double n, double dt, double rho, double gammi1) {
     Pmold = Pm;
   }
   return Pm;
+}
+
+/* ------------------------------------------------------------------ */
+/* Chemistry form of the cooling and the network (ifchem, ifrad = 2)    */
+/* ------------------------------------------------------------------ */
+/* Constants of module chmvar (M3:27-28). */
+#define CH_G0 1.0
+#define CH_XO 3.2e-4
+#define CH_DV 2.0
+#define CH_TGR 5.0e-3
+
+#define COL(q, d) (o->col[2 * (q) + (d)])
+enum { C_NTOT = 0, C_NH2, C_NNC, C_NCO, C_TCII };
+
+/* fesc (M3:2883-2888): escape probability of an optical depth */
+static double fesc(double tau) {
+  return (0.5 + copysign(0.5, 0.1 - 1.0e-16 - tau)) * (0.5 - 0.585 * tau + 0.4563 * (tau * tau)) +
+         (0.5 + copysign(0.5, 11.9025 - (tau - 3.55) * (tau - 3.55))) * (1.0 - exp(-2.34 * tau)) / (4.68 * tau + 1.0e-8) +
+         (0.5 + copysign(0.5, tau - 7.0)) / (4.0 * tau * sqrt(log(DMAX(tau * 0.56419, 4.0))) + 1.0e-8);
+}
+
+/* LEVC2 (M3:2890-2895): CII two-level populations */
+static void levc2(double T, double b21, double ndH, double ndH2, double nde, double ndCp, double *n1, double *n2) {
+  double c21 = 8.854e-8 * nde / sqrt(T) + 9.399e-10 * pow(T, 0.07) * (ndH + ndH2);
+  double c12 = 2.0 * c21 * exp(-0.092 / T);
+  double A21 = 2.4e-6 * b21;
+  *n2 = ndCp * c12 / (c12 + c21 + A21);
+  *n1 = ndCp * (c21 + A21) / (c12 + c21 + A21);
+}
+
+static inline double nde_of(const mcfo *o, long c) { return SPC(SP)[c] + SPC(SHEP)[c] + SPC(SCP)[c]; }
+static inline double ndtot_of(const mcfo *o, long c) {
+  return SPC(SP)[c] + SPC(SH)[c] + 2.0 * SPC(SH2)[c] + SPC(SHE)[c] + SPC(SHEP)[c];
+}
+
+/* Fcool, chemistry form (M3:2825-2881), at cell c with the electron and
+ * total number densities nde, ndtot the reference holds for it. */
+static double fcool_chem(const mcfo *o, long c, double T, double nde, double ndtot) {
+  const double ndH = SPC(SH)[c], ndp = SPC(SP)[c], ndH2 = SPC(SH2)[c], ndHe = SPC(SHE)[c], ndHep = SPC(SHEP)[c];
+  const double ndCO = SPC(SCO)[c], ndCp = SPC(SCP)[c];
+  double Av1 = 1.63542e-3 * COL(C_NTOT, 0)[c], x1 = 6.1714e3 * COL(C_NH2, 0)[c];
+  double Av2 = 1.63542e-3 * COL(C_NTOT, 1)[c], x2 = 6.1714e3 * COL(C_NH2, 1)[c];
+  double ATN1 = (exp(-2.5 * Av1) + exp(-2.5 * Av2)) * 0.5;
+  double ATN2 = (exp(-3.77358 * Av1) + exp(-3.77358 * Av2)) * 0.5;
+  double pha = CH_G0 * sqrt(1.0e3 * T) * ATN1 / nde;
+  double Laml = ndH * nde * 1.44049e9 * exp(-1.184e2 / T);
+  Laml = Laml + ndp * nde * 1.48e6 * exp(-8.0e1 / T);
+  double fesC1 = fesc(COL(C_TCII, 0)[c]), fesC2 = fesc(COL(C_TCII, 1)[c]);
+  double b21 = fesC1 + fesC2, n1, n2;
+  levc2(T, b21, ndH, ndH2, nde, ndCp, &n1, &n2);
+  double Lamc = 6.0157e7 * n2 * b21;
+  double tO1 = 3.40057e-6 * COL(C_NTOT, 0)[c] / CH_XO, tO2 = 3.40057e-6 * COL(C_NTOT, 1)[c] / CH_XO;
+  double fesO1 = fesc(tO1), fesO2 = fesc(tO2);
+  double Lamo = (DMAX(ndtot * CH_XO - ndCO, 0.0) / CH_XO) * (ndH + 0.5 * ndH2) * 1.23916e1 * pow(T, 0.4) * exp(-0.228 / T);
+  Lamo = Lamo * (fesO1 + fesO2);
+  double Lamd = nde * ndtot * 6.06236 * pow(T, 0.94) * pow(pha, 0.462628 / pow(T, 6.8e-2));
+  double ncr = 3.3e6 * pow(T, 0.75) / (ndH + ndp + ndH2 + ndHe + ndHep);
+  double tau1 = 1.33194e1 * COL(C_NCO, 0)[c] / (T * CH_DV), tau2 = 1.33194e1 * COL(C_NCO, 1)[c] / (T * CH_DV);
+  double ct1 = tau1 * sqrt(6.283 * log(2.13 + (tau1 * 0.36788) * (tau1 * 0.36788)));
+  double ct2 = tau2 * sqrt(6.283 * log(2.13 + (tau2 * 0.36788) * (tau2 * 0.36788)));
+  double ym1 = log(1.0 + ct1 / (1.0 + 1.0e1 * ncr)), ym2 = log(1.0 + ct2 / (1.0 + 1.0e1 * ncr));
+  double fes = (2.0 + ym1 + 0.6 * (ym1 * ym1)) / (1.0 + ct1 + ncr + 1.5 * sqrt(ncr)) +
+               (2.0 + ym2 + 0.6 * (ym2 * ym2)) / (1.0 + ct2 + ncr + 1.5 * sqrt(ncr));
+  double LCOr = ndCO * 1.91505e10 * (T * T) * fes;
+  double LCOH = ndH * ndCO * 7.96086e4 * sqrt(T) * exp(-pow(2.0 / T, 3.43)) * exp(-3.08 / T);
+  double LCOH2 = ndH2 * ndCO * 3.60834e4 * T * exp(-pow(3.14e2 / T, 0.333)) * exp(-3.08 / T);
+  double Gampe = ndtot * 2.56526e3 * CH_G0 * ATN1 *
+                 (7.382e-3 * pow(T, 0.7) / (1.0 + 2.0e-4 * pha) + 4.9e-2 / (1.0 + 4.0e-3 * pow(pha, 0.73)));
+  double Gamcr = (ndH + ndHe + ndH2) * 1.89435;
+  double SHLD1 = 0.965 / ((1.0 + x1 / CH_DV) * (1.0 + x1 / CH_DV)) + 0.035 / sqrt(1.0 + x1) * exp(-8.5e-4 * sqrt(1.0 + x1));
+  double SHLD2 = 0.965 / ((1.0 + x2 / CH_DV) * (1.0 + x2 / CH_DV)) + 0.035 / sqrt(1.0 + x2) * exp(-8.5e-4 * sqrt(1.0 + x2));
+  double Gampd = ndH2 * 4.16362e4 * CH_G0 * ATN2 * (SHLD1 + SHLD2) * 0.5;
+  return Laml + Lamc + Lamo + Lamd + LCOr + LCOH + LCOH2 - Gampe - Gamcr - Gampd;
+}
+
+/* The Fcool the run uses at cell c (the KI form reads only the density). */
+static double fcool_cell(const mcfo *o, long c, double T) {
+  if (o->p.kicool) return fcool_ki(U(1)[c], T);
+  return fcool_chem(o, c, T, nde_of(o, c), ndtot_of(o, c));
+}
+
+/* IMC (M3:2898-2921) with the run's Fcool at cell c. */
+static double imc_cell(const mcfo *o, long c, double P, double n, double dt, double gammi1) {
+  double Pu = 1.0e10, Pd = 1.0e-10, Pm = 1.0e1, Pmold = -1.0;
+  double nkbi = 1.0 / (KB * n);
+  for (int kkk = 1; kkk <= 30; kkk++) {
+    double T = Pm * nkbi;
+    double CooL = fcool_cell(o, c, T);
+    double fev = Pm + CooL * dt * gammi1 - P;
+    double iud = copysign(1.0, fev);
+    Pd = DMAX(-iud * Pm, Pd);
+    Pu = DMIN(2.0 * Pm / (iud + 1.0 + 1.0e-10), Pu);
+    Pm = pow(1.0e1, 0.5 * (log10(Pu) + log10(Pd)));
+    if (fabs(Pm - Pmold) < 1.0e-5) break;
+    Pmold = Pm;
+  }
+  return Pm;
+}
+
+/* SHIELD (M3:2924-3036): column densities of the total gas, H2, C, CO and
+ * the CII optical depth from each x end, per (j,k) row.  Each x block of
+ * the reproduced MPI run scans its own cells (half a cell's column at the
+ * cell, dx of the block's first cell), then adds the totals of the blocks
+ * beyond it in order of distance. */
+static void shield(mcfo *o) {
+  const int nx = o->n[0], ny = o->n[1], nz = o->n[2], nsx = o->nsp[0], cx = nx / nsx;
+  const double dvin = 1.0 / CH_DV;
+  double *temp = (double *)malloc(sizeof(double) * (size_t)o->ncell);
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        long c = ix3(o, i, j, k);
+        double b21 = fesc(COL(C_TCII, 0)[c]) + fesc(COL(C_TCII, 1)[c]);
+        double T = U(5)[c] / (KB * (SPC(SP)[c] + SPC(SH)[c] + SPC(SH2)[c] + SPC(SHE)[c] + SPC(SHEP)[c]));
+        double n1, n2;
+        levc2(T, b21, SPC(SH)[c], SPC(SH2)[c], nde_of(o, c), SPC(SCP)[c], &n1, &n2);
+        temp[c] = 1.15563e1 * (n1 - n2) * dvin;
+      }
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      double tot[nsx][5];
+      double *q[5] = {NULL, SPC(SH2), SPC(SC), SPC(SCO), temp};
+      for (int b = 0; b < nsx; b++) {
+        const double dxh = 0.5 * dxg(o, 0, b * cx + 1);
+        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int il = 1; il <= cx; il++) {
+          long c = ix3(o, b * cx + il, j, k);
+          for (int m = 0; m < 5; m++) {
+            double v = m == 0 ? ndtot_of(o, c) : q[m][c];
+            t[m] = t[m] + dxh * v;
+            COL(m, 0)[c] = t[m];
+            t[m] = t[m] + dxh * v;
+          }
+        }
+        for (int m = 0; m < 5; m++) t[m] = 0.0;
+        for (int il = cx; il >= 1; il--) {
+          long c = ix3(o, b * cx + il, j, k);
+          for (int m = 0; m < 5; m++) {
+            double v = m == 0 ? ndtot_of(o, c) : q[m][c];
+            t[m] = t[m] + dxh * v;
+            COL(m, 1)[c] = t[m];
+            t[m] = t[m] + dxh * v;
+          }
+        }
+        for (int m = 0; m < 5; m++) tot[b][m] = t[m];
+      }
+      for (int b = 0; b < nsx; b++)
+        for (int il = 1; il <= cx; il++) {
+          long c = ix3(o, b * cx + il, j, k);
+          for (int m = 0; m < 5; m++) {
+            for (int L = 1; b + L < nsx; L++) COL(m, 1)[c] = COL(m, 1)[c] + tot[b + L][m];
+            for (int L = 1; b - L >= 0; L++) COL(m, 0)[c] = COL(m, 0)[c] + tot[b - L][m];
+          }
+        }
+    }
+  free(temp);
+}
+
+void mcfo_init_columns(mcfo *o) {
+  if (!o->col[0]) return;
+  for (int q = 0; q < 10; q++) memset(o->col[q], 0, sizeof(double) * (size_t)o->ncell);
+  if (o->p.ifrad == 2)
+    for (int l = 0; l < 20; l++) shield(o);
+}
+
+double *mcfo_column(mcfo *o, int q, int dir) {
+  if (q < 0 || q > 4 || dir < 0 || dir > 1 || !o->col[0]) return NULL;
+  return COL(q, dir);
+}
+
+double mcfo_fcool_cell(mcfo *o, int i, int j, int k, double T) {
+  long c = ix3(o, i, j, k);
+  return fcool_chem(o, c, T, nde_of(o, c), ndtot_of(o, c));
+}
+
+/* Omexp (M3:3136-3141): 1 - exp(-eps), by its Taylor series below 1e-4 */
+static double omexp(double eps) {
+  return (0.5 + copysign(0.5, eps - 1.0e-4)) * (1.0 - exp(-eps)) +
+         (0.5 - copysign(0.5, eps - 1.0e-4)) *
+             (eps - 0.5 * (eps * eps) + 0.16666666666666667 * (eps * eps * eps) - 4.16666666666666667e-2 * (eps * eps * eps * eps));
+}
+
+typedef struct {
+  double zeta, kHrec, kHerec, kH2, kH2ph, kH2dH, kH2de, kCO, kCOph, kCi, kCrec, kCOde, kCOdH, kHie, kHeie, kCie, kHiH,
+      kHeiH, kCiH, kCOdHep, kH2dHep;
+} chem_rates;
+
+/* RATES (M3:3040-3133) at cell c and temperature T, with the cell's
+ * electron density nde. */
+static chem_rates rates(const mcfo *o, long c, double T, double nde) {
+  chem_rates r;
+  double Av1 = 1.63542e-3 * COL(C_NTOT, 0)[c], x1 = 6.1714e3 * COL(C_NH2, 0)[c];
+  double Av2 = 1.63542e-3 * COL(C_NTOT, 1)[c], x2 = 6.1714e3 * COL(C_NH2, 1)[c];
+  double ATN2 = (exp(-3.77358 * Av1) + exp(-3.77358 * Av2)) * 0.5;
+  double ATN3 = (exp(-2.3585 * Av1) + exp(-2.3585 * Av2)) * 0.5;
+  double NH21 = COL(C_NH2, 0)[c], NH22 = COL(C_NH2, 1)[c];
+  double ATN4 = DMIN(1.0, pow(3.0857e1 * NH21, -3.0e-2) * exp(-1.54285e-3 * NH21)) +
+                DMIN(1.0, pow(3.0857e1 * NH22, -3.0e-2) * exp(-1.54285e-3 * NH22));
+  ATN4 = ATN4 * 0.5;
+  const double p06 = (double)0.6f; /* the exponent is a default-real (REAL*4) literal */
+  double NC1 = COL(C_NCO, 0)[c], NC2 = COL(C_NCO, 1)[c];
+  double ATN5 = (0.5 - copysign(0.5, NC1 - 1.40995e-3)) * exp(-1.0 * pow(1.02857e3 * NC1, p06)) +
+                (0.5 + copysign(0.5, NC1 - 1.40995e-3)) *
+                    DMIN(pow(3.75439e3 * NC1 + 1.0e-100, -0.75), pow(7.3469e1 * NC1 + 1.0e-100, -1.3));
+  ATN5 = ATN5 + (0.5 - copysign(0.5, NC2 - 1.40995e-3)) * exp(-1.0 * pow(1.02857e3 * NC2, p06)) +
+         (0.5 + copysign(0.5, NC2 - 1.40995e-3)) *
+             DMIN(pow(3.75439e3 * NC2 + 1.0e-100, -0.75), pow(7.3469e1 * NC2 + 1.0e-100, -1.3));
+  ATN5 = ATN5 * 0.5;
+  r.zeta = 9.4671e-4;
+  r.kHrec = (0.5 + copysign(0.5, 15.78 - T)) * 0.45 * log(1.578e2 / T) +
+            (0.5 - copysign(0.5, 15.78 - T)) * 0.4 * sqrt(1.578e2 / T);
+  r.kHrec = 2.05572e1 * pow(T, -0.5) * r.kHrec;
+  r.kHerec = 6.37623e1 * pow(T, -0.672);
+  r.kH2 = 3.4569e-3 * sqrt(T) / (1.0 + 1.26491 * sqrt(T + CH_TGR) + 2.0 * T + 8.0 * (T * T));
+  double SHLD1 = 0.965 / ((1.0 + x1 / CH_DV) * (1.0 + x1 / CH_DV)) + 0.035 / sqrt(1.0 + x1) * exp(-8.5e-4 * sqrt(1.0 + x1));
+  double SHLD2 = 0.965 / ((1.0 + x2 / CH_DV) * (1.0 + x2 / CH_DV)) + 0.035 / sqrt(1.0 + x2) * exp(-8.5e-4 * sqrt(1.0 + x2));
+  r.kH2ph = 1.04138e3 * CH_G0 * ATN2 * (SHLD1 + SHLD2) * 0.5;
+  r.kH2dH = 1.07294e5 * exp(-4.39e1 / T);
+  r.kH2de = 1.133e6 * pow(T, 0.35) * exp(-1.02e2 / T);
+  r.kCO = 1.57785e-2 / (1.0 + CH_G0 * ATN3 / (SPC(SH2)[c] * CH_XO));
+  r.kCOph = 3.155e3 * CH_G0 * ATN3 * ATN4 * ATN5;
+  double SHLC1 = exp(-2.6 * Av1 - 3.0857e1 * COL(C_NNC, 0)[c] - 4.33743413e-4 * NH21) / (1.0 + 4.33743413e-4 * NH21);
+  double SHLC2 = exp(-2.6 * Av2 - 3.0857e1 * COL(C_NNC, 1)[c] - 4.33743413e-4 * NH22) / (1.0 + 4.33743413e-4 * NH22);
+  r.kCi = 3.97 * r.zeta + 6.62697e3 * CH_G0 * (SHLC1 + SHLC2) * 0.5;
+  r.kCrec = 2.69267e1 * pow(T, -0.62) * (1.0 + 1.0e-4 * sqrt(nde) / (T * T));
+  r.kCOde = 1.4e4 * pow(T, -0.5) * exp(-1.14e2 / T);
+  r.kCOdH = 1.91858e3 * pow(T, -0.5) * exp(-7.77e1 / T);
+  r.kHie = 5.76e4 * sqrt(T) * exp(-1.58e2 / T);
+  r.kHeie = 2.76e4 * pow(T, 0.43) * exp(-2.85e2 / T);
+  r.kCie = 1.74e5 * pow(T, 0.4) * exp(-1.31e2 / T);
+  r.kHiH = 9.79 * sqrt(T) * exp(-1.58e2 / T);
+  r.kHeiH = 4.57 * pow(T, 0.43) * exp(-2.85e2 / T);
+  r.kCiH = 3.01e1 * pow(T, 0.4) * exp(-1.31e2 / T);
+  r.kH2dHep = 1.1676 * exp(-0.035 / T);
+  r.kCOdHep = 5.0491e4;
+  return r;
+}
+
+/* The chemistry network of SOURCE (M3:2576-2774) at cell c over dt. */
+static void chem_cell(mcfo *o, long c, double dt) {
+  const double mH = 1.0, mHe = 4.0, mH2 = 2.0;
+  double ndpold = SPC(SP)[c], ndHold = SPC(SH)[c], ndH2old = SPC(SH2)[c], ndHeold = SPC(SHE)[c];
+  double ndHepold = SPC(SHEP)[c], ndCold = SPC(SC)[c], ndCpold = SPC(SCP)[c], ndCOold = SPC(SCO)[c];
+  double ndH, ndp, ndH2, ndHe, ndHep, ndC, ndCp, ndCO;
+  const double ndtot = ndpold + ndHold + 2.0 * ndH2old + ndHeold + ndHepold;
+  double nde = ndpold + ndHepold + ndCpold;
+  double T = U(5)[c] / KB / (ndpold + ndHold + ndH2old + ndHeold + ndHepold);
+  chem_rates r = rates(o, c, T, nde);
+  double temp1, temp2, temp3, omeps, eps;
+  /* H recombination & ionization by CR */
+  temp1 = r.kHrec * nde;
+  temp2 = exp(-dt * (r.zeta + temp1));
+  temp3 = 1.0 / (r.zeta + temp1);
+  omeps = omexp(dt * (r.zeta + temp1));
+  ndH = (temp1 * ndHold + r.zeta * ndHold * temp2 + temp1 * ndpold * omeps) * temp3;
+  ndp = (r.zeta * ndpold + temp1 * ndpold * temp2 + r.zeta * ndHold * omeps) * temp3;
+  ndHold = ndH; ndpold = ndp; nde = ndp + ndHepold + ndCpold;
+  /* He recombination & ionization by CR */
+  temp1 = r.kHerec * nde;
+  temp2 = exp(-dt * (r.zeta + temp1));
+  temp3 = 1.0 / (r.zeta + temp1);
+  omeps = omexp(dt * (r.zeta + temp1));
+  ndHe = (temp1 * ndHeold + r.zeta * ndHeold * temp2 + temp1 * ndHepold * omeps) * temp3;
+  ndHep = (r.zeta * ndHepold + temp1 * ndHepold * temp2 + r.zeta * ndHeold * omeps) * temp3;
+  ndHeold = ndHe; ndHepold = ndHep; nde = ndp + ndHep + ndCpold;
+  /* H2 formation & dissociation by UV & electron collision */
+  temp1 = r.kH2ph + r.kH2de * nde;
+  temp2 = temp1 + 2.0 * r.kH2 * ndtot;
+  temp3 = exp(-dt * temp2);
+  omeps = omexp(dt * temp2);
+  ndH = (ndHold + 2.0 * ndH2old * omeps) * temp1 + 2.0 * r.kH2 * ndHold * ndtot * temp3;
+  ndH = ndH / temp2;
+  ndH2 = (2.0 * ndH2old + ndHold * omeps) * r.kH2 * ndtot + temp1 * ndH2old * temp3;
+  ndH2 = ndH2 / temp2;
+  ndHold = ndH; ndH2old = ndH2;
+  /* H2 dissociation by H collision */
+  temp1 = ndHold + 2.0 * ndH2old;
+  temp2 = exp(-dt * temp1 * r.kH2dH);
+  temp3 = 1.0 / (ndHold + 2.0 * ndH2old * temp2);
+  ndH = ndHold * temp1 * temp3;
+  ndH2 = ndH2old * temp1 * temp2 * temp3;
+  ndHold = ndH; ndH2old = ndH2;
+  /* CO formation */
+  temp1 = exp(-dt * r.kCO * ndtot);
+  eps = dt * r.kCO * ndtot;
+  omeps = omexp(eps);
+  ndCO = ndCOold + omeps * ndCpold;
+  ndCp = temp1 * ndCpold;
+  ndCOold = ndCO; ndCpold = ndCp; nde = ndp + ndHep + ndCp;
+  /* CO dissociation by UV & electron collision & H collision */
+  temp1 = exp(-dt * (r.kCOph + r.kCOde * nde + r.kCOdH * ndHold));
+  eps = dt * (r.kCOph + r.kCOde * nde + r.kCOdH * ndHold);
+  omeps = omexp(eps);
+  ndC = ndCold + omeps * ndCOold;
+  ndCO = temp1 * ndCOold;
+  ndCold = ndC; ndCOold = ndCO;
+  /* C recombination & ionization by CR & UV */
+  temp1 = r.kCrec * nde;
+  temp2 = exp(-dt * (r.kCi + temp1));
+  temp3 = 1.0 / (r.kCi + temp1);
+  omeps = omexp(dt * (r.kCi + temp1));
+  ndC = (temp1 * ndCold + r.kCi * ndCold * temp2 + temp1 * ndCpold * omeps) * temp3;
+  ndCp = (r.kCi * ndCpold + temp1 * ndCpold * temp2 + r.kCi * ndCold * omeps) * temp3;
+  ndCold = ndC; ndCpold = ndCp; nde = ndp + ndHep + ndCp;
+  /* H, He, C ionization by e collision */
+  ndH = ndHold * exp(-dt * r.kHie * nde);
+  ndHe = ndHeold * exp(-dt * r.kHeie * nde);
+  ndC = ndCold * exp(-dt * r.kCie * nde);
+  omeps = omexp(dt * r.kHie * nde); ndp = omeps * ndHold + ndpold;
+  omeps = omexp(dt * r.kHeie * nde); ndHep = omeps * ndHeold + ndHepold;
+  omeps = omexp(dt * r.kCie * nde); ndCp = omeps * ndCold + ndCpold;
+  ndHold = ndH; ndHeold = ndHe; ndCold = ndC;
+  ndpold = ndp; ndHepold = ndHep; ndCpold = ndCp;
+  nde = ndp + ndHep + ndCp;
+  /* H, He, C ionization by H or H2 or p collision */
+  temp1 = 1.0 / (1.0 + r.kHiH * dt * ndHold);
+  ndH = ndHold * temp1;
+  ndp = ndpold + temp1 * r.kHiH * dt * (ndHold * ndHold);
+  temp1 = ndH + ndp + ndH2old;
+  ndHe = ndHeold * exp(-dt * r.kHeiH * temp1);
+  ndC = ndCold * exp(-dt * r.kCiH * temp1);
+  omeps = omexp(dt * r.kHeiH * temp1); ndHep = omeps * ndHeold + ndHepold;
+  omeps = omexp(dt * r.kCiH * temp1); ndCp = omeps * ndCold + ndCpold;
+  ndHold = ndH; ndHeold = ndHe; ndCold = ndC;
+  ndpold = ndp; ndHepold = ndHep; ndCpold = ndCp;
+  nde = ndp + ndHep + ndCp;
+  /* H2 dissociation by Hep recombination */
+  temp1 = ndHepold - ndH2old;
+  if (temp1 >= 0.0) {
+    temp3 = exp(-dt * r.kH2dHep * temp1);
+    temp2 = 1.0 / (ndHepold - ndH2old * temp3);
+    ndHep = ndHepold * temp1 * temp2;
+    ndH2 = ndH2old * temp1 * temp2 * temp3;
+    temp3 = ndHepold * (1.0 - temp1 * temp2);
+    ndp = ndpold + temp3;
+    ndH = ndHold + temp3;
+    ndHe = ndHeold + temp3;
+  } else {
+    temp3 = exp(dt * r.kH2dHep * temp1);
+    temp2 = 1.0 / (ndHepold * temp3 - ndH2old);
+    ndHep = ndHepold * temp1 * temp2 * temp3;
+    ndH2 = ndH2old * temp1 * temp2;
+    temp3 = ndHepold * (1.0 - temp1 * temp2 * temp3);
+    ndp = ndpold + temp3;
+    ndH = ndHold + temp3;
+    ndHe = ndHeold + temp3;
+  }
+  ndHold = ndH; ndpold = ndp; ndH2old = ndH2;
+  ndHeold = ndHe; ndHepold = ndHep;
+  /* CO dissociation by Hep recombination */
+  temp1 = ndHepold - ndCOold;
+  if (temp1 >= 0.0) {
+    temp3 = exp(-dt * r.kCOdHep * temp1);
+    temp2 = 1.0 / (ndHepold - ndCOold * temp3);
+    ndHep = ndHepold * temp1 * temp2;
+    ndCO = ndCOold * temp1 * temp2 * temp3;
+    temp3 = ndHepold * (1.0 - temp1 * temp2);
+    ndCp = ndCpold + temp3;
+    ndHe = ndHeold + temp3;
+  } else {
+    temp3 = exp(dt * r.kCOdHep * temp1);
+    temp2 = 1.0 / (ndHepold * temp3 - ndCOold);
+    ndHep = ndHepold * temp1 * temp2 * temp3;
+    ndCO = ndCOold * temp1 * temp2;
+    temp3 = ndHepold * (1.0 - temp1 * temp2 * temp3);
+    ndCp = ndCpold + temp3;
+    ndHe = ndHeold + temp3;
+  }
+  SPC(SH)[c] = DMAX(o->spmin[SH], ndH);
+  SPC(SP)[c] = DMAX(o->spmin[SP], ndp);
+  SPC(SH2)[c] = DMAX(o->spmin[SH2], ndH2);
+  SPC(SHE)[c] = DMAX(o->spmin[SHE], ndHe);
+  SPC(SHEP)[c] = DMAX(o->spmin[SHEP], ndHep);
+  SPC(SC)[c] = DMAX(o->spmin[SC], ndC);
+  SPC(SCP)[c] = DMAX(o->spmin[SCP], ndCp);
+  SPC(SCO)[c] = DMAX(o->spmin[SCO], ndCO);
+  U(1)[c] = mH * SPC(SP)[c] + mH * SPC(SH)[c] + mH2 * SPC(SH2)[c] + mHe * SPC(SHE)[c] + mHe * SPC(SHEP)[c];
 }
 
 /* VLIMIT / VLIMIT_OT body (M3:1692-1906) for one pencil.  q, U5, U1 are
@@ -1149,8 +1532,25 @@ static void temp_and_flux(mcfo *o) {
       }
 }
 
+static void thermal(mcfo *o, double dt);
+
+/* SOURCE (M3:2448-2780): SHIELD (ifrad = 2), the thermal stage (ifthrm =
+ * 2), then the chemistry network (ifchem = 2).  nde and ndtot are formed
+ * from the species wherever they are read; the reference's arrays of them
+ * always equal that sum when read. */
 void mcfo_source(mcfo *o, double dt) {
-  if (o->p.ifthrm != 2) return;
+  if (o->p.ifrad == 2) shield(o);
+  if (o->p.ifthrm == 2) thermal(o, dt);
+  if (o->p.ifchem == 2) {
+    const int nx = o->n[0], ny = o->n[1], nz = o->n[2];
+#pragma omp parallel for schedule(static) num_threads(o->nthr)
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) chem_cell(o, ix3(o, i, j, k), dt);
+  }
+}
+
+static void thermal(mcfo *o, double dt) {
   int nx = o->n[0], ny = o->n[1], nz = o->n[2];
   int l15[2] = {1, 5}, l5[1] = {5};
   fillU(o, 7, 1, 1, l15, 2);
@@ -1170,19 +1570,19 @@ void mcfo_source(mcfo *o, double dt) {
       for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) {
           long c = ix3(o, i, j, k);
-          double CooL = fcool_ki(U(1)[c], o->Tn[c]);
+          double CooL = fcool_cell(o, c, o->Tn[c]);
           double gi = gammi1_of(o, c);
           double nsum = SPC(SH)[c] + SPC(SP)[c] + SPC(SHE)[c] + SPC(SHEP)[c] + SPC(SH2)[c];
           if (pass == 0) {
             if (dt <= 0.2 * o->Pn[c] / (gi * fabs(CooL)))
               U(5)[c] = U(5)[c] - gi * CooL * dt * 0.5;
             else
-              U(5)[c] = imc(U(5)[c], nsum, dt * 0.5, U(1)[c], gi);
+              U(5)[c] = imc_cell(o, c, U(5)[c], nsum, dt * 0.5, gi);
           } else {
             if (dt <= 0.2 * o->Pn[c] / (gi * fabs(CooL)))
               U(5)[c] = o->Pn[c] - gi * CooL * dt;
             else {
-              o->Pn[c] = imc(o->Pn[c], nsum, dt, U(1)[c], gi);
+              o->Pn[c] = imc_cell(o, c, o->Pn[c], nsum, dt, gi);
               U(5)[c] = o->Pn[c];
             }
           }
@@ -1243,7 +1643,7 @@ double mcfo_stblty(mcfo *o) {
         double gi = gammi1_of(o, c);
         double alpha = gi * KCOND * sqrt(DMIN(Tn, 15.0));
         alpha = 0.4 * Nn * KB / alpha * (dl * dl);
-        double CooL = fcool_ki(U(1)[c], Tn);
+        double CooL = fcool_cell(o, c, Tn);
         double tauC = U(5)[c] / gi / fabs(CooL);
         tauC = DMAX(0.2 * tauC, 2.5e-4);
         tLMT = DMIN(tLMT, tauC);

@@ -43,6 +43,11 @@ typedef struct {
    * hold the specific energy, so the result depends on the block edges;
    * DISSIP treats only the right x end when NSPLTx = 1 (M3:3156-3158). */
   int nsplitx, nsplity, nsplitz;
+  /* cooling function: 1 = the Koyama-Inutsuka form (K1:1146-1173) the
+   * KI-cooling configurations substitute; 0 = the reference's own
+   * chemistry-network form (M3:2825-2881, with fesc and LEVC2), which reads
+   * the column densities SHIELD maintains (ifrad = 2). */
+  int kicool;
 } mcfo_params;
 
 /* Field ids for mcfo_field(): 1..8 = U(:,:,:,1..8); 9..16 = species in the
@@ -70,6 +75,16 @@ double *mcfo_grid(mcfo *o, int axis, int spacing); /* x/dx arrays, index -1..n+2
 void    mcfo_get_time(const mcfo *o, double *time, int *ifevo);
 void    mcfo_set_time(mcfo *o, double time, int ifevo);
 void    mcfo_bbrv(const mcfo *o, double *bbrv /* 10*2*2, Fortran order */, double *bbrv_cm /* 8 */, double *dtf);
+
+/* INITIA's set-up of the radiation state after a restart read (M3:471-478):
+ * column densities and CII optical depths zeroed, then 20 SHIELD sweeps when
+ * ifrad = 2.  Call after the state is set. */
+void    mcfo_init_columns(mcfo *o);
+/* Column density array q (0 Ntot, 1 NH2, 2 NnC, 3 NCO, 4 tCII) seen from the
+ * left (dir 0) or the right (dir 1) end of x; NULL when not carried. */
+double *mcfo_column(mcfo *o, int q, int dir);
+/* Chemistry-form Fcool at cell (i,j,k) and temperature T (units of 10^3 K). */
+double  mcfo_fcool_cell(mcfo *o, int i, int j, int k, double T);
 
 /* Pieces of the step (same names as the reference routines). */
 void   mcfo_cc4(mcfo *o);                           /* CC(4): Bcc from faces + halo   */

@@ -27,7 +27,8 @@ class Params(C.Structure):
                                 "ifchem", "ifthrm", "ifrad", "ifgrv")] + \
         [(n, C.c_double) for n in ("pinit_over_kb", "Hini", "pini", "H2ini", "Heini", "Hepini",
                                    "Cini", "COini", "Cpini")] + \
-        [(n, C.c_int) for n in ("nsplitx", "nsplity", "nsplitz")]
+        [(n, C.c_int) for n in ("nsplitx", "nsplity", "nsplitz")] + \
+        [("kicool", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -81,28 +82,41 @@ def lib():
         L.mcfo_imc.restype = D
         L.mcfo_imc.argtypes = [D, D, D, D, D]
         L.mcfo_vlimit1d.argtypes = [C.c_int, pd, pd, pd, pd]
+        L.mcfo_init_columns.argtypes = [P]
+        L.mcfo_column.restype = pd
+        L.mcfo_column.argtypes = [P, C.c_int, C.c_int]
+        L.mcfo_fcool_cell.restype = D
+        L.mcfo_fcool_cell.argtypes = [P, C.c_int, C.c_int, C.c_int, D]
         _lib = L
     return _lib
 
 
-def to_params(p, nsplit=(2, 2, 2)) -> Params:
+COOLING = {"ki": 1, "chem": 0}
+COLUMNS = ("Ntot", "NH2", "NnC", "NCO", "tCII")
+
+
+def to_params(p, nsplit=(2, 2, 2), cooling="ki") -> Params:
     out = Params()
     for name, _ in Params._fields_:
-        if name.startswith("nsplit"):
+        if name.startswith("nsplit") or name == "kicool":
             continue
         setattr(out, name, getattr(p, name))
     out.nsplitx, out.nsplity, out.nsplitz = nsplit
+    out.kicool = COOLING[cooling]
     return out
 
 
 class Oracle:
-    """One single-block CPU instance of the reference step."""
+    """One single-block CPU instance of the reference step.
+
+    cooling: "ki" for the Koyama-Inutsuka Fcool the KI configurations
+    substitute, "chem" for the reference's chemistry-network Fcool."""
 
     def __init__(self, p, state: np.ndarray | None = None, time: float = 0.0, ifevo: int = 1,
-                 nsplit=(2, 2, 2), threads: int = 0):
+                 nsplit=(2, 2, 2), threads: int = 0, cooling: str = "ki"):
         self.p = p
         self.n = p.n
-        self._params = to_params(p, nsplit)
+        self._params = to_params(p, nsplit, cooling)
         self.h = lib().mcfo_create(C.byref(self._params))
         # threads < 1: OpenMP's default (OMP_NUM_THREADS or all cores)
         self.threads = lib().mcfo_set_threads(self.h, threads)
@@ -113,6 +127,7 @@ class Oracle:
         if state is not None:
             self.set_state(state)
             lib().mcfo_cc4(self.h)
+            lib().mcfo_init_columns(self.h)
         lib().mcfo_set_time(self.h, time, ifevo)
 
     def __del__(self):
@@ -141,6 +156,17 @@ class Oracle:
 
     def state(self) -> np.ndarray:
         return np.stack([self.field(f + 1).copy() for f in range(16)])
+
+    def columns(self) -> np.ndarray | None:
+        """[q, dir, k, j, i]: the column densities of COLUMNS seen from the
+        left (dir 0) and right (dir 1) x ends, or None when not carried."""
+        if not lib().mcfo_column(self.h, 0, 0):
+            return None
+        return np.stack([np.stack([np.ctypeslib.as_array(lib().mcfo_column(self.h, q, d), shape=self.shape).copy()
+                                   for d in range(2)]) for q in range(5)])
+
+    def fcool_cell(self, i, j, k, T):
+        return lib().mcfo_fcool_cell(self.h, i, j, k, T)
 
     def bcc(self) -> np.ndarray:
         return np.stack([self.field(17 + c).copy() for c in range(3)])

@@ -11,16 +11,18 @@ Edits (each recorded in the fixture manifest written by ref_runner.py):
      that N/NSPLT <= NDX-2 holds for the grid being run;
   2. run directory: the hard-coded '/work/maedarn/3DMHD/samplecnv2/' prefix
      (M3:134,302,309,...) becomes the process working directory;
-  3. cooling: the chemistry-network Fcool body (M3:2825-2881) is replaced by
-     the Koyama-Inutsuka form of 1DMHDKIcool/1DMHDwCC3KIcool.f90:1165-1171
-     with N = U(i,j,k,1)/1.27, the same substitution the reference's own
-     3DMHD-feedback/MCF3DSGincfca3-feedback.f90:3586 makes;
+  3. cooling (not with the "chem" option): the chemistry-network Fcool body
+     (M3:2825-2881) is replaced by the Koyama-Inutsuka form of
+     1DMHDKIcool/1DMHDwCC3KIcool.f90:1165-1171 with N = U(i,j,k,1)/1.27, the
+     same substitution the reference's own
+     3DMHD-feedback/MCF3DSGincfca3-feedback.f90:3586 makes; the "chem"
+     build keeps the reference's own chemistry Fcool;
   4. (only with the "dtf" option) the jump over the read of the
      inhomogeneous density field (`goto 119` at M3:404) is removed, so INITIA
      reads DTF/D{rank}.dat (M3:405-431) and the x-inflow ghost density is the
      DTF field shifted by -v t (M3:818-826, 854-861).
 
-usage: patch_ref.py SRC DST NDX [dtf]
+usage: patch_ref.py SRC DST NDX [dtf] [chem]
 """
 import re
 import sys
@@ -39,15 +41,16 @@ END SUBROUTINE Fcool
 """
 
 
-def patch(text: str, ndx: int, dtf: bool = False) -> str:
+def patch(text: str, ndx: int, dtf: bool = False, chem: bool = False) -> str:
     text, n = re.subn(r"^INTEGER, parameter :: ndx=\d+, ndy=\d+, ndz=\d+, ndmax=\d+, Dim=3.*$",
                       f"INTEGER, parameter :: ndx={ndx}, ndy={ndx}, ndz={ndx}, ndmax={ndx}, Dim=3",
                       text, count=1, flags=re.M)
     assert n == 1, "block-size line not found"
     text = text.replace("'/work/maedarn/3DMHD/samplecnv2/", "'")
-    start = text.index("SUBROUTINE Fcool(CooL,T,i,j,k)")
-    end = text.index("END SUBROUTINE Fcool", start) + len("END SUBROUTINE Fcool")
-    text = text[:start] + KI_FCOOL.rstrip("\n") + text[end:]
+    if not chem:
+        start = text.index("SUBROUTINE Fcool(CooL,T,i,j,k)")
+        end = text.index("END SUBROUTINE Fcool", start) + len("END SUBROUTINE Fcool")
+        text = text[:start] + KI_FCOOL.rstrip("\n") + text[end:]
     if dtf:
         text, n = re.subn(r"^(  DTF\(:,:,:\) = dinit1\n)  goto 119\n", r"\1", text, count=1, flags=re.M)
         assert n == 1, "DTF read skip not found"
@@ -56,8 +59,9 @@ def patch(text: str, ndx: int, dtf: bool = False) -> str:
 
 if __name__ == "__main__":
     src, dst, ndx = sys.argv[1], sys.argv[2], int(sys.argv[3])
-    dtf = len(sys.argv) > 4 and sys.argv[4] == "dtf"
+    opts = sys.argv[4:]
+    assert all(o in ("dtf", "chem") for o in opts), opts
     with open(src) as f:
         t = f.read()
     with open(dst, "w") as f:
-        f.write(patch(t, ndx, dtf))
+        f.write(patch(t, ndx, "dtf" in opts, "chem" in opts))

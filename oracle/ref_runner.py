@@ -26,10 +26,11 @@ REF_BIN = os.path.join(HERE, "_ref", "bin")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
-def ref_binary(ndx: int, dtf: bool = False) -> str:
-    path = os.path.join(REF_BIN, f"mcf_ref_ndx{ndx}" + ("_dtf" if dtf else ""))
+def ref_binary(ndx: int, dtf: bool = False, chem: bool = False) -> str:
+    opts = (["dtf"] if dtf else []) + (["chem"] if chem else [])
+    path = os.path.join(REF_BIN, f"mcf_ref_ndx{ndx}" + "".join("_" + o for o in opts))
     if not os.path.exists(path):
-        cmd = ["bash", os.path.join(HERE, "build_ref.sh"), str(ndx)] + (["dtf"] if dtf else [])
+        cmd = ["bash", os.path.join(HERE, "build_ref.sh"), str(ndx)] + opts
         subprocess.run(cmd, check=True)
     return path
 
@@ -55,15 +56,19 @@ def write_dtf_files(run: str, dtf: np.ndarray, n, nsplit):
 
 
 def run_reference(params: rs.Input3D, state: np.ndarray, nsteps: int, nranks: int = 8,
-                  tag: str = "run", time0: float = 0.0, dtf: np.ndarray | None = None) -> dict:
+                  tag: str = "run", time0: float = 0.0, dtf: np.ndarray | None = None,
+                  cooling: str = "ki") -> dict:
     """dtf (float32 [k, j, i], global interior): run the build that reads
-    the inhomogeneous inflow density field, with these values."""
+    the inhomogeneous inflow density field, with these values.  cooling:
+    "ki" runs the build with the Koyama-Inutsuka Fcool, "chem" the build
+    with the reference's own chemistry-network Fcool."""
+    assert cooling in ("ki", "chem"), cooling
     nsplit = rs.NSPLIT_TABLE[nranks]
     nx, ny, nz = params.n
     cx, cy, cz = nx // nsplit[0], ny // nsplit[1], nz // nsplit[2]
     assert cx * nsplit[0] == nx and cy * nsplit[1] == ny and cz * nsplit[2] == nz
     ndx = max(cx, cy, cz) + 2
-    exe = ref_binary(ndx, dtf is not None)
+    exe = ref_binary(ndx, dtf is not None, cooling == "chem")
     run = os.path.join(HERE, "_ref", "runs", tag)
     if os.path.isdir(run):
         shutil.rmtree(run)

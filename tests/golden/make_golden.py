@@ -23,6 +23,14 @@ Cases:
   converging_dtf_24  the same with the inflow density read from the DTF
                    field and shifted by -v t (the build that does not skip
                    the DTF read, oracle/patch_ref.py), 8 steps.
+  chem_rad_24x8x8  the reference's own chemistry-network Fcool (the build
+                   without the KI substitution) with the chemistry network
+                   (ifchem = 2) and SHIELD (ifrad = 2): x inflow, 8 ranks,
+                   1 and 6 steps
+  chem_rad_48x8x8  the same over 4 x blocks (16 ranks (4,2,2)), 6 steps: the
+                   SHIELD columns summed over blocks
+  chem_norad_24x8x8  chemistry network with zero column densities
+                   (ifrad = 0), periodic x, 6 steps
 usage: python tests/golden/make_golden.py [case ...]     (default: all)
 """
 import json
@@ -53,11 +61,12 @@ def small_input(nx, periodic):
     return p
 
 
-def case(name, p, st, steps, nranks):
+def case(name, p, st, steps, nranks, cooling="ki"):
     out = {"input": crop(st, p)}
-    meta = {"params": p.__dict__, "nranks": nranks, "nsplit": NSPLIT_TABLE[nranks], "steps": steps}
+    meta = {"params": p.__dict__, "nranks": nranks, "nsplit": NSPLIT_TABLE[nranks], "steps": steps,
+            "cooling": cooling}
     for k in steps:
-        r = ref_runner.run_reference(p, st, k, nranks=nranks, tag=f"golden_{name}_{k}")
+        r = ref_runner.run_reference(p, st, k, nranks=nranks, tag=f"golden_{name}_{k}", cooling=cooling)
         out[f"out{k}"] = crop(r["state"], p)
         out[f"dts{k}"] = np.array(r["dts"])
         out[f"time{k}"] = np.array(r["time"])
@@ -154,6 +163,18 @@ def main():
         converging_24()
     if not cases or "converging_dtf_24" in cases:
         converging_dtf_24()
+    if not cases or "chem_rad_24x8x8" in cases:
+        p = small_input(24, periodic=False)
+        p.ifchem, p.ifrad = 2, 2
+        case("chem_rad_24x8x8", p, ics.random_state(p, seed=11), (1, 6), 8, cooling="chem")
+    if not cases or "chem_rad_48x8x8" in cases:
+        p = small_input(48, periodic=False)
+        p.ifchem, p.ifrad = 2, 2
+        case("chem_rad_48x8x8", p, ics.random_state(p, seed=13), (6,), 16, cooling="chem")
+    if not cases or "chem_norad_24x8x8" in cases:
+        p = small_input(24, periodic=True)
+        p.ifchem, p.ifrad = 2, 0
+        case("chem_norad_24x8x8", p, ics.random_state(p, seed=17), (6,), 8, cooling="chem")
 
 
 if __name__ == "__main__":

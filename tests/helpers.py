@@ -18,6 +18,13 @@ def load_case(name):
     return p, tuple(meta["nsplit"]), {k: z[k] for k in z.files if k != "meta"}
 
 
+def case_cooling(name):
+    """The Fcool a golden case was written with: "ki" (the Koyama-Inutsuka
+    substitution) or "chem" (the reference's chemistry-network form)."""
+    z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    return json.loads(str(z["meta"])).get("cooling", "ki")
+
+
 def expand(crop, p):
     """Index-1..N+1 block -> full state on -1..N+2 (other ghosts zero)."""
     nx, ny, nz = p.n

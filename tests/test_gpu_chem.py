@@ -1,0 +1,116 @@
+"""HIP chemistry path (SHIELD, chemistry Fcool, network) against the
+reference fixtures and the oracle, through the C ABI.
+
+Tolerances as in test_gpu_parity.py: FP64 relative L-infinity error per
+field, 1e-12 for one step, 1e-11 over several; the device exp/log/pow differ
+from glibc's by at most an ulp, and SHIELD's prefix sums run as wave scans
+whose additions are grouped differently from the reference's sequential
+sums (round-off only).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from code_tohoku_amd import ics, mcf
+from code_tohoku_amd.restart import Input3D
+from helpers import case_cooling, expand, interior, load_case, rel_linf, worst
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-12
+MULTI_TOL = 1e-11
+COOLING = {"ki": mcf.COOL_KI, "chem": mcf.COOL_CHEM}
+
+
+@pytest.mark.parametrize("name,steps", [("chem_rad_24x8x8", 1), ("chem_rad_24x8x8", 6), ("chem_rad_48x8x8", 6),
+                                        ("chem_norad_24x8x8", 6)])
+def test_chem_reference_fixture(name, steps):
+    p, nsplit, arr = load_case(name)
+    print("case", name, steps, flush=True)
+    g = mcf.Domain(p, nsplit=nsplit, cooling=COOLING[case_cooling(name)])
+    g.upload(expand(arr["input"], p))
+    dts = g.run(steps)
+    assert len(dts) == steps
+    np.testing.assert_allclose(dts, arr[f"dts{steps}"], rtol=1e-13, atol=0)
+    errs = rel_linf(interior(g.download(), p), interior(expand(arr[f"out{steps}"], p), p))
+    print(name, steps, "worst", worst(errs))
+    assert worst(errs) <= (STEP_TOL if steps == 1 else MULTI_TOL), errs
+
+
+def _col_errs(a, b, p):
+    nx, ny, nz = p.n
+    out = {}
+    for q, name in enumerate(mcf.COLUMNS):
+        for d in range(2):
+            x = a[q, d, 2:nz + 2, 2:ny + 2, 2:nx + 2]
+            y = b[q, d, 2:nz + 2, 2:ny + 2, 2:nx + 2]
+            out[f"{name}{d + 1}"] = float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300))
+    return out
+
+
+@pytest.mark.parametrize("nsplit", [(2, 2, 2), (4, 2, 2)])
+def test_shield_columns_vs_oracle(nsplit):
+    """The column densities after INITIA's 20 SHIELD sweeps and after two
+    steps against the oracle's."""
+    p, _, arr = load_case("chem_rad_48x8x8")
+    st = expand(arr["input"], p)
+    print("shield", nsplit, flush=True)
+    o = oracle.Oracle(p, st, nsplit=nsplit, cooling="chem")
+    g = mcf.Domain(p, nsplit=nsplit, cooling=mcf.COOL_CHEM)
+    g.upload(st)
+    errs = _col_errs(g.columns(), o.columns(), p)
+    print("init columns worst", max(errs.values()))
+    assert max(errs.values()) <= STEP_TOL, errs
+    dts_o, _ = o.run(2)
+    dts_g = g.run(2)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13, atol=0)
+    errs = _col_errs(g.columns(), o.columns(), p)
+    print("2-step columns worst", max(errs.values()))
+    assert max(errs.values()) <= MULTI_TOL, errs
+
+
+@pytest.mark.parametrize("ifchem,ifrad,cooling", [(2, 2, "ki"), (2, 0, "ki"), (0, 2, "chem"), (0, 0, "chem")])
+def test_chem_combinations_vs_oracle(ifchem, ifrad, cooling):
+    """Each switch on its own: the network with the KI Fcool, SHIELD feeding
+    only the chemistry Fcool, and the chemistry Fcool with zero columns; 48^3
+    converging flow (4 x blocks), 3 steps."""
+    p = ics.converging_flow_input(48)
+    base = Input3D()
+    for k in ("Hini", "pini", "H2ini", "Heini", "Hepini", "Cini", "COini", "Cpini", "pinit_over_kb"):
+        setattr(p, k, getattr(base, k))
+    p.ifchem, p.ifrad = ifchem, ifrad
+    st = ics.converging_flow_state(p)
+    print("combo", ifchem, ifrad, cooling, flush=True)
+    o = oracle.Oracle(p, st, nsplit=(4, 2, 2), cooling=cooling)
+    dts_o, _ = o.run(3)
+    g = mcf.Domain(p, nsplit=(4, 2, 2), cooling=COOLING[cooling])
+    g.upload(st)
+    dts_g = g.run(3)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13, atol=0)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    print("combo worst", worst(errs))
+    assert worst(errs) <= MULTI_TOL, errs
+
+
+def test_chem_mixed_precision():
+    """MCF_PREC_MIXED with the chemistry Fcool: the sweeps and CT in mixed
+    precision, SHIELD, the cooling and the network in FP64; 6 steps of the
+    chem_rad fixture on its dt sequence, against the reference at the
+    mixed-precision bar used for the other configurations (1e-5)."""
+    p, nsplit, arr = load_case("chem_rad_24x8x8")
+    g = mcf.Domain(p, nsplit=nsplit, precision=mcf.PREC_MIXED, cooling=mcf.COOL_CHEM)
+    g.upload(expand(arr["input"], p))
+    for dt in arr["dts6"]:
+        g.step(float(dt))
+    errs = rel_linf(interior(g.download(), p), interior(expand(arr["out6"], p), p))
+    print("chem mixed worst", worst(errs))
+    assert worst(errs) <= 1e-5, errs
+
+
+def test_columns_need_chemistry():
+    """mcf_download_columns on a KI run without network or SHIELD: no
+    columns are carried, and the call says so."""
+    p = ics.converging_flow_input(16, nsplit=(1, 1, 1))
+    g = mcf.Domain(p, nsplit=(1, 1, 1))
+    with pytest.raises(mcf.MCFError, match="no column densities"):
+        g.columns()

@@ -14,16 +14,17 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import expand, faces, interior, load_case, rel_linf, worst
+from helpers import case_cooling, expand, faces, interior, load_case, rel_linf, worst
 
 CASES = [("inflow_24x8x8", 1), ("inflow_24x8x8", 6), ("periodic_24x8x8", 1), ("periodic_24x8x8", 6),
-         ("inflow_48x8x8", 6), ("briowu_64x8x8", 200), ("converging_24", 8)]
+         ("inflow_48x8x8", 6), ("briowu_64x8x8", 200), ("converging_24", 8),
+         ("chem_rad_24x8x8", 1), ("chem_rad_24x8x8", 6), ("chem_rad_48x8x8", 6), ("chem_norad_24x8x8", 6)]
 
 
 @pytest.mark.parametrize("name,steps", CASES)
 def test_oracle_matches_reference(name, steps):
     p, nsplit, arr = load_case(name)
-    o = oracle.Oracle(p, expand(arr["input"], p), nsplit=nsplit)
+    o = oracle.Oracle(p, expand(arr["input"], p), nsplit=nsplit, cooling=case_cooling(name))
     dts, _ = o.run(steps)
     assert len(dts) == steps
     np.testing.assert_array_equal(dts, arr[f"dts{steps}"])
