@@ -712,8 +712,17 @@ static int do_ct(mcf_ctx *x, double dt) {
   }
   // the three edge orientations in one launch over (1..n+1)^3: reads rho, p,
   // Vfc(3), face B(3), writes EMF(3)
-  if (x->mixed) KLAUNCH_N("k_cct3", k_cct3<float>, dim3(nblocks(boxsize(b1))), dim3(TPB), 8.0 * boxsize(b1) * 11, A, x->g, b1, dt);
-  else KLAUNCH_N("k_cct3", k_cct3<double>, dim3(nblocks(boxsize(b1))), dim3(TPB), 8.0 * boxsize(b1) * 11, A, x->g, b1, dt);
+  {
+    // z march over 61-cell rows, CTM_ROWS rows and CTM_ZSEG planes per
+    // block.  A variant with full 64-cell tiles (the tile's outer
+    // neighbours loaded by the edge lanes, the 257th column per cell) was
+    // slower: 1.08 / 2.11 ms against 1.00 / 1.77 (mixed / FP64), its extra
+    // registers cost more than the idle lanes of the fifth tile.
+    dim3 gr((unsigned)((n[0] + 1 + CTM_OUT - 1) / CTM_OUT), (unsigned)((n[1] + 1 + CTM_ROWS - 1) / CTM_ROWS),
+            (unsigned)((n[2] + 1 + CTM_ZSEG - 1) / CTM_ZSEG));
+    if (x->mixed) KLAUNCH_N("k_ct_march", k_ct_march<float>, gr, dim3(64 * CTM_ROWS), 8.0 * boxsize(b1) * 11, A, x->g, dt);
+    else KLAUNCH_N("k_ct_march", k_ct_march<double>, gr, dim3(64 * CTM_ROWS), 8.0 * boxsize(b1) * 11, A, x->g, dt);
+  }
   KLAUNCH512(k_cc3, boxsize(b1), 9, A, x->g, b1, dt);
   return do_cc4(x);
 }

@@ -192,7 +192,11 @@ struct DisArgs {
 __global__ void k_fill_yz(FillList L, Geo g, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
-template <class R> __global__ void k_cct3(CtArgs A, Geo g, Box b, double dt);
+template <class R> __global__ void k_ct_march(CtArgs A, Geo g, double dt);
+// k_ct_march: EMF cells per wave along x, rows (waves) per block, planes
+// per block (16, 32 and 64 planes measured 1.01 / 1.04 / 1.11 ms, mixed,
+// 256^3; 2 and 8 rows no faster than 4)
+constexpr int CTM_OUT = 61, CTM_ROWS = 4, CTM_ZSEG = 16;
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt);
 __global__ void k_cc4(CtArgs A, Geo g, Box b);
 // CHEM: the chemistry-network Fcool (R = double only) instead of the KI one

@@ -168,52 +168,29 @@ __global__ void k_cc1(CtArgs A, Geo g, Box b) {
   }
 }
 
-// CCT (M3:2186-2445) at the edge of orientation (d1, d2) of a cell.  The
-// edge's transverse slopes dQ (M3:2203-2268) are computed where they are
-// used: dQ1/dQ3 (along d2 of Vfc_d1 and B_d1) at the cells c0-s2 and c0,
-// dQ2/dQ4 (along d1 of Vfc_d2 and B_d2) at c0-s1 and c0, each with that
-// cell's limiter switch.  Then the four candidate pairs with their 2x2
-// corrections (M3:2278-2385), the flags and 1e13 weights (M3:2387-2415), the
-// weighted averages (M3:2419-2433) and EMF = 1/4 (v1 B2 - v2 B1) sqrt(dnc).
-// Arithmetic in R (FP32 on the mixed path).
-template <class R>
-__device__ __forceinline__ double cct_edge(const CtArgs &A, const Geo &g, int i, int j, int k, int orient,
-                                         double dt_d) {
+// What CCT needs at one edge of orientation (d1, d2), corner c0 - s1/2 -
+// s2/2: the face values of V_d1, B_d1 at c0 - s2 ([0]) and c0 ([1]) and of
+// V_d2, B_d2 at c0 - s1 ([0]) and c0 ([1]); their transverse VLIMIT slopes
+// dQ0/dQ2 (of V_d1, B_d1 along d2) and dQ1/dQ3 (of V_d2, B_d2 along d1) at
+// the same cells; the widths dy = dx_d2 at p2 and p2+1 ... as dye[p2],
+// dye[p2+1] and dx = dxe[p1], dxe[p1+1]; and the corner density dnc.
+template <class R> struct EdgeIn {
+  R v1[2], b1[2], v2[2], b2[2];
+  R dq0[2], dq2[2], dq1[2], dq3[2];
+  R dy[2], dx[2];
+  double dnc;
+};
+
+// CCT (M3:2186-2445) from the edge's inputs: the four candidate pairs with
+// their 2x2 corrections (M3:2278-2385), the flags and 1e13 weights
+// (M3:2387-2415), the weighted averages (M3:2419-2433) and EMF =
+// 1/4 (v1 B2 - v2 B1) sqrt(dnc).  Arithmetic in R (FP32 on the mixed path).
+template <class R> __device__ __forceinline__ double cct_core(const EdgeIn<R> &E, double facdep, double dt_d) {
   using Q = Ar<R>;
-  const int d1 = orient, d2 = (orient + 1) % 3;
-  const int c0 = ix3(g, i, j, k);
-  const int s1 = g.st[d1], s2 = g.st[d2];
-  const int p1 = d1 == 0 ? i : d1 == 1 ? j : k;
-  const int p2 = d2 == 0 ? i : d2 == 1 ? j : k;
-  const double *r = A.U[0];
-  const double *Vx = A.vfc[d1], *Vy = A.vfc[d2], *Bx = A.U[5 + d1], *By = A.U[5 + d2];
-  const double *dxe = A.gdx[d1], *dye = A.gdx[d2];
-  // transverse slopes at the four cells the candidates use
-  const bool mmA = use_minmod_r<R>(A.U[4][c0 - s2], r[c0 - s2]);
-  const bool mmB = use_minmod_r<R>(A.U[4][c0 - s1], r[c0 - s1]);
-  const bool mm0 = use_minmod_r<R>(A.U[4][c0], r[c0]);
-  R dQ0[2], dQ2[2], dQ1[2], dQ3[2]; // [0]: cell c0-s2 (dQ0/2) or c0-s1 (dQ1/3), [1]: c0
-  {
-    // along d2 at c0-s2 (position p2-1) and c0 (position p2)
-    const R ym = (R)dye[p2 - 1], y0 = (R)dye[p2], y1 = (R)dye[p2 + 1], y2 = (R)dye[p2 + 2];
-    dQ0[0] = vslope<R>(Vx[c0 - 2 * s2], Vx[c0 - s2], Vx[c0], ym, y0, y1, mmA);
-    dQ2[0] = vslope<R>(Bx[c0 - 2 * s2], Bx[c0 - s2], Bx[c0], ym, y0, y1, mmA);
-    dQ0[1] = vslope<R>(Vx[c0 - s2], Vx[c0], Vx[c0 + s2], y0, y1, y2, mm0);
-    dQ2[1] = vslope<R>(Bx[c0 - s2], Bx[c0], Bx[c0 + s2], y0, y1, y2, mm0);
-    // along d1 at c0-s1 (position p1-1) and c0 (position p1)
-    const R xm = (R)dxe[p1 - 1], x0 = (R)dxe[p1], x1 = (R)dxe[p1 + 1], x2 = (R)dxe[p1 + 2];
-    dQ1[0] = vslope<R>(Vy[c0 - 2 * s1], Vy[c0 - s1], Vy[c0], xm, x0, x1, mmB);
-    dQ3[0] = vslope<R>(By[c0 - 2 * s1], By[c0 - s1], By[c0], xm, x0, x1, mmB);
-    dQ1[1] = vslope<R>(Vy[c0 - s1], Vy[c0], Vy[c0 + s1], x0, x1, x2, mm0);
-    dQ3[1] = vslope<R>(By[c0 - s1], By[c0], By[c0 + s1], x0, x1, x2, mm0);
-  }
-  double dncd;
-  if (orient == 2) dncd = 0.25 * (r[c0] + r[c0 - s2] + r[c0 - s1] + r[c0 - s1 - s2]);
-  else dncd = 0.25 * (r[c0] + r[c0 - s1] + r[c0 - s2] + r[c0 - s1 - s2]);
-  const R dnc = (R)dncd;
+  const R dnc = (R)E.dnc;
   const R dt = (R)dt_d;
   const R dsqinv = Q::div((R)1.0, Q::sqrt_(dnc));
-  const R facm = (R)A.facdep * (R)0.5;
+  const R facm = (R)facdep * (R)0.5;
   R C1p[4], C1m[4], C2p[4], C2m[4];
 #pragma unroll
   for (int cs = 0; cs < 4; ++cs) {
@@ -222,19 +199,17 @@ __device__ __forceinline__ double cct_edge(const CtArgs &A, const Geo &g, int i,
     const R S2 = (cs == 0 || cs == 1) ? (R)1.0 : (R)-1.0;
     const int u1 = (cs == 0 || cs == 2) ? 0 : 1; // q1 = c0 - s2 or c0
     const int u2 = (cs == 0 || cs == 1) ? 0 : 1; // q2 = c0 - s1 or c0
-    const int q1 = u1 ? c0 : c0 - s2;
-    const int q2 = u2 ? c0 : c0 - s1;
-    const R dyv = (R)(u1 ? dye[p2 + 1] : dye[p2]);
-    const R dxv = (R)(u2 ? dxe[p1 + 1] : dxe[p1]);
-    const R vx = (R)Vx[q1], bx = (R)Bx[q1], vy = (R)Vy[q2], by = (R)By[q2];
+    const R dyv = E.dy[u1];
+    const R dxv = E.dx[u2];
+    const R vx = E.v1[u1], bx = E.b1[u1], vy = E.v2[u2], by = E.b2[u2];
     R a1p = vx + bx * dsqinv;
-    R da1p = (dQ0[u1] + dQ2[u1] * dsqinv) * facm;
+    R da1p = (E.dq0[u1] + E.dq2[u1] * dsqinv) * facm;
     R a1m = vx - bx * dsqinv;
-    R da1m = (dQ0[u1] - dQ2[u1] * dsqinv) * facm;
+    R da1m = (E.dq0[u1] - E.dq2[u1] * dsqinv) * facm;
     R a2m = vy - by * dsqinv;
-    R da2m = (dQ1[u2] - dQ3[u2] * dsqinv) * facm;
+    R da2m = (E.dq1[u2] - E.dq3[u2] * dsqinv) * facm;
     R a2p = vy + by * dsqinv;
-    R da2p = (dQ1[u2] + dQ3[u2] * dsqinv) * facm;
+    R da2p = (E.dq1[u2] + E.dq3[u2] * dsqinv) * facm;
     R t;
     t = a1p + da1p * dyv * S1 - da1p * (a2m + da2m * dxv * S2) * dt;
     C1p[cs] = Q::div(t, (R)1.0 - da1p * da2m * dt * dt);
@@ -285,29 +260,151 @@ __device__ __forceinline__ double cct_edge(const CtArgs &A, const Geo &g, int i,
   return (double)((R)0.25 * (v1a * B2a - v2a * B1a) * Q::sqrt_(dnc));
 }
 
-// The three edge orientations of a cell in one thread, so the face fields
-// each orientation shares with another are read once.  Box (1..n+1)^3; the
-// edges of orientation o exist where the position along its edge direction
-// d3 = (o+2)%3 is at most n[d3].
+// CCT for the whole box (1..n+1)^3 as a march along z, every transverse
+// slope formed once per cell.  A wave holds 64 consecutive cells of one
+// (j, k) row, lane l at x = ib - 2 + l, and writes the EMFs of lanes 2..62
+// (61 cells: the x slopes at i and i-1 need the values at i-2..i+1).  Per
+// plane k each lane forms, with its own cell's limiter switch:
+//   along x: V_y, B_y, V_z, B_z from its neighbours' values (lane shuffles);
+//            the slopes at i-1 then come from the left lane;
+//   along y: V_x, B_x, V_z, B_z at rows j and j-1 (rows j-2..j+1 loaded;
+//            the slope at j-1 is also formed by the wave of row j-1);
+//   along z: V_x, B_x, V_y, B_y at plane k (planes k-1, k+1 in registers),
+//            the slopes at k-1 kept from the previous plane.
+// The three edges of the cell then go through cct_core.  Against a kernel
+// with one thread per cell that formed all 24 slopes its three edges use
+// (each slope twice), the march took 1.00 / 1.77 ms instead of 1.12 / 1.93
+// (mixed / FP64, 256^3) with bit-identical EMFs; sharing the row-(j-1)
+// slopes through LDS instead of forming them in each wave needs a barrier
+// per plane and took 1.29 / 1.94.
 template <class R>
-__global__ __launch_bounds__(256) void k_cct3(CtArgs A, Geo g, Box b, double dt_d) {
-  int i, j, k;
-  if (!box_ijk(b, i, j, k)) return;
-  // the three EMFs are stored after all are formed, so the loads and limiter
-  // switches the orientations share are not repeated after a store
-  const int c = ix3(g, i, j, k);
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-  if (k <= g.n[2]) e0 = cct_edge<R>(A, g, i, j, k, 0, dt_d);
-  if (i <= g.n[0]) e1 = cct_edge<R>(A, g, i, j, k, 1, dt_d);
-  if (j <= g.n[1]) e2 = cct_edge<R>(A, g, i, j, k, 2, dt_d);
-  if (k <= g.n[2]) A.emf[2][c] = e0;
-  if (i <= g.n[0]) A.emf[0][c] = e1;
-  if (j <= g.n[1]) A.emf[1][c] = e2;
+__global__ __launch_bounds__(64 * CTM_ROWS) void k_ct_march(CtArgs A, Geo g, double dt_d) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int j = 1 + CTM_ROWS * (int)blockIdx.y + w;
+  if (j > ny + 1) return; // whole waves; no barrier in this kernel
+  const int ib = 1 + CTM_OUT * (int)blockIdx.x;
+  const int i = ib - 2 + lane;
+  const int ic = min(i, nx + 2); // lanes past the last ghost read the last ghost
+  const bool outx = lane >= 2 && lane <= 62 && i <= nx + 1;
+  const int k0 = 1 + CTM_ZSEG * (int)blockIdx.z;
+  const int k1 = min(nz + 1, k0 + CTM_ZSEG - 1);
+  const int sy = g.st[1], sz = g.st[2];
+  const double *Vx = A.vfc[0], *Vy = A.vfc[1], *Vz = A.vfc[2];
+  const double *Bx = A.U[5], *By = A.U[6], *Bz = A.U[7], *rh = A.U[0], *pp = A.U[4];
+  // widths: x at the lane's cell (indices clamped to the dx array for the
+  // halo lanes, whose slopes are not used), y at row j
+  const double *gx = A.gdx[0], *gy = A.gdx[1], *gz = A.gdx[2];
+  const int ix0 = max(0, min(ic, nx + 3)), ix1 = min(ic + 1, nx + 3), ix2 = min(ic + 2, nx + 3);
+  const R xw0 = (R)gx[ix0], xw1 = (R)gx[ix1], xw2 = (R)gx[ix2];
+  const R yw_m = (R)gy[j - 1], yw0 = (R)gy[j], yw1 = (R)gy[j + 1], yw2 = (R)gy[j + 2];
+  // z window of the own column: V_x, B_x, V_y, B_y at planes k-1 and k,
+  // their z slopes at k-1, and rho at k-1 in rows j and j-1
+  int c = ix3(g, ic, j, k0 - 1);
+  double wVx0 = Vx[c], wBx0 = Bx[c], wVy0 = Vy[c], wBy0 = By[c];
+  double wVx1 = Vx[c + sz], wBx1 = Bx[c + sz], wVy1 = Vy[c + sz], wBy1 = By[c + sz];
+  double rho_km = rh[c], rhoJ_km = rh[c - sy];
+  R zVx0, zBx0, zVy0, zBy0;
+  {
+    const bool mm = use_minmod_r<R>(pp[c], rh[c]);
+    const R z0 = (R)gz[k0 - 1], z1 = (R)gz[k0], z2 = (R)gz[k0 + 1];
+    const int cm = c - sz;
+    zVx0 = vslope<R>(Vx[cm], wVx0, wVx1, z0, z1, z2, mm);
+    zBx0 = vslope<R>(Bx[cm], wBx0, wBx1, z0, z1, z2, mm);
+    zVy0 = vslope<R>(Vy[cm], wVy0, wVy1, z0, z1, z2, mm);
+    zBy0 = vslope<R>(By[cm], wBy0, wBy1, z0, z1, z2, mm);
+  }
+  for (int k = k0; k <= k1; ++k) {
+    c += sz; // (ic, j, k)
+    // loads: own column at k+1 (z window) and k; rows j-1, j-2, j+1 at k
+    const double nVx = Vx[c + sz], nBx = Bx[c + sz], nVy = Vy[c + sz], nBy = By[c + sz];
+    const double vz = Vz[c], bz = Bz[c], rho = rh[c], p = pp[c];
+    const int cJ = c - sy, cJJ = c - 2 * sy, cP = c + sy;
+    const double VxJ = Vx[cJ], BxJ = Bx[cJ], VzJ = Vz[cJ], BzJ = Bz[cJ], rhoJ = rh[cJ], pJ = pp[cJ];
+    const double VxJJ = Vx[cJJ], BxJJ = Bx[cJJ], VzJJ = Vz[cJJ], BzJJ = Bz[cJJ];
+    const double VxP = Vx[cP], BxP = Bx[cP], VzP = Vz[cP], BzP = Bz[cP];
+    const bool mm0 = use_minmod_r<R>(p, rho);
+    const bool mmJ = use_minmod_r<R>(pJ, rhoJ);
+    const R zw0 = (R)gz[k], zw1 = (R)gz[k + 1], zw2 = (R)gz[k + 2];
+    // along z at k
+    const R zVx1 = vslope<R>(wVx0, wVx1, nVx, zw0, zw1, zw2, mm0);
+    const R zBx1 = vslope<R>(wBx0, wBx1, nBx, zw0, zw1, zw2, mm0);
+    const R zVy1 = vslope<R>(wVy0, wVy1, nVy, zw0, zw1, zw2, mm0);
+    const R zBy1 = vslope<R>(wBy0, wBy1, nBy, zw0, zw1, zw2, mm0);
+    // along y at j and j-1
+    const R yVx1 = vslope<R>(VxJ, wVx1, VxP, yw0, yw1, yw2, mm0);
+    const R yBx1 = vslope<R>(BxJ, wBx1, BxP, yw0, yw1, yw2, mm0);
+    const R yVz1 = vslope<R>(VzJ, vz, VzP, yw0, yw1, yw2, mm0);
+    const R yBz1 = vslope<R>(BzJ, bz, BzP, yw0, yw1, yw2, mm0);
+    const R yVx0 = vslope<R>(VxJJ, VxJ, wVx1, yw_m, yw0, yw1, mmJ);
+    const R yBx0 = vslope<R>(BxJJ, BxJ, wBx1, yw_m, yw0, yw1, mmJ);
+    const R yVz0 = vslope<R>(VzJJ, VzJ, vz, yw_m, yw0, yw1, mmJ);
+    const R yBz0 = vslope<R>(BzJJ, BzJ, bz, yw_m, yw0, yw1, mmJ);
+    // along x at i (neighbour values from the adjacent lanes), then at i-1
+    const double VyL = __shfl_up(wVy1, 1, 64), VyR = __shfl_down(wVy1, 1, 64);
+    const double ByL = __shfl_up(wBy1, 1, 64), ByR = __shfl_down(wBy1, 1, 64);
+    const double VzL = __shfl_up(vz, 1, 64), VzR = __shfl_down(vz, 1, 64);
+    const double BzL = __shfl_up(bz, 1, 64), BzR = __shfl_down(bz, 1, 64);
+    const R xVy1 = vslope<R>(VyL, wVy1, VyR, xw0, xw1, xw2, mm0);
+    const R xBy1 = vslope<R>(ByL, wBy1, ByR, xw0, xw1, xw2, mm0);
+    const R xVz1 = vslope<R>(VzL, vz, VzR, xw0, xw1, xw2, mm0);
+    const R xBz1 = vslope<R>(BzL, bz, BzR, xw0, xw1, xw2, mm0);
+    const R xVy0 = __shfl_up(xVy1, 1, 64), xBy0 = __shfl_up(xBy1, 1, 64);
+    const R xVz0 = __shfl_up(xVz1, 1, 64), xBz0 = __shfl_up(xBz1, 1, 64);
+    const double rhoL = __shfl_up(rho, 1, 64), rhoJL = __shfl_up(rhoJ, 1, 64), rhoKL = __shfl_up(rho_km, 1, 64);
+    if (outx) {
+      double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+      if (k <= nz) { // edge along z: (d1, d2) = (x, y)
+        EdgeIn<R> E;
+        E.v1[0] = (R)VxJ; E.b1[0] = (R)BxJ; E.v1[1] = (R)wVx1; E.b1[1] = (R)wBx1;
+        E.v2[0] = (R)VyL; E.b2[0] = (R)ByL; E.v2[1] = (R)wVy1; E.b2[1] = (R)wBy1;
+        E.dq0[0] = yVx0; E.dq2[0] = yBx0; E.dq0[1] = yVx1; E.dq2[1] = yBx1;
+        E.dq1[0] = xVy0; E.dq3[0] = xBy0; E.dq1[1] = xVy1; E.dq3[1] = xBy1;
+        E.dy[0] = yw0; E.dy[1] = yw1;
+        E.dx[0] = xw0; E.dx[1] = xw1;
+        E.dnc = 0.25 * (rho + rhoL + rhoJ + rhoJL);
+        e0 = cct_core<R>(E, A.facdep, dt_d);
+      }
+      if (i <= nx) { // edge along x: (d1, d2) = (y, z)
+        EdgeIn<R> E;
+        E.v1[0] = (R)wVy0; E.b1[0] = (R)wBy0; E.v1[1] = (R)wVy1; E.b1[1] = (R)wBy1;
+        E.v2[0] = (R)VzJ; E.b2[0] = (R)BzJ; E.v2[1] = (R)vz; E.b2[1] = (R)bz;
+        E.dq0[0] = zVy0; E.dq2[0] = zBy0; E.dq0[1] = zVy1; E.dq2[1] = zBy1;
+        E.dq1[0] = yVz0; E.dq3[0] = yBz0; E.dq1[1] = yVz1; E.dq3[1] = yBz1;
+        E.dy[0] = zw0; E.dy[1] = zw1;
+        E.dx[0] = yw0; E.dx[1] = yw1;
+        E.dnc = 0.25 * (rho + rhoJ + rho_km + rhoJ_km);
+        e1 = cct_core<R>(E, A.facdep, dt_d);
+      }
+      if (j <= ny) { // edge along y: (d1, d2) = (z, x)
+        EdgeIn<R> E;
+        E.v1[0] = (R)VzL; E.b1[0] = (R)BzL; E.v1[1] = (R)vz; E.b1[1] = (R)bz;
+        E.v2[0] = (R)wVx0; E.b2[0] = (R)wBx0; E.v2[1] = (R)wVx1; E.b2[1] = (R)wBx1;
+        E.dq0[0] = xVz0; E.dq2[0] = xBz0; E.dq0[1] = xVz1; E.dq2[1] = xBz1;
+        E.dq1[0] = zVx0; E.dq3[0] = zBx0; E.dq1[1] = zVx1; E.dq3[1] = zBx1;
+        E.dy[0] = xw0; E.dy[1] = xw1;
+        E.dx[0] = zw0; E.dx[1] = zw1;
+        E.dnc = 0.25 * (rho + rhoL + rho_km + rhoKL);
+        e2 = cct_core<R>(E, A.facdep, dt_d);
+      }
+      if (k <= nz) A.emf[2][c] = e0;
+      if (i <= nx) A.emf[0][c] = e1;
+      if (j <= ny) A.emf[1][c] = e2;
+    }
+    // advance the z window
+    wVx0 = wVx1; wBx0 = wBx1; wVy0 = wVy1; wBy0 = wBy1;
+    wVx1 = nVx; wBx1 = nBx; wVy1 = nVy; wBy1 = nBy;
+    zVx0 = zVx1; zBx0 = zBx1; zVy0 = zVy1; zBy0 = zBy1;
+    rho_km = rho; rhoJ_km = rhoJ;
+  }
 }
-template __global__ void k_cct3<double>(CtArgs, Geo, Box, double);
-template __global__ void k_cct3<float>(CtArgs, Geo, Box, double);
+template __global__ void k_ct_march<double>(CtArgs, Geo, double);
+template __global__ void k_ct_march<float>(CtArgs, Geo, double);
 
-// CC(3): face-B induction (M3:2159-2169)
+// CC(3): face-B induction (M3:2159-2169).  A variant fused with CC(4),
+// out of place over the whole box (each thread also forming the new faces
+// at i+1, j+1, k+1 for its cell-centred field), took 0.57 ms against 0.40
+// for the two kernels (256^3).
 __global__ void k_cc3(CtArgs A, Geo g, Box b, double dt) {
   int i, j, k;
   if (!box_ijk(b, i, j, k)) return;

@@ -11,6 +11,7 @@ module mcf_abi_mod
 
   integer(c_int), parameter :: MCF_OK = 0
   integer(c_int), parameter :: MCF_PREC_FP64 = 0, MCF_PREC_MIXED = 1
+  integer(c_int), parameter :: MCF_COOL_KI = 1, MCF_COOL_SURROGATE = 2, MCF_COOL_CHEM = 3
 
   type, bind(C) :: mcf_config
     integer(c_int) :: np1x, np2x, np1y, np2y, np1z, np2z
@@ -56,6 +57,11 @@ module mcf_abi_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: bcc3(*)
+    end function
+    integer(c_int) function mcf_download_columns(ctx, col10) bind(C, name='mcf_download_columns')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: col10(*)
     end function
     integer(c_int) function mcf_set_time(ctx, time, ifevo) bind(C, name='mcf_set_time')
       import :: c_int, c_ptr, c_double

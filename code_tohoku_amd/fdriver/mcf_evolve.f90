@@ -13,7 +13,9 @@
 ! Couran+Stblty+reduction, mcf_step for SOURCE / MHD x3 / CT / SOURCE /
 ! DISSIP).
 !
-! Environment: MCF_WALL_HOURS overrides the 11.7-hour wall-clock limit.
+! Environment: MCF_WALL_HOURS overrides the 11.7-hour wall-clock limit;
+! MCF_COOLING=chem selects the reference's chemistry-network Fcool (default:
+! the Koyama-Inutsuka Fcool); MCF_PRECISION=mixed the mixed-precision path.
 !
 ! usage: mcf_evolve [rundir [NPE]]   (NPE: rank count of the reproduced run,
 !                                     default 8; fixes the block split)
@@ -52,6 +54,30 @@ program mcf_evolve
   wall_limit = 11.7d0
   call get_environment_variable('MCF_WALL_HOURS', arg, status=ios)
   if (ios == 0) read(arg, *) wall_limit
+  call get_environment_variable('MCF_COOLING', arg, status=ios)
+  if (ios == 0) then
+    select case (trim(arg))
+    case ('chem')
+      cfg%cooling = MCF_COOL_CHEM
+    case ('ki')
+      cfg%cooling = MCF_COOL_KI
+    case default
+      write(0, *) 'MCF_COOLING must be ki or chem'
+      stop 2
+    end select
+  end if
+  call get_environment_variable('MCF_PRECISION', arg, status=ios)
+  if (ios == 0) then
+    select case (trim(arg))
+    case ('mixed')
+      cfg%precision = MCF_PREC_MIXED
+    case ('fp64')
+      cfg%precision = MCF_PREC_FP64
+    case default
+      write(0, *) 'MCF_PRECISION must be fp64 or mixed'
+      stop 2
+    end select
+  end if
 
   open(2, file=trim(rundir)//'/tsave.DAT')
   read(2, *) time

@@ -83,3 +83,30 @@ def test_fortran_driver_slices_and_cdnt(tmp_path):
             scale = np.abs(ref[u, :, :, :, :]).max(axis=(0, 1, 2))
             assert np.all(np.abs(got - ref[u, r]) <= 2.0 ** -22 * scale), (u, r)
     assert not os.path.exists(os.path.join(run, f"2D{ref.shape[0]:03d}000.dat"))
+
+
+def test_fortran_driver_chemistry(tmp_path):
+    """The chem_rad fixture (chemistry network, SHIELD, the reference's own
+    chemistry Fcool) through the Fortran driver with MCF_COOLING=chem: the
+    INPUT3D.DAT switches ifchem = ifrad = 2 reach the library, and the 6-step
+    restart files match the reference's."""
+    p, nsplit, arr = load_case("chem_rad_24x8x8")
+    assert p.ifchem == 2 and p.ifrad == 2
+    run = str(tmp_path)
+    q = rs.Input3D(**p.__dict__)
+    q.maxstp, q.nitera = 1, 6
+    with open(os.path.join(run, "INPUT3D.DAT"), "w") as f:
+        f.write(q.to_text())
+    rs.write_counters(run, 0.0, 2)
+    for r, b in enumerate(rs.split_state(expand(arr["input"], p), nsplit)):
+        rs.write_restart_rank(os.path.join(run, f"000{r:03d}.dat"), b)
+    env = dict(os.environ, MCF_COOLING="chem")
+    res = subprocess.run([build.EVOLVE, run, "8"], capture_output=True, text=True, timeout=120, env=env)
+    assert res.returncode == 0, res.stderr
+    nx, ny, nz = p.n
+    blocks = [rs.read_restart_rank(os.path.join(run, f"000{r:03d}.dat"), nx // 2, ny // 2, nz // 2)[0]
+              for r in range(8)]
+    out = rs.assemble_state(blocks, p.n, nsplit)
+    errs = rel_linf(interior(out, p), interior(expand(arr["out6"], p), p))
+    print("fortran chem worst", worst(errs))
+    assert worst(errs) <= 1e-11, errs
