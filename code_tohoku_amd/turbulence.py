@@ -7,8 +7,10 @@ rms Mach 5, B = 3 muG along z) and advance with the FP64 run's dt sequence.
 Reported: the density PDF of both runs and their distance (L1 and
 Kolmogorov-Smirnov), the shell-averaged kinetic and magnetic energy spectra
 and their relative differences, per-cell relative L1 / L-infinity errors of
-rho, p and B, max|div B|, the surrogate's cell counts, and the time per step
-of each path at this size.
+rho, p and B, max|div B|, the backward-Euler cell counts of each path per
+phase (the comparison steps, its first ten steps, each throughput run) with
+the surrogate's IMC fallbacks, and the time per step of each path at this
+size.
 
     python -m code_tohoku_amd.turbulence [--n 512] [--steps 500] [--out FILE]
 """
@@ -73,11 +75,27 @@ def run(n: int = 512, steps: int = 500, log=print) -> dict:
     b.upload(st)
     b.mlp_set_weights(w)
     del st
+    # backward-Euler cells (and, for the surrogate, IMC fallbacks) counted
+    # per phase: the comparison steps, then each path's throughput steps
+    stiff = {}
+
+    def phase_stats(name, d, reset_after=True):
+        nst, nfb = d.cool_stats(reset=reset_after)
+        stiff[name] = {"backward_euler": nst, "imc_fallbacks": nfb,
+                       "fallback_fraction": (nfb / nst) if nst else None}
+
+    a.cool_stats(reset=True)
+    b.cool_stats(reset=True)
     t0 = time.perf_counter()
+    first = {}
     for k in range(steps):
         dt, _tag = a.timestep(1e30)
         a.step(dt)
         b.step(dt)
+        if k + 1 == min(10, steps):
+            # the first steps separately: the initial 100 K gas is where the
+            # backward-Euler branch is taken
+            first = {"fp64": a.cool_stats(reset=False)[0], "mixed+surrogate": b.cool_stats(reset=False)[0]}
         if (k + 1) % 50 == 0:
             b.sync()
             log(f"[turb] step {k + 1}/{steps} t={a.time[0]:.5f} dt={dt:.3e} {time.perf_counter() - t0:.1f} s")
@@ -110,9 +128,9 @@ def run(n: int = 512, steps: int = 500, log=print) -> dict:
                       "kinetic_fp64": ka[1:kmax + 1].tolist(), "kinetic_mixed": kb[1:kmax + 1].tolist(),
                       "magnetic_fp64": ma[1:kmax + 1].tolist(), "magnetic_mixed": mb[1:kmax + 1].tolist(),
                       "kinetic_max_rel_diff": max(rel(ka, kb)), "magnetic_max_rel_diff": max(rel(ma, mb))}
-    nst, nfb = b.cool_stats()
-    res["surrogate_cells"] = {"backward_euler": nst, "imc_fallbacks": nfb,
-                              "fallback_fraction": (nfb / nst) if nst else None}
+    phase_stats("comparison fp64", a)
+    phase_stats("comparison mixed+surrogate", b)
+    stiff["comparison first steps (backward-Euler cells)"] = {"steps": min(10, steps), **first}
     # throughput of each path at this size (10 more steps, each domain alone)
     for name, d in (("fp64", a), ("mixed+surrogate", b)):
         d.run(1, tsave=1e30)
@@ -122,6 +140,8 @@ def run(n: int = 512, steps: int = 500, log=print) -> dict:
         d.sync()
         el = time.perf_counter() - t1
         res.setdefault("throughput", {})[name] = {"ms_per_step": 100.0 * el, "zone_updates_per_s": n ** 3 * 10 / el}
+        phase_stats(f"throughput {name} (11 steps)", d)
+    res["stiff_cells"] = stiff
     a.close()
     b.close()
     return res
