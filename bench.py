@@ -26,7 +26,10 @@ where SOURCE's backward-Euler branch (IMC or the surrogate) does the work.
 The CPU restatement (oracle/, OpenMP on the host's cores) is timed on a
 bounded sample of the same problem as the CPU baseline.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pmc] [--no-stiff]
+``chemistry`` times the same problem with the reference's chemistry network,
+SHIELD column densities and chemistry-network Fcool switched on.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pmc] [--no-stiff] [--no-chem]
 """
 import argparse
 import csv
@@ -124,9 +127,9 @@ def kernel_stage(name):
         return "sweep_y"
     if name.endswith("<2>"):
         return "sweep_z"
-    if name in ("k_cc1", "k_cct3", "k_cc3", "k_cc4"):
+    if name in ("k_cc1", "k_ct_march", "k_cc3", "k_cc4"):
         return "ct"
-    if name in ("k_src_col", "k_shell_copy", "k_src_stiff", "k_src_imc"):
+    if name in ("k_src_col", "k_shell_copy", "k_src_stiff", "k_src_imc", "k_shield", "k_chem"):
         return "source"
     if name in ("k_dtmin", "k_dtfinal"):
         return "dt"
@@ -248,6 +251,46 @@ def stiff_compare(n, steps, warmup, w):
     return out
 
 
+def chem_compare(n, steps, warmup):
+    """The same converging flow with the reference's chemistry switched on:
+    the chemistry network (ifchem = 2), SHIELD column densities (ifrad = 2)
+    and the chemistry-network Fcool, on the FP64 and the mixed path (whose
+    SOURCE runs in FP64 here).  ms/step and the per-kernel times of SHIELD,
+    the network and SOURCE's thermal kernels."""
+    from code_tohoku_amd import mcf
+    p, st = build_state(n)
+    p.ifchem, p.ifrad = 2, 2
+    nsplit = (max(1, n // 64),) * 3
+    out = {"workload": f"converging_flow_{n}^3 with ifchem = 2, ifrad = 2 and the chemistry-network Fcool",
+           "steps": steps,
+           "accounting": "compulsory bytes per cell: SHIELD reads 8 species, p and tCII(2) and writes 10 column "
+                         "arrays (21 doubles); the network reads 8 species, p and 8 columns and writes 8 species "
+                         "and rho (26 doubles)"}
+    cells = float(n) ** 3
+    for tag, prec in (("fp64", mcf.PREC_FP64), ("mixed", mcf.PREC_MIXED)):
+        d = mcf.Domain(p, nsplit=nsplit, precision=prec, cooling=mcf.COOL_CHEM)
+        t0 = time.perf_counter()
+        d.upload(st)
+        d.sync()
+        up = time.perf_counter() - t0
+        el, dts, rep, stg, nprof = timed(d, steps, warmup, profile_steps=5)
+        d.close()
+        kern = {k: {"avg_us": round(1e3 * v[1] / v[0], 2), "launches_per_step": v[0] / nprof}
+                for k, v in rep.items() if k in ("k_shield", "k_chem", "k_src_col", "k_src_imc", "k_dtmin")}
+        for k, nb in (("k_shield", 21.0), ("k_chem", 26.0)):
+            if k in kern:
+                gbs = 8.0 * nb * cells / (kern[k]["avg_us"] * 1e-6) / 1e9
+                kern[k]["compulsory_GBs"] = round(gbs, 1)
+                kern[k]["frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        src = stg.get("source")
+        out[tag] = {"ms_per_step": 1e3 * el / steps, "zone_updates_per_s": n ** 3 * steps / el,
+                    "upload_with_20_shield_s": round(up, 3),
+                    "source_ms_per_step": (src[1] / nprof) if src else None, "kernels": kern,
+                    "dt_mean": float(np.mean(dts))}
+        log(f"[bench] chemistry {tag}: {out[tag]}")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # HBM traffic of the dominant kernel: child rocprofv3 --pmc runs
 # ---------------------------------------------------------------------------
@@ -347,6 +390,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--no-stiff", action="store_true")
+    ap.add_argument("--no-chem", action="store_true")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -421,6 +465,8 @@ def main():
            "copy_probe": {"bytes": 16 * (1 << 28), "ms": cms, "GBs": round(16 * (1 << 28) / (cms * 1e-3) / 1e9, 1)}}
     if not args.no_stiff:
         out["stiff"] = stiff_compare(n, 10, 2, w)
+    if not args.no_chem:
+        out["chemistry"] = chem_compare(n, 10, 2)
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle {args.cpu_steps} steps at {args.cpu_n}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
