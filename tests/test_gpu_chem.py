@@ -114,3 +114,57 @@ def test_columns_need_chemistry():
     g = mcf.Domain(p, nsplit=(1, 1, 1))
     with pytest.raises(mcf.MCFError, match="no column densities"):
         g.columns()
+
+
+def _chem_256(ifthrm, ifchem, ifrad):
+    p = ics.converging_flow_input(256)
+    base = Input3D()
+    for k in ("Hini", "pini", "H2ini", "Heini", "Hepini", "Cini", "COini", "Cpini", "pinit_over_kb"):
+        setattr(p, k, getattr(base, k))
+    p.ifthrm, p.ifchem, p.ifrad = ifthrm, ifchem, ifrad
+    return p, ics.converging_flow_state(p)
+
+
+def test_network_conserves_nuclei_256():
+    """At the full 256^3 size: SOURCE with only the network (no thermal
+    update, no SHIELD) conserves H (H + p + 2 H2), He (He + He+) and C
+    (C + C+ + CO) nuclei in every cell to round-off, recomputes the density
+    from the species and leaves velocities and pressure untouched."""
+    p, st = _chem_256(0, 2, 0)
+    print("network 256^3", flush=True)
+    g = mcf.Domain(p, nsplit=(4, 4, 4), cooling=mcf.COOL_CHEM)
+    g.upload(st)
+    before = interior(g.download(), p)
+    g.source(0.05)
+    after = interior(g.download(), p)
+    g.close()
+    sb, sa = before[8:], after[8:]
+    assert np.abs(sa - sb).max() > 0.0
+    for name, idx, w in (("H", [0, 1, 2], [1, 1, 2]), ("He", [3, 4], [1, 1]), ("C", [5, 6, 7], [1, 1, 1])):
+        nb = sum(wi * sb[i] for i, wi in zip(idx, w))
+        na = sum(wi * sa[i] for i, wi in zip(idx, w))
+        rel = float(np.abs(na - nb).max() / np.abs(nb).max())
+        print(name, "nuclei rel change", rel)
+        assert rel <= 1e-13, (name, rel)
+    rho = sa[1] + sa[0] + 2.0 * sa[2] + 4.0 * sa[3] + 4.0 * sa[4]
+    np.testing.assert_allclose(after[0], rho, rtol=1e-15, atol=0)
+    np.testing.assert_array_equal(after[1:5], before[1:5])
+
+
+def test_shield_columns_row_totals_256():
+    """At the full 256^3 size with 4 x blocks: after INITIA's 20 SHIELD
+    sweeps every column seen from the left plus the one seen from the right
+    is the row total, for all five columns."""
+    p, st = _chem_256(2, 2, 2)
+    print("shield 256^3", flush=True)
+    g = mcf.Domain(p, nsplit=(4, 4, 4), cooling=mcf.COOL_CHEM)
+    g.upload(st)
+    col = g.columns()
+    g.close()
+    n = 256
+    inner = col[:, :, 2:n + 2, 2:n + 2, 2:n + 2]
+    for q, name in enumerate(mcf.COLUMNS):
+        tot = inner[q, 0] + inner[q, 1]
+        dev = float(np.abs(tot - tot[:, :, :1]).max() / np.abs(inner[q]).max())
+        print(name, "row-total deviation", dev)
+        assert dev <= 1e-13, (name, dev)
