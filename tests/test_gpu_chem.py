@@ -168,3 +168,38 @@ def test_shield_columns_row_totals_256():
         dev = float(np.abs(tot - tot[:, :, :1]).max() / np.abs(inner[q]).max())
         print(name, "row-total deviation", dev)
         assert dev <= 1e-13, (name, dev)
+
+
+def test_chem_backward_euler_cells_vs_oracle():
+    """The chemistry Fcool in SOURCE's backward-Euler branch (IMC with the
+    chemistry cooling, k_src_imc<CHEM>): the 24^3 converging flow at 1e3
+    times the density and 3e3 times the pressure (T = 18000 K), where
+    Stblty's dt floor (2.5e-4) exceeds 0.2 t_cool in every cell; two steps
+    against the oracle at the multi-step bar (measured: 62530 backward-Euler
+    cells, worst field 1.8e-13).  The bisection's midpoints come from the
+    device's pow/log10, which may differ from glibc's in the last bit; the
+    branch decisions have not differed."""
+    p = ics.converging_flow_input(24)
+    base = Input3D()
+    for k in ("Hini", "pini", "H2ini", "Heini", "Hepini", "Cini", "COini", "Cpini", "pinit_over_kb"):
+        setattr(p, k, getattr(base, k))
+    p.ifchem, p.ifrad = 2, 2
+    st = ics.converging_flow_state(p)
+    st[0] *= 1e3
+    st[4] *= 3e3
+    st[8:16] *= 1e3
+    print("chem stiff", flush=True)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2), cooling="chem")
+    dts_o, _ = o.run(2)
+    g = mcf.Domain(p, nsplit=(2, 2, 2), cooling=mcf.COOL_CHEM)
+    g.upload(st)
+    g.cool_stats(reset=True)
+    dts_g = g.run(2)
+    nst, _ = g.cool_stats()
+    print("backward-Euler cells", nst)
+    assert nst > 10000
+    print("dts", dts_g, dts_o)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    print("chem stiff errors", errs)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-12, atol=0)
+    assert worst(errs) <= MULTI_TOL, errs
