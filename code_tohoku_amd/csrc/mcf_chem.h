@@ -7,7 +7,8 @@
 // q = 0 Ntot, 1 NH2, 2 NnC, 3 NCO, 4 tCII and dir 0 = seen from the left x
 // end, 1 = from the right one.  Every expression keeps the reference's
 // evaluation order (the translation units are built with
-// -ffp-contract=off); exp, log, pow and sqrt are the device library's.
+// -ffp-contract=off); exp, log, pow and sqrt are the device library's, and
+// powers T**a of the temperature are exp(a*log(T)) with log(T) formed once.
 #pragma once
 #include "mcf_device.h"
 
@@ -28,10 +29,10 @@ __device__ __forceinline__ double ch_fesc(double tau) {
          (0.5 + copysign(0.5, tau - 7.0)) / (4.0 * tau * sqrt(log(dmax(tau * 0.56419, 4.0))) + 1.0e-8);
 }
 
-// LEVC2 (M3:2890-2895)
-__device__ __forceinline__ void ch_levc2(double T, double b21, double ndH, double ndH2, double nde, double ndCp,
-                                         double &n1, double &n2) {
-  const double c21 = 8.854e-8 * nde / sqrt(T) + 9.399e-10 * pow(T, 0.07) * (ndH + ndH2);
+// LEVC2 (M3:2890-2895); t007 = T**0.07
+__device__ __forceinline__ void ch_levc2(double T, double t007, double b21, double ndH, double ndH2, double nde,
+                                         double ndCp, double &n1, double &n2) {
+  const double c21 = 8.854e-8 * nde / sqrt(T) + 9.399e-10 * t007 * (ndH + ndH2);
   const double c12 = 2.0 * c21 * exp(-0.092 / T);
   const double A21 = 2.4e-6 * b21;
   n2 = ndCp * c12 / (c12 + c21 + A21);
@@ -87,18 +88,22 @@ __device__ __forceinline__ ChemFc ch_fc_setup(const double *const *sp, const Che
   return f;
 }
 
-// Fcool (M3:2825-2881) at temperature T (units of 10^3 K)
+// Fcool (M3:2825-2881) at temperature T (units of 10^3 K).  Its nine powers
+// x**a are formed as exp(a*log(x)) from one log(T) and one log(pha), which
+// differs from pow(x, a) by about (|a log x| + 2) units of the last place
+// (|a log x| stays below 20 over 10 K - 10^5 K) and costs a third as much.
 __device__ __forceinline__ double ch_fcool(const ChemFc &f, double T) {
   const double pha = CH_G0 * sqrt(1.0e3 * T) * f.ATN1 / f.nde;
+  const double lT = log(T), lpha = log(pha);
   double Laml = f.laml_h * exp(-1.184e2 / T);
   Laml = Laml + f.laml_p * exp(-8.0e1 / T);
   double n1, n2;
-  ch_levc2(T, f.b21, f.ndH, f.ndH2, f.nde, f.ndCp, n1, n2);
+  ch_levc2(T, exp(0.07 * lT), f.b21, f.ndH, f.ndH2, f.nde, f.ndCp, n1, n2);
   const double Lamc = 6.0157e7 * n2 * f.b21;
-  double Lamo = f.lamo * pow(T, 0.4) * exp(-0.228 / T);
+  double Lamo = f.lamo * exp(0.4 * lT) * exp(-0.228 / T);
   Lamo = Lamo * f.feso;
-  const double Lamd = f.lamd * pow(T, 0.94) * pow(pha, 0.462628 / pow(T, 6.8e-2));
-  const double ncr = 3.3e6 * pow(T, 0.75) / f.nsum;
+  const double Lamd = f.lamd * exp(0.94 * lT) * exp(0.462628 / exp(6.8e-2 * lT) * lpha);
+  const double ncr = 3.3e6 * exp(0.75 * lT) / f.nsum;
   const double tau1 = f.tco1 / (T * CH_DV), tau2 = f.tco2 / (T * CH_DV);
   const double ct1 = tau1 * sqrt(6.283 * log(2.13 + (tau1 * 0.36788) * (tau1 * 0.36788)));
   const double ct2 = tau2 * sqrt(6.283 * log(2.13 + (tau2 * 0.36788) * (tau2 * 0.36788)));
@@ -106,9 +111,10 @@ __device__ __forceinline__ double ch_fcool(const ChemFc &f, double T) {
   const double fes = (2.0 + ym1 + 0.6 * (ym1 * ym1)) / (1.0 + ct1 + ncr + 1.5 * sqrt(ncr)) +
                      (2.0 + ym2 + 0.6 * (ym2 * ym2)) / (1.0 + ct2 + ncr + 1.5 * sqrt(ncr));
   const double LCOr = f.lcor * (T * T) * fes;
-  const double LCOH = f.lcoh * sqrt(T) * exp(-pow(2.0 / T, 3.43)) * exp(-3.08 / T);
-  const double LCOH2 = f.lcoh2 * T * exp(-pow(3.14e2 / T, 0.333)) * exp(-3.08 / T);
-  const double Gampe = f.gampe * (7.382e-3 * pow(T, 0.7) / (1.0 + 2.0e-4 * pha) + 4.9e-2 / (1.0 + 4.0e-3 * pow(pha, 0.73)));
+  const double e308 = exp(-3.08 / T);
+  const double LCOH = f.lcoh * sqrt(T) * exp(-exp(3.43 * (0.69314718055994531 - lT))) * e308;
+  const double LCOH2 = f.lcoh2 * T * exp(-exp(0.333 * (5.7493929859082531 - lT))) * e308;
+  const double Gampe = f.gampe * (7.382e-3 * exp(0.7 * lT) / (1.0 + 2.0e-4 * pha) + 4.9e-2 / (1.0 + 4.0e-3 * exp(0.73 * lpha)));
   return Laml + Lamc + Lamo + Lamd + LCOr + LCOH + LCOH2 - Gampe - f.gamcr - f.gampd;
 }
 
@@ -167,30 +173,31 @@ __device__ __forceinline__ void ch_network(double *const *sp, double *rho, const
              dmin(pow(3.75439e3 * NC2 + 1.0e-100, -0.75), pow(7.3469e1 * NC2 + 1.0e-100, -1.3));
   ATN5 = ATN5 * 0.5;
   const double zeta = 9.4671e-4;
+  const double lT = log(T); // T**a below is exp(a*log(T)), as in Fcool
   double kHrec = (0.5 + copysign(0.5, 15.78 - T)) * 0.45 * log(1.578e2 / T) +
                  (0.5 - copysign(0.5, 15.78 - T)) * 0.4 * sqrt(1.578e2 / T);
-  kHrec = 2.05572e1 * pow(T, -0.5) * kHrec;
-  const double kHerec = 6.37623e1 * pow(T, -0.672);
+  kHrec = 2.05572e1 * exp(-0.5 * lT) * kHrec;
+  const double kHerec = 6.37623e1 * exp(-0.672 * lT);
   const double kH2 = 3.4569e-3 * sqrt(T) / (1.0 + 1.26491 * sqrt(T + CH_TGR) + 2.0 * T + 8.0 * (T * T));
   const double SHLD1 = 0.965 / ((1.0 + x1 / CH_DV) * (1.0 + x1 / CH_DV)) + 0.035 / sqrt(1.0 + x1) * exp(-8.5e-4 * sqrt(1.0 + x1));
   const double SHLD2 = 0.965 / ((1.0 + x2 / CH_DV) * (1.0 + x2 / CH_DV)) + 0.035 / sqrt(1.0 + x2) * exp(-8.5e-4 * sqrt(1.0 + x2));
   const double kH2ph = 1.04138e3 * CH_G0 * ATN2 * (SHLD1 + SHLD2) * 0.5;
   const double kH2dH = 1.07294e5 * exp(-4.39e1 / T);
-  const double kH2de = 1.133e6 * pow(T, 0.35) * exp(-1.02e2 / T);
+  const double kH2de = 1.133e6 * exp(0.35 * lT) * exp(-1.02e2 / T);
   const double kCO = 1.57785e-2 / (1.0 + CH_G0 * ATN3 / (ndH2old * CH_XO));
   const double kCOph = 3.155e3 * CH_G0 * ATN3 * ATN4 * ATN5;
   const double SHLC1 = exp(-2.6 * Av1 - 3.0857e1 * NnC1 - 4.33743413e-4 * NH21) / (1.0 + 4.33743413e-4 * NH21);
   const double SHLC2 = exp(-2.6 * Av2 - 3.0857e1 * NnC2 - 4.33743413e-4 * NH22) / (1.0 + 4.33743413e-4 * NH22);
   const double kCi = 3.97 * zeta + 6.62697e3 * CH_G0 * (SHLC1 + SHLC2) * 0.5;
-  const double kCrec = 2.69267e1 * pow(T, -0.62) * (1.0 + 1.0e-4 * sqrt(nde) / (T * T));
-  const double kCOde = 1.4e4 * pow(T, -0.5) * exp(-1.14e2 / T);
-  const double kCOdH = 1.91858e3 * pow(T, -0.5) * exp(-7.77e1 / T);
+  const double kCrec = 2.69267e1 * exp(-0.62 * lT) * (1.0 + 1.0e-4 * sqrt(nde) / (T * T));
+  const double kCOde = 1.4e4 * exp(-0.5 * lT) * exp(-1.14e2 / T);
+  const double kCOdH = 1.91858e3 * exp(-0.5 * lT) * exp(-7.77e1 / T);
   const double kHie = 5.76e4 * sqrt(T) * exp(-1.58e2 / T);
-  const double kHeie = 2.76e4 * pow(T, 0.43) * exp(-2.85e2 / T);
-  const double kCie = 1.74e5 * pow(T, 0.4) * exp(-1.31e2 / T);
+  const double kHeie = 2.76e4 * exp(0.43 * lT) * exp(-2.85e2 / T);
+  const double kCie = 1.74e5 * exp(0.4 * lT) * exp(-1.31e2 / T);
   const double kHiH = 9.79 * sqrt(T) * exp(-1.58e2 / T);
-  const double kHeiH = 4.57 * pow(T, 0.43) * exp(-2.85e2 / T);
-  const double kCiH = 3.01e1 * pow(T, 0.4) * exp(-1.31e2 / T);
+  const double kHeiH = 4.57 * exp(0.43 * lT) * exp(-2.85e2 / T);
+  const double kCiH = 3.01e1 * exp(0.4 * lT) * exp(-1.31e2 / T);
   const double kH2dHep = 1.1676 * exp(-0.035 / T);
   const double kCOdHep = 5.0491e4;
   double temp1, temp2, temp3, omeps, eps;

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_shield(ShieldArgs S, Geo g) {
         const double b21 = ch_fesc(S.col[8][c]) + ch_fesc(S.col[9][c]);
         const double T = S.p[c] / (KB * (ndp + ndH + ndH2 + ndHe + ndHep));
         double n1, n2;
-        ch_levc2(T, b21, ndH, ndH2, ndp + ndHep + ndCp, ndCp, n1, n2);
+        ch_levc2(T, exp(0.07 * log(T)), b21, ndH, ndH2, ndp + ndHep + ndCp, ndCp, n1, n2);
         const double tmp = 1.15563e1 * (n1 - n2) * (1.0 / CH_DV);
         S.temp[c] = tmp;
         const double a0 = dxh * (ndp + ndH + 2.0 * ndH2 + ndHe + ndHep);
