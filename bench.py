@@ -16,7 +16,7 @@ cooling surrogate; the plain FP64 path is timed on the same state in the
 same run (``fp64``).  The timed steps carry no profiling events; ten more
 steps with every launch bracketed by HIP events give the per-kernel and
 per-stage tables.  ``roofline`` is the stage (dt, SOURCE call, x/y/z sweep,
-CT) that takes the most time per step: its compulsory HBM bytes (each
+CT) of the kernel that takes the most time per step: its compulsory HBM bytes (each
 persistent array it must read or write, once per interior cell: 30 doubles
 per cell for a sweep, 14 for CT, 8 for a SOURCE call, 13 for dt) over its
 measured time; two child ``rocprofv3 --pmc`` runs (FETCH_SIZE, then
@@ -161,10 +161,16 @@ def stage_rows(stg, rep, steps, traffic=None):
     return rows
 
 
-def roofline(rows, traffic):
-    """The roofline of the stage that takes the most time per step."""
-    name, r = max(rows.items(), key=lambda kv: kv[1]["ms_per_step"])
-    out = {"bound": "hbm", "stage": name, "kernels_avg_us": r["kernels_avg_us"], "achieved": r["achieved_GBs"],
+def roofline(rows, traffic, rep):
+    """The roofline of the dominant kernel's stage: the stage of the kernel
+    that takes the most time per step (a sweep's stage is that one kernel).
+    The stage with the most time per step is named beside it."""
+    kname = max((k for k in rep if kernel_stage(k) in rows), key=lambda k: rep[k][1])
+    name = kernel_stage(kname)
+    r = rows[name]
+    slowest = max(rows.items(), key=lambda kv: kv[1]["ms_per_step"])[0]
+    out = {"bound": "hbm", "stage": name, "dominant_kernel": kname, "slowest_stage": slowest,
+           "kernels_avg_us": r["kernels_avg_us"], "achieved": r["achieved_GBs"],
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac"],
            "traffic": r.get("traffic_bytes_per_run"),
            "alg_bytes_per_run": r["compulsory_bytes_per_run"], "ms_per_run": r["ms_per_run"],
@@ -447,11 +453,11 @@ def main():
            "config": {"workload": f"converging_flow_{n}^3_ki_cooling", "grid": [n, n, n],
                       "block_split": list(nsplit), "precision": "mixed", "cooling": "surrogate",
                       "surrogate_weights": wsrc},
-           "roofline": roofline(rows, pmc),
+           "roofline": roofline(rows, pmc, rep),
            "stages": rows,
            "kernels": kernel_table(rep, pmc, nprof),
            "speedup_vs_fp64": value / value64,
-           "fp64": {"value": value64, "ms_per_step": 1e3 * el64 / args.steps, "roofline": roofline(rows64, None),
+           "fp64": {"value": value64, "ms_per_step": 1e3 * el64 / args.steps, "roofline": roofline(rows64, None, rep64),
                     "stages": rows64, "kernels": kernel_table(rep64, None, nprof64),
                     "dt_mean": float(sum(dts64) / len(dts64))},
            "surrogate": {"backward_euler_cells": nstiff, "imc_fallbacks": nfall,
