@@ -93,6 +93,7 @@ struct mcf_ctx {
   bool chem = false;    // MCF_COOL_CHEM: the chemistry-network Fcool
   bool has_cols = false;
   int nsp[3]{1, 1, 1};
+  int ncu = 256; // compute units of the device
   double time = 0.0;
   int ifevo = 1;
   double *hred = nullptr; // pinned 2 doubles
@@ -280,6 +281,14 @@ static long boxsize(const Box &b) { return (long)b.cnt[0] * b.cnt[1] * b.cnt[2];
     if (nt_ > 0)                                                                                   \
       KLAUNCH_N(#kern, (kern<R>), dim3(nblocks(nt_)), dim3(TPB), 8.0 * (double)nt_ * (double)(narr), __VA_ARGS__); \
   } while (0)
+
+// y/z sweeps as two waves per row of columns (k_sweep_col2) on the mixed
+// path: 1.39 / 1.45 ms per y / z sweep against 1.62 / 1.70 for
+// k_sweep_col at 256^3 (250 registers, no spill, two waves per SIMD).  The
+// FP64 instance spills 224 bytes per lane to scratch at two waves per SIMD
+// (4.47 against 2.31 ms at 256^3), so FP64 takes it only for launches of
+// fewer blocks than the device has CUs, where the second wave per column
+// row is otherwise idle hardware (2.2x faster on a 72 x 44 x 136 grid).
 
 // ----------------------------------------------------------------------------
 // ghost fills
@@ -563,15 +572,19 @@ static int do_sweep(mcf_ctx *x, int d, double dt) {
     // reads only interior cells (periodic images), so no fills precede it.
     // One kernel instead of k_rhsmoc_col + k_remap_col (the MOC results no
     // longer go through HBM): 1.66 against 1.93 ms per sweep (mixed), 2.32
-    // against 2.75 (FP64), at 1 wave per SIMD (354 / 450 registers)
+    // against 2.75 (FP64), at 1 wave per SIMD (354 / 450 registers); on the
+    // mixed path split over two waves per column row (k_sweep_col2)
     const int n = x->g.n[d], ntr = x->g.n[d == 1 ? 2 : 1];
     const int seg = x->col_seg;
     dim3 gr((unsigned)((n + seg - 1) / seg), (unsigned)((x->g.n[0] + 63) / 64), (unsigned)((ntr + 3) / 4));
     // reads rho, v(3), p, Bcc(3), face B_n, species(8); writes rho, v(3), p, species(8)
     const double by = 8.0 * (double)x->g.n[0] * x->g.n[1] * x->g.n[2] * 30;
     if (x->mixed) {
-      if (d == 1) KLAUNCH_N("k_sweep_col<1>", (k_sweep_col<1, float>), gr, dim3(256), by, A, x->g, dt, seg);
-      else KLAUNCH_N("k_sweep_col<2>", (k_sweep_col<2, float>), gr, dim3(256), by, A, x->g, dt, seg);
+      if (d == 1) KLAUNCH_N("k_sweep_col2<1>", (k_sweep_col2<1, float>), gr, dim3(512), by, A, x->g, dt, seg);
+      else KLAUNCH_N("k_sweep_col2<2>", (k_sweep_col2<2, float>), gr, dim3(512), by, A, x->g, dt, seg);
+    } else if ((long)gr.x * gr.y * gr.z < (long)x->ncu) {
+      if (d == 1) KLAUNCH_N("k_sweep_col2<1>", (k_sweep_col2<1, double>), gr, dim3(512), by, A, x->g, dt, seg);
+      else KLAUNCH_N("k_sweep_col2<2>", (k_sweep_col2<2, double>), gr, dim3(512), by, A, x->g, dt, seg);
     } else {
       if (d == 1) KLAUNCH_N("k_sweep_col<1>", (k_sweep_col<1, double>), gr, dim3(256), by, A, x->g, dt, seg);
       else KLAUNCH_N("k_sweep_col<2>", (k_sweep_col<2, double>), gr, dim3(256), by, A, x->g, dt, seg);
@@ -903,6 +916,7 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   x->ncell = (size_t)vol;
   setup_constants(x);
   hipError_t e = hipGetDevice(&x->dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&x->ncu, hipDeviceAttributeMultiprocessorCount, x->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { g_err = std::string("HIP init: ") + hipGetErrorString(e); delete x; return MCF_E_HIP; }
   {

@@ -110,6 +110,7 @@ template <int D, class R> __global__ void k_rhs(SweepIO A, Geo g, double dt);
 template <int D, class R> __global__ void k_rhsmoc_col(SweepIO A, Geo g, double dt); // D = 1, 2
 template <int D, class R> __global__ void k_remap_col(SweepIO A, Geo g, double dt); // D = 1, 2
 template <int D, class R> __global__ void k_sweep_col(SweepIO A, Geo g, double dt, int seg); // D = 1, 2: whole sweep
+template <int D, class R> __global__ void k_sweep_col2(SweepIO A, Geo g, double dt, int seg); // two waves per column row
 constexpr int SWEEP_COL_MAXN = 2048; // longest axis k_sweep_col takes (cell edges in LDS)
 template <class R> __global__ void k_rhsmoc_x(SweepIO A, Geo g, XSide S, double dt);
 constexpr int XHALO = 4, XCELLS = 64 - 2 * XHALO; // k_rhsmoc_x: halo and cells per wave

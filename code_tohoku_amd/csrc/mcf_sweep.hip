@@ -1379,6 +1379,331 @@ __global__ __launch_bounds__(256) void k_sweep_col(SweepIO A, Geo g, double dt, 
   }
 }
 
+// A whole y or z sweep as k_sweep_col, over two waves per row of 64
+// columns: the producer wave runs the RHS and MOC chain and puts, per cell J,
+// its MOC-updated state (rho, m_n, m_t1, m_t2, E, Blg1, Blg2), dx_lag and V*
+// into an LDS slot; the consumer wave takes it one iteration later, forms
+// the Lagrangian species and remaps.  Each wave holds only its own windows,
+// so a SIMD runs a producer and a consumer wave side by side.  The arithmetic
+// is k_sweep_col's, statement for statement.  Lanes past the grid's end in x
+// or in the transverse axis march the last column and write nothing (every
+// wave reaches each barrier).
+template <int D, class R>
+__global__ __launch_bounds__(512) void k_sweep_col2(SweepIO A, Geo g, double dt, int seg) {
+  constexpr int VN = D + 1, VT1 = (D + 1) % 3 + 1, VT2 = (D + 2) % 3 + 1;
+  constexpr int BT1 = (D + 1) % 3, BT2 = (D + 2) % 3;
+  const int nx = g.n[0];
+  const int n = g.n[D], ntr = g.n[D == 1 ? 2 : 1];
+  int bseg, bxc, btr;
+  xcd_tile(bseg, bxc, btr);
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6, r4 = wv & 3;
+  const bool producer = wv < 4;
+  const int i_ = 1 + 64 * bxc + lane, tr_ = 1 + 4 * btr + r4;
+  const bool act = i_ <= nx && tr_ <= ntr;
+  const int i = min(i_, nx), tr = min(tr_, ntr);
+  const int p0 = 1 + seg * bseg;
+  const int p1 = min(n, p0 + seg - 1);
+  const int m0 = p0 - 2; // first MOC result (m0 .. p1+2)
+  const int nblk = A.nblk;
+  __shared__ double sxe[SWEEP_COL_MAXN + 4];
+  // producer -> consumer: [slot][quantity][row][lane], quantities 0..6 the
+  // MOC-updated state in REMAP's order, 7 dx_lag, 8 V*
+  __shared__ double hand[2][9][4][64];
+  for (int t = (int)threadIdx.x; t < n + 4; t += (int)blockDim.x) sxe[t] = A.xe[t];
+  __syncthreads();
+  // image position of p (n >= 8, so one shift suffices for p0-6 .. p1+7)
+  auto img = [&](int p) { return p < 1 ? p + n : (p > n ? p - n : p); };
+  auto at = [&](int p) { return D == 1 ? ix3s(g, i, p, tr) : ix3s(g, i, tr, p); };
+  auto cell = [&](int p) { return at(img(p)); };
+  const double *const *sp = A.sp;
+  const double *xe = sxe;
+  const double dxu = A.dxe[1];
+  const int ps = m0 - 2, pe = p1 + 5;
+  if (producer) {
+    const double *U1 = A.U[0], *UVN = A.U[VN], *U5 = A.U[4];
+    const double *B1 = A.bcc[BT1], *B2 = A.bcc[BT2];
+    const double *UBn = A.U[5 + D];
+    // ---- RHS window: positions q-1, q, q+1 and the values of q+2 in flight
+    double wr[4], wv[4], wp[4], wa[4], wc[4];
+    double spw[3][5];
+    R qprev[8];
+    R pa_prev = 0, va_prev = 0;
+    {
+      const int c0 = cell(ps - 2), c1 = cell(ps - 1), c2 = cell(ps), c3 = cell(ps + 1);
+      wr[0] = U1[c0]; wr[1] = U1[c1]; wr[2] = U1[c2]; wr[3] = U1[c3];
+      wv[0] = UVN[c0]; wv[1] = UVN[c1]; wv[2] = UVN[c2]; wv[3] = UVN[c3];
+      wp[0] = U5[c0]; wp[1] = U5[c1]; wp[2] = U5[c2]; wp[3] = U5[c3];
+      wa[0] = B1[c0]; wa[1] = B1[c1]; wa[2] = B1[c2]; wa[3] = B1[c3];
+      wc[0] = B2[c0]; wc[1] = B2[c1]; wc[2] = B2[c2]; wc[3] = B2[c3];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) { spw[0][f] = sp[f][c1]; spw[1][f] = sp[f][c2]; }
+      rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
+                    wc[2], spw[0], dxu, dxu, dxu, dt, A.facdep, qprev);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { wr[k] = wr[k + 1]; wv[k] = wv[k + 1]; wp[k] = wp[k + 1]; wa[k] = wa[k + 1]; wc[k] = wc[k + 1]; }
+    }
+    double vt1 = 0, vt2 = 0, bnc = 0, fb = 0;
+    // ---- MOC windows (as k_rhsmoc_col)
+    OR<R> o0{}, o1{}, o2{};
+    double f0 = 0, f1 = 0, f2 = 0;
+    double va0 = 0, va1 = 0, va2 = 0;
+    R G0[5], G1[5], H0[5], H1[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { G0[v] = G1[v] = H0[v] = H1[v] = (R)0; }
+    MocV<R> rvp{};            // right view of interface J-1
+    double bnp = 0, vap = 0;  // B_n of interface J-1 in the right view, V* of J-1
+    for (int q = ps; q <= pe + 1; ++q) {
+      __syncthreads();
+      if (q > pe) continue;
+      const int cf = cell(q + 2), cn = cell(q + 1), cq = cell(q);
+      wr[3] = U1[cf]; wv[3] = UVN[cf]; wp[3] = U5[cf]; wa[3] = B1[cf]; wc[3] = B2[cf];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) spw[2][f] = sp[f][cn];
+      const double nvt1 = A.U[VT1][cq], nvt2 = A.U[VT2][cq], nbnc = A.bcc[D][cq], nfb = UBn[cq];
+      R qcur[8];
+      rhs_states<R>(wr[1], wv[1], wp[1], wa[1], wc[1], wr[0], wr[2], wv[0], wv[2], wp[0], wp[2], wa[0], wa[2], wc[0],
+                    wc[2], spw[1], dxu, dxu, dxu, dt, A.facdep, qcur);
+      R Pa, Va;
+      rhs_iface<R>(spw[0], spw[1], qprev, qcur, Pa, Va);
+      const int k = q - 1;
+      if (k >= ps) {
+        // Lagrangian update of cell k (M3:1448-1466) at its image position
+        const int kk = img(k);
+        const double xq = xe[kk + 1], xqm = xe[kk];
+        const double rho = wr[0], vn = wv[0], p = wp[0], b1 = wa[0], b2 = wc[0];
+        double vv[3], bb[3];
+        vv[D] = vn; vv[(D + 1) % 3] = vt1; vv[(D + 2) % 3] = vt2;
+        bb[D] = bnc; bb[BT1] = b1; bb[BT2] = b2;
+        const double vx = vv[0], vy = vv[1], vz = vv[2];
+        const double bx = bb[0], by = bb[1], bz = bb[2];
+        const double dm = dxu * rho;
+        const double pa = (double)Pa, pam = (double)pa_prev, va = (double)Va, vam = (double)va_prev;
+        const double xl = xq + va * dt, xlm = xqm + vam * dt;
+        const double dxl = xl - xlm;
+        const double *spl = spw[0];
+        double rhoL, vnL, eL, bl1, bl2;
+        if constexpr (Ar<R>::strict) {
+          double s4 = spl[0] + spl[1] + spl[3] + spl[4];
+          double gg = 3.0 * s4 + 5.0 * spl[2];
+          double gi = (2.0 * s4 + 2.0 * spl[2]) / gg;
+          double e = p / (rho * gi) + 0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) / rho;
+          rhoL = dm / dxl;
+          vnL = vn - dt / dm * (pa - pam);
+          eL = e - dt / dm * (pa * va - pam * vam);
+          bl1 = b1 * dxu / dxl;
+          bl2 = b2 * dxu / dxl;
+        } else {
+          double s4 = spl[0] + spl[1] + spl[3] + spl[4];
+          double irho = 1.0 / rho;
+          double e = p * (3.0 * s4 + 5.0 * spl[2]) / (2.0 * s4 + 2.0 * spl[2]) * irho +
+                     0.5 * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz) * irho;
+          double idl = 1.0 / dxl, dtdm = dt / dm, cmp = dxu * idl;
+          rhoL = dm * idl;
+          vnL = vn - dtdm * (pa - pam);
+          eL = e - dtdm * (pa * va - pam * vam);
+          bl1 = b1 * cmp;
+          bl2 = b2 * cmp;
+        }
+        o2.v[0] = rhoL; o2.v[1] = vt1; o2.v[2] = vt2; o2.v[3] = bl1; o2.v[4] = bl2; o2.v[5] = dxl;
+        o2.vn = vnL; o2.e = eL;
+        o2.mme = use_minmod_r<R>(eL, rhoL);
+        o2.mmp = use_minmod_r<R>(p, rhoL);
+        f2 = fb;
+        va2 = va;
+        if (k >= m0) {
+          // MOC slope of cell k-1 (M3:2011-2020) in its own block's view; next
+          // to a block edge also with the ghost switch (the neighbour's view)
+          const int km = k - 1;
+          const bool mm = o1.mme;
+          const R dxm = (R)o0.v[5], dx0 = (R)o1.v[5], dxp = (R)o2.v[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) G1[v] = vslope<R>(o0.v[v], o1.v[v], o2.v[v], dxm, dx0, dxp, mm);
+          if ((km % nblk == 0 || (km - 1) % nblk == 0) && o1.mmp != mm) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) H1[v] = vslope<R>(o0.v[v], o1.v[v], o2.v[v], dxm, dx0, dxp, o1.mmp);
+          }
+          if (k >= m0 + 1) {
+            // interface J = k-2 between cells J and J+1; the owners of J and
+            // J+1 see it at positions img(J) and img(J+1)-1
+            const int J = k - 2;
+            const int gL = img(J), gR = img(J + 1) - 1;
+            const double bnL = moc_bn(f0, f1, f2, xe[gL + 1] + va0 * dt, xe, gL);
+            const double bnR = gR == gL ? bnL : moc_bn(f0, f1, f2, xe[gR + 1] + va0 * dt, xe, gR);
+            MocV<R> L, Rv;
+            if (J % nblk != 0) {
+              L = moc_iface_r<R>(o0.v, o1.v, G0, G1, bnL, A.facdep, dt);
+              Rv = L;
+            } else {
+              const bool g0 = o0.mmp != o0.mme, g1 = o1.mmp != o1.mme;
+              L = moc_iface_r<R>(o0.v, o1.v, G0, g1 ? H1 : G1, bnL, A.facdep, dt);
+              if (g0 || g1 || gR != gL) Rv = moc_iface_r<R>(o0.v, o1.v, g0 ? H0 : G0, G1, bnR, A.facdep, dt);
+              else Rv = L;
+            }
+            if (J >= m0) {
+              // MOC update of cell J (M3:2053-2095), to the consumer wave
+              const double Bnap = bnL, Bnam = bnp, Vnap = va0, Vnam = vap;
+              const double dxl0 = o0.v[5], rho0 = o0.v[0];
+              const double BayL = (double)L.Bay, VayL = (double)L.Vay, BazL = (double)L.Baz, VazL = (double)L.Vaz;
+              const double BayR = (double)rvp.Bay, VayR = (double)rvp.Vay, BazR = (double)rvp.Baz, VazR = (double)rvp.Vaz;
+              double Q3[7];
+              if constexpr (Ar<R>::strict) {
+                Q3[VN] = o0.vn * rho0 + dt / dxl0 * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+                Q3[VT1] = o0.v[1] * rho0 + dt / dxl0 * (Bnap * BayL - Bnam * BayR);
+                Q3[VT2] = o0.v[2] * rho0 + dt / dxl0 * (Bnap * BazL - Bnam * BazR);
+                Q3[4] = o0.e * rho0 + dt / dxl0 *
+                                          ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                           (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+                Q3[5] = o0.v[3] + dt / dxl0 * (Bnap * VayL - Bnam * VayR);
+                Q3[6] = o0.v[4] + dt / dxl0 * (Bnap * VazL - Bnam * VazR);
+              } else {
+                const double fq = dt / dxl0;
+                Q3[VN] = o0.vn * rho0 + fq * 0.5 * (Bnap * Bnap - Bnam * Bnam);
+                Q3[VT1] = o0.v[1] * rho0 + fq * (Bnap * BayL - Bnam * BayR);
+                Q3[VT2] = o0.v[2] * rho0 + fq * (Bnap * BazL - Bnam * BazR);
+                Q3[4] = o0.e * rho0 + fq * ((0.5 * Bnap * Vnap + BayL * VayL + BazL * VazL) * Bnap -
+                                            (0.5 * Bnam * Vnam + BayR * VayR + BazR * VazR) * Bnam);
+                Q3[5] = o0.v[3] + fq * (Bnap * VayL - Bnam * VayR);
+                Q3[6] = o0.v[4] + fq * (Bnap * VazL - Bnam * VazR);
+              }
+              Q3[0] = rho0;
+#pragma unroll
+              for (int v = 0; v < 7; ++v) hand[q & 1][v][r4][lane] = Q3[v];
+              hand[q & 1][7][r4][lane] = dxl0;
+              hand[q & 1][8][r4][lane] = Vnap;
+            }
+            rvp = Rv;
+            bnp = bnR;
+            vap = va0;
+          }
+#pragma unroll
+          for (int v = 0; v < 5; ++v) { G0[v] = G1[v]; H0[v] = H1[v]; }
+        }
+        o0 = o1; o1 = o2;
+        f0 = f1; f1 = f2;
+        va0 = va1; va1 = va2;
+      }
+      // advance the RHS window by one position
+#pragma unroll
+      for (int j = 0; j < 3; ++j) { wr[j] = wr[j + 1]; wv[j] = wv[j + 1]; wp[j] = wp[j + 1]; wa[j] = wa[j + 1]; wc[j] = wc[j + 1]; }
+#pragma unroll
+      for (int f = 0; f < 5; ++f) { spw[0][f] = spw[1][f]; spw[1][f] = spw[2][f]; }
+#pragma unroll
+      for (int f = 0; f < 8; ++f) qprev[f] = qcur[f];
+      pa_prev = Pa;
+      va_prev = Va;
+      vt1 = nvt1; vt2 = nvt2; bnc = nbnc; fb = nfb;
+    }
+  } else {
+    // ---- REMAP window: remapped quantities of cells Jf-1 .. Jf+2 (15 each),
+    // their dx_lag, V* of face Jf, the flux of face Jf-1
+    double Q0[15], Q1[15], Q2[15], Q3[15], W0 = 0, W1 = 0, W2 = 0, W3 = 0, V1 = 0, V2 = 0, V3 = 0;
+    double Fp[15];
+#pragma unroll
+    for (int v = 0; v < 15; ++v) { Q0[v] = Q1[v] = Q2[v] = Q3[v] = 0.0; Fp[v] = 0.0; }
+    // pre-sweep species of the cell whose MOC result enters REMAP's window in
+    // this iteration (J = q-3) and B_n of the cell REMAP updates (q-5), each
+    // loaded one iteration ahead
+    double csp[8], cbn;
+    {
+      const int cs = cell(ps - 3), cb = cell(ps - 5);
+#pragma unroll
+      for (int f = 0; f < 8; ++f) csp[f] = sp[f][cs];
+      cbn = A.bcc[D][cb];
+    }
+    for (int t = ps; t <= pe + 1; ++t) {
+      __syncthreads();
+      // the producer's iteration whose cell J = q-3 the slot holds
+      const int q = t - 1;
+      if (q < ps) continue;
+      double nsp[8];
+      {
+        const int cs = cell(q - 2);
+#pragma unroll
+        for (int f = 0; f < 8; ++f) nsp[f] = sp[f][cs];
+      }
+      const double nbn = A.bcc[D][cell(q - 4)];
+      const int J = q - 3;
+      if (J >= m0) {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) Q3[v] = hand[q & 1][v][r4][lane];
+        const double dxl0 = hand[q & 1][7][r4][lane];
+        W3 = dxl0;
+        V3 = hand[q & 1][8][r4][lane];
+        // Lagrangian species of cell J (M3:1466) from the pre-sweep ones
+        if constexpr (Ar<R>::strict) {
+#pragma unroll
+          for (int f = 0; f < 8; ++f) Q3[7 + f] = dxu * csp[f] / dxl0;
+        } else {
+          const double cmp = dxu * (1.0 / dxl0);
+#pragma unroll
+          for (int f = 0; f < 8; ++f) Q3[7 + f] = csp[f] * cmp;
+        }
+        // REMAP of face Jf = J-2 (cells Jf-1 .. Jf+2 in Q0 .. Q3) and of
+        // cell Jf (faces Jf-1 and Jf)
+        const int Jf = J - 2;
+        if (Jf >= p0 - 1) {
+          const double vadt = V1 * dt;
+          const bool up = !(vadt <= 0.0);
+          const double u4 = up ? Q1[4] : Q2[4], u0 = up ? Q1[0] : Q2[0];
+          const double depend = up ? 0.5 * (W1 - vadt) * A.facdep : 0.5 * (W2 + vadt) * A.facdep;
+          const R dxm2 = (R)(up ? W0 : W1), dx02 = (R)(up ? W1 : W2), dxp2 = (R)(up ? W2 : W3);
+          const bool mmr = use_minmod_r<R>(u4, u0);
+          const R rdep = (R)depend;
+          double F[15];
+#pragma unroll
+          for (int v = 0; v < 15; ++v) {
+            const R gr = vslope<R>(up ? Q0[v] : Q1[v], up ? Q1[v] : Q2[v], up ? Q2[v] : Q3[v], dxm2, dx02, dxp2, mmr);
+            const double qu = up ? Q1[v] : Q2[v];
+            double face;
+            if constexpr (Ar<R>::strict) face = up ? qu + depend * gr : qu - depend * gr;
+            else face = up ? qu + (double)(rdep * gr) : qu - (double)(rdep * gr);
+            F[v] = -(vadt * face);
+          }
+          if (Jf >= p0 && act) {
+            // remapped conserved state (M3:1645-1686), primitives (M3:1291-1321)
+            const double dxE = xe[Jf + 1] - xe[Jf];
+            double Qn[15];
+#pragma unroll
+            for (int v = 0; v < 15; ++v) {
+              if constexpr (Ar<R>::strict) Qn[v] = (Q1[v] * W1 + F[v] - Fp[v]) / dxE;
+              else Qn[v] = (Q1[v] * W1 + F[v] - Fp[v]) * (1.0 / dxE);
+            }
+            const int c = at(Jf);
+            const double bn = cbn;
+            double rho = dmax(1.27 * A.rmin, Qn[0]);
+            double invd = 1.0 / rho;
+            double vx = Qn[1] * invd, vy = Qn[2] * invd, vz = Qn[3] * invd;
+            double s4 = Qn[7] + Qn[8] + Qn[10] + Qn[11];
+            double gg = 3.0 * s4 + 5.0 * Qn[9];
+            double gi = (2.0 * s4 + 2.0 * Qn[9]) / gg;
+            double ekin = 0.5 * (vx * vx + vy * vy + vz * vz) * rho;
+            double emag = 0.5 * (bn * bn + Qn[5] * Qn[5] + Qn[6] * Qn[6]);
+            double pr = (Qn[4] - ekin - emag) * gi;
+            pr = dmax(A.pmin, pr);
+            pr = dmin(A.pmax, pr);
+            A.Uo[0][c] = rho;
+            A.Uo[1][c] = vx;
+            A.Uo[2][c] = vy;
+            A.Uo[3][c] = vz;
+            A.Uo[4][c] = pr;
+#pragma unroll
+            for (int f = 0; f < 8; ++f) A.spo[f][c] = dmax(A.spmin[f], Qn[7 + f]);
+          }
+#pragma unroll
+          for (int v = 0; v < 15; ++v) Fp[v] = F[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 15; ++v) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = Q3[v]; }
+        W0 = W1; W1 = W2; W2 = W3;
+        V1 = V2; V2 = V3;
+      }
+#pragma unroll
+      for (int f = 0; f < 8; ++f) csp[f] = nsp[f];
+      cbn = nbn;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // RHS + MOC fused along x
 // ---------------------------------------------------------------------------
@@ -1865,8 +2190,10 @@ template __global__ void k_rhsmoc_col<1, float>(SweepIO, Geo, double);
 template __global__ void k_rhsmoc_col<2, float>(SweepIO, Geo, double);
 template __global__ void k_sweep_col<1, double>(SweepIO, Geo, double, int);
 template __global__ void k_sweep_col<2, double>(SweepIO, Geo, double, int);
-template __global__ void k_sweep_col<1, float>(SweepIO, Geo, double, int);
-template __global__ void k_sweep_col<2, float>(SweepIO, Geo, double, int);
+template __global__ void k_sweep_col2<1, double>(SweepIO, Geo, double, int);
+template __global__ void k_sweep_col2<2, double>(SweepIO, Geo, double, int);
+template __global__ void k_sweep_col2<1, float>(SweepIO, Geo, double, int);
+template __global__ void k_sweep_col2<2, float>(SweepIO, Geo, double, int);
 template __global__ void k_remap_col<1, double>(SweepIO, Geo, double);
 template __global__ void k_remap_col<2, double>(SweepIO, Geo, double);
 template __global__ void k_remap_col<1, float>(SweepIO, Geo, double);
