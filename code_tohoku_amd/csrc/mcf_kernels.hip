@@ -521,6 +521,15 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
   const int ic = min(i, hi0 + 1), jc = min(j, hi1 + 1);
   const int w = (int)(threadIdx.x >> 6);
   const int i0 = b.lo[0] + 64 * bx, j0 = b.lo[1] + SRC_ROWS * by;
+  // backward-Euler cells: counted in LDS and staged in the block's own range
+  // of A.fqueue (free while this kernel runs), at qoff: the tiles numbered
+  // z segment, then row group, then x tile, each range as long as the
+  // block's own cells
+  __shared__ unsigned s_qn, s_qbase;
+  if (threadIdx.x == 0) s_qn = 0u;
+  const long qoff = (long)(k0 - b.lo[2]) * b.cnt[0] * b.cnt[1] +
+                    (long)(k1 - k0 + 1) * ((long)(j0 - b.lo[1]) * b.cnt[0] +
+                                           (long)min(SRC_ROWS, b.cnt[1] - SRC_ROWS * by) * (i0 - b.lo[0]));
   // temperature tiles of planes k-1 .. k+2 (ring of 4): rows j0-1 ..
   // j0+SRC_ROWS by columns i0-1 .. i0+64, formed by all threads together
   // (coalesced rows, the frame included); gamma-1 and n of the block's own
@@ -614,11 +623,11 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
     if (m) {
       const int leader = __ffsll((long long)m) - 1;
       unsigned base = 0;
-      if (lane == leader) base = atomicAdd(A.qn, (unsigned)__popcll(m));
+      if (lane == leader) base = atomicAdd(&s_qn, (unsigned)__popcll(m));
       base = __shfl(base, leader, 64);
       if (stiff) {
         const unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-        A.queue[pos] = c;
+        A.fqueue[qoff + (long)pos] = c;
         A.qdc[c] = dc;
       }
     }
@@ -629,6 +638,12 @@ __global__ __launch_bounds__(64 * SRC_ROWS) void k_src_col(SrcArgs A, Geo g, Box
       A.Pout[c] = u;
     }
   }
+  // the block's backward-Euler cells into the stage's queue: one range per
+  // block, taken with one atomic
+  __syncthreads();
+  if (threadIdx.x == 0) s_qbase = s_qn ? atomicAdd(A.qn, s_qn) : 0u;
+  __syncthreads();
+  for (unsigned t = threadIdx.x; t < s_qn; t += blockDim.x) A.queue[s_qbase + t] = A.fqueue[qoff + (long)t];
 }
 template __global__ void k_src_col<double, false>(SrcArgs, Geo, Box, double, int);
 template __global__ void k_src_col<float, false>(SrcArgs, Geo, Box, double, int);
