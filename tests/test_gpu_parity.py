@@ -211,19 +211,49 @@ def test_multistep_vs_oracle_64():
     assert worst(errs) <= MULTI_TOL, errs
 
 
-def test_multistep_vs_oracle_long_columns():
-    """y longer than one column segment of the fused y/z sweep kernels (128
-    cells) and not a multiple of it; z shorter than one segment."""
-    p = small(24, 136, 40)
+@pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
+def test_multistep_vs_oracle_long_columns(precision):
+    """y longer than one column segment of the y/z sweep kernels (256 cells)
+    and not a multiple of it, so the march runs in two segments whose halo
+    cells each evaluate in their own block's view; z shorter than one
+    segment: the oracle's result (FP64) and agreement to FP32 round-off
+    (mixed)."""
+    p = small(24, 264, 40)
     st = ics.random_state(p, seed=6)
     o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
-    g = mcf.Domain(p, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2), precision=precision)
     g.upload(st)
     dts_o, _ = o.run(2)
-    dts_g = g.run(2)
+    if precision == mcf.PREC_FP64:
+        dts_g = g.run(2)
+        np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
+        errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+        assert worst(errs) <= MULTI_TOL, errs
+    else:
+        for dt in dts_o:
+            g.step(dt)
+        got = interior(g.download(), p)
+        ref = interior(o.state(), p)
+        for f in range(8):
+            assert np.abs(got[f] - ref[f]).max() <= 1e-5 * np.abs(ref[f]).max(), f
+
+
+def test_one_wave_columns_vs_oracle():
+    """FP64 on a grid whose y/z launches reach the device's CU count (here
+    256 x 264 x 128: 2 x 4 x 32 = 256 blocks for the y sweep), so the y and z
+    sweeps run as the one-wave column march, the y axis in two segments:
+    one step against the oracle."""
+    p = small(256, 264, 128, periodic=True)
+    st = ics.random_state(p, seed=9)
+    o = oracle.Oracle(p, st, nsplit=(4, 2, 2))
+    g = mcf.Domain(p, nsplit=(4, 2, 2))
+    g.upload(st)
+    print("one-wave columns 256x264x128", flush=True)
+    dts_o, _ = o.run(1)
+    dts_g = g.run(1)
     np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
     errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
-    assert worst(errs) <= MULTI_TOL, errs
+    assert worst(errs) <= STEP_TOL, errs
 
 
 def test_axis_longer_than_sweep_col_uses_split_column_kernels():
@@ -244,10 +274,10 @@ def test_axis_longer_than_sweep_col_uses_split_column_kernels():
 
 @pytest.mark.parametrize("precision", [mcf.PREC_FP64, mcf.PREC_MIXED])
 def test_multistep_vs_oracle_long_z(precision):
-    """z longer than one column segment (128 cells) and than SOURCE's
+    """z longer than one column segment (256 cells) and than SOURCE's
     z segments (64 planes mixed, 32 FP64), not a multiple of either: the
     oracle's result (FP64) and agreement to FP32 round-off (mixed)."""
-    p = small(24, 16, 136)
+    p = small(24, 16, 264)
     st = ics.random_state(p, seed=14)
     o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
     g = mcf.Domain(p, nsplit=(2, 2, 2), precision=precision)
