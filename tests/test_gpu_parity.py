@@ -198,6 +198,39 @@ def test_source_ct_ghosts_vs_oracle(periodic):
         g.upload(ref)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_source_many_backward_euler_cells_vs_oracle(periodic):
+    """SOURCE on a ragged grid (72 x 44 x 136: partial x tiles and row
+    groups, three z segments) at 1e4 times the density and pressure, where
+    most cells take the backward-Euler branch, so every block of the
+    explicit kernel queues cells through its own range and one atomic: the
+    whole arrays after the call against the oracle."""
+    p = Input3D(np1x=36, np2x=36, np1y=22, np2y=22, np1z=68, np2z=68,
+                ql1x=10.0, ql2x=10.0, ql1y=5.0, ql2y=5.0, ql1z=5.0, ql2z=5.0)
+    if periodic:
+        p.bcx1 = p.bcx2 = 3
+    st = ics.random_state(p, seed=51)
+    for f in (0, 4) + tuple(range(8, 16)):
+        st[f] *= 1.0e4
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2))
+    g = mcf.Domain(p, nsplit=(2, 2, 2))
+    g.upload(st)
+    print("source backward-Euler", periodic, flush=True)
+    dt, _ = o.timestep(0.25)
+    g.timestep(0.25)
+    g.cool_stats(reset=True)
+    o.source(0.5 * dt)
+    g.source(0.5 * dt)
+    nst, _ = g.cool_stats()
+    ncell = 72 * 44 * 136
+    assert nst > ncell, (nst, ncell)  # over half the cells in each of the two stages
+    ref = o.state()
+    got = g.download()
+    for f in range(ref.shape[0]):
+        scale = max(np.abs(ref[f]).max(), 1e-300)
+        assert np.abs(got[f] - ref[f]).max() <= STEP_TOL * scale, f
+
+
 def test_multistep_vs_oracle_64():
     p = small(64, 32, 32)
     st = ics.random_state(p, seed=5)
