@@ -127,7 +127,7 @@ def kernel_stage(name):
         return "sweep_y"
     if name.endswith("<2>"):
         return "sweep_z"
-    if name in ("k_cc1", "k_ct_march", "k_cc3", "k_cc4"):
+    if name in ("k_cc1", "k_ct_march", "k_ct_march_mixed", "k_cc3", "k_cc4"):
         return "ct"
     if name in ("k_src_col", "k_shell_copy", "k_src_stiff", "k_src_imc", "k_shield", "k_chem"):
         return "source"

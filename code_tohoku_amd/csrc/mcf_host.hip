@@ -733,7 +733,7 @@ static int do_ct(mcf_ctx *x, double dt) {
     // registers cost more than the idle lanes of the fifth tile.
     dim3 gr((unsigned)((n[0] + 1 + CTM_OUT - 1) / CTM_OUT), (unsigned)((n[1] + 1 + CTM_ROWS - 1) / CTM_ROWS),
             (unsigned)((n[2] + 1 + CTM_ZSEG - 1) / CTM_ZSEG));
-    if (x->mixed) KLAUNCH_N("k_ct_march", k_ct_march<float>, gr, dim3(64 * CTM_ROWS), 8.0 * boxsize(b1) * 11, A, x->g, dt);
+    if (x->mixed) KLAUNCH_N("k_ct_march_mixed", k_ct_march_mixed, gr, dim3(64 * CTM_ROWS), 8.0 * boxsize(b1) * 11, A, x->g, dt);
     else KLAUNCH_N("k_ct_march", k_ct_march<double>, gr, dim3(64 * CTM_ROWS), 8.0 * boxsize(b1) * 11, A, x->g, dt);
   }
   KLAUNCH512(k_cc3, boxsize(b1), 9, A, x->g, b1, dt);

@@ -194,6 +194,7 @@ __global__ void k_fill_yz(FillList L, Geo g, int nol);
 __global__ void k_fill_xrule(XFill X, Geo g, int nol);
 __global__ void k_cc1(CtArgs A, Geo g, Box b);
 template <class R> __global__ void k_ct_march(CtArgs A, Geo g, double dt);
+__global__ void k_ct_march_mixed(CtArgs A, Geo g, double dt); // the FP32 march at three waves per SIMD
 // k_ct_march: EMF cells per wave along x, rows (waves) per block, planes
 // per block (16, 32 and 64 planes measured 1.01 / 1.04 / 1.11 ms, mixed,
 // 256^3; 2 and 8 rows no faster than 4)

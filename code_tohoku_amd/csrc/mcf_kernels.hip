@@ -277,8 +277,7 @@ template <class R> __device__ __forceinline__ double cct_core(const EdgeIn<R> &E
 // (mixed / FP64, 256^3) with bit-identical EMFs; sharing the row-(j-1)
 // slopes through LDS instead of forming them in each wave needs a barrier
 // per plane and took 1.29 / 1.94.
-template <class R>
-__global__ __launch_bounds__(64 * CTM_ROWS) void k_ct_march(CtArgs A, Geo g, double dt_d) {
+template <class R> __device__ __forceinline__ void ct_march(const CtArgs &A, const Geo &g, double dt_d) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int j = 1 + CTM_ROWS * (int)blockIdx.y + w;
@@ -398,8 +397,17 @@ __global__ __launch_bounds__(64 * CTM_ROWS) void k_ct_march(CtArgs A, Geo g, dou
     rho_km = rho; rhoJ_km = rhoJ;
   }
 }
+template <class R>
+__global__ __launch_bounds__(64 * CTM_ROWS) void k_ct_march(CtArgs A, Geo g, double dt_d) {
+  ct_march<R>(A, g, dt_d);
+}
 template __global__ void k_ct_march<double>(CtArgs, Geo, double);
-template __global__ void k_ct_march<float>(CtArgs, Geo, double);
+// The FP32 (mixed) march capped at 168 registers, three waves per SIMD
+// instead of two (36 bytes of scratch spill): 0.99-1.02 against 1.03 ms at
+// 256^3, the same EMFs.  The FP64 march would spill 348 bytes at that cap.
+__global__ __launch_bounds__(64 * CTM_ROWS, 3) void k_ct_march_mixed(CtArgs A, Geo g, double dt_d) {
+  ct_march<float>(A, g, dt_d);
+}
 
 // CC(3): face-B induction (M3:2159-2169).  A variant fused with CC(4),
 // out of place over the whole box (each thread also forming the new faces
