@@ -5,9 +5,10 @@
 # /opt/conda.  Test infrastructure: the binary goes to oracle/_ref/bin/ and is
 # used only to write golden fixtures in this container.
 #
-# usage: oracle/build_ref.sh [NDX [dtf] [chem]]  (NDX = compile-time block size, default 18;
+# usage: oracle/build_ref.sh [NDX [dtf] [chem] [grav]]  (NDX = compile-time block size, default 18;
 #        dtf: read the inhomogeneous inflow density field DTF/D{rank}.dat;
-#        chem: keep the reference's chemistry-network Fcool)
+#        chem: keep the reference's chemistry-network Fcool;
+#        grav: self-gravity runs, the multigrid arrays zero-filled on entry)
 #
 # -O2 without -march: baseline x86-64 has no FMA, so nothing is fused.
 set -euo pipefail
@@ -29,7 +30,10 @@ mkdir -p "$OUT/bin"
 TMP=$(mktemp -d "$OUT/tmp_build.XXXXXX")
 trap 'rm -rf "$TMP"' EXIT
 python3 "$HERE/patch_ref.py" "$REF/MCF3DSGincfca3.f90" "$TMP/m3.f90" "$NDX" $VARIANT
-cp "$REF/selfG_MPIfull8.f90" "$TMP/sg8.f90"
+case " $VARIANT " in
+  *" grav "*) python3 "$HERE/patch_ref.py" --sg8 "$REF/selfG_MPIfull8.f90" "$TMP/sg8.f90" ;;
+  *) cp "$REF/selfG_MPIfull8.f90" "$TMP/sg8.f90" ;;
+esac
 cd "$TMP"
 $FC -O2 -I$MPI/include -c m3.f90 -o m3.o 2>build.log || { cat build.log; exit 1; }
 $FC -O2 -I$MPI/include -c sg8.f90 -o sg8.o 2>>build.log || { cat build.log; exit 1; }

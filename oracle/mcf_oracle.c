@@ -22,6 +22,7 @@
  * depend on the thread count (tests/test_oracle_golden.py checks 1 and 4).
  */
 #include "mcf_oracle.h"
+#include "mcf_grav.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -59,6 +60,8 @@ struct mcfo {
   float *dtfa;  /* inhomogeneous inflow density field DTF (REAL*4), NULL = uniform */
   double *col[10]; /* Ntot, NH2, NnC, NCO, tCII seen from the left / right x end (col[2q+dir]) */
   int nsp[3];   /* emulated MPI block split */
+  double *phi;  /* gravitational potential Phi (ifgrv = 2), same layout as U */
+  mcfg *grav;   /* the multigrid solver of GRAVTY (mcf_grav.c) */
   double *pw;   /* pencil work pool, one slice of 64 arrays per thread */
   int nthr;     /* threads the pool has slices for */
 };
@@ -196,6 +199,15 @@ mcfo *mcfo_create(const mcfo_params *p) {
   }
   o->ifevo = 1;
   setup_constants(o);
+  if (p->ifgrv == 2) {
+    const double *xs[3] = {o->gx[0], o->gx[1], o->gx[2]}, *ds[3] = {o->gdx[0], o->gdx[1], o->gdx[2]};
+    o->phi = (double *)calloc(o->ncell, sizeof(double));
+    o->grav = mcfg_create(o->n, o->nsp, p->ql1x + p->ql2x, xs, ds);
+    if (!o->grav) {
+      mcfo_destroy(o);
+      return NULL;
+    }
+  }
   return o;
 }
 
@@ -239,6 +251,8 @@ void mcfo_destroy(mcfo *o) {
   free(o->pw);
   free(o->dtfa);
   for (int q = 0; q < 10; q++) free(o->col[q]);
+  free(o->phi);
+  mcfg_destroy(o->grav);
   free(o);
 }
 
@@ -1704,16 +1718,52 @@ void mcfo_dissip(mcfo *o) {
   free(divv);
 }
 
+/* GRAVTY(dt,2) (SG8:15-79): the density ghosts as BC_MPI(1,1) and PB's
+ * BC_MPI(2,1) fill them, then the multigrid solve into Phi. */
+int mcfo_grav_solve(mcfo *o) {
+  if (!o->grav) return -1;
+  static const int l1[1] = {1};
+  fillU(o, 7, 1, 1, l1, 1);
+  fillU(o, 7, 2, 1, l1, 1);
+  mcfg_solve(o->grav, U(1), o->phi);
+  return 0;
+}
+
+/* GRAVTY(dt,3) (SG8:82-89): the velocities take dt/2 of the fourth-order
+ * potential gradient, dxi = 1/(12 dx(0)) with dx(0) of the cell's rank. */
+void mcfo_grav_kick(mcfo *o, double dt) {
+  if (!o->phi) return;
+  const int nx = o->n[0], cx = o->n[0] / o->nsp[0];
+  const double *P = o->phi;
+  const long sx = o->st[0], sy = o->st[1], sz = o->st[2];
+  for (int k = 1; k <= o->n[2]; k++)
+    for (int j = 1; j <= o->n[1]; j++)
+      for (int i = 1; i <= nx; i++) {
+        const double dxi = 1.0 / (12.0 * dxg(o, 0, ((i - 1) / cx) * cx));
+        const long c = ix3(o, i, j, k);
+        U(2)[c] = U(2)[c] - dt * (-P[c + 2 * sx] + 8.0 * P[c + sx] - 8.0 * P[c - sx] + P[c - 2 * sx]) * dxi * 0.5;
+        U(3)[c] = U(3)[c] - dt * (-P[c + 2 * sy] + 8.0 * P[c + sy] - 8.0 * P[c - sy] + P[c - 2 * sy]) * dxi * 0.5;
+        U(4)[c] = U(4)[c] - dt * (-P[c + 2 * sz] + 8.0 * P[c + sz] - 8.0 * P[c - sz] + P[c - 2 * sz]) * dxi * 0.5;
+      }
+}
+
+double *mcfo_phi(mcfo *o) { return o->phi; }
+
 /* Sweep order for ifEVO = 1..6 (M3:611-635) */
 static const int sweep_order[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 0}};
 
 void mcfo_step(mcfo *o, double dt) {
+  if (o->p.ifgrv == 2) mcfo_grav_kick(o, dt); /* M3:607 */
   mcfo_source(o, 0.5 * dt);
   const int *ord = sweep_order[o->ifevo - 1];
   for (int t = 0; t < 3; t++) mhd_sweep(o, ord[t], dt);
   o->ifevo = (o->ifevo % 6) + 1;
   mcfo_ct(o, dt);
   mcfo_source(o, 0.5 * dt);
+  if (o->p.ifgrv == 2) { /* M3:653 */
+    mcfo_grav_solve(o);
+    mcfo_grav_kick(o, dt);
+  }
   mcfo_dissip(o);
   o->time = o->time + dt;
 }

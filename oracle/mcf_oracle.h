@@ -86,6 +86,16 @@ double *mcfo_column(mcfo *o, int q, int dir);
 /* Chemistry-form Fcool at cell (i,j,k) and temperature T (units of 10^3 K). */
 double  mcfo_fcool_cell(mcfo *o, int i, int j, int k, double T);
 
+/* Self-gravity (ifgrv = 2; mcf_grav.c).  mcfo_grav_solve is GRAVTY(dt,2):
+ * fills the density ghosts and solves for Phi (INITIA calls it once after
+ * the state is read, M3:481-484); returns -1 without gravity.
+ * mcfo_grav_kick is GRAVTY(dt,3).  mcfo_phi: Phi, the layout of U (NULL
+ * without gravity).  mcfo_create returns NULL for a grid the solver does
+ * not take (power-of-two cells per axis, power-of-two splits). */
+int     mcfo_grav_solve(mcfo *o);
+void    mcfo_grav_kick(mcfo *o, double dt);
+double *mcfo_phi(mcfo *o);
+
 /* Pieces of the step (same names as the reference routines). */
 void   mcfo_cc4(mcfo *o);                           /* CC(4): Bcc from faces + halo   */
 double mcfo_couran(mcfo *o);                        /* Couran  M3:1912-1930           */

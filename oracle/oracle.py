@@ -85,6 +85,11 @@ def lib():
         L.mcfo_init_columns.argtypes = [P]
         L.mcfo_column.restype = pd
         L.mcfo_column.argtypes = [P, C.c_int, C.c_int]
+        L.mcfo_grav_solve.restype = C.c_int
+        L.mcfo_grav_solve.argtypes = [P]
+        L.mcfo_grav_kick.argtypes = [P, D]
+        L.mcfo_phi.restype = pd
+        L.mcfo_phi.argtypes = [P]
         L.mcfo_fcool_cell.restype = D
         L.mcfo_fcool_cell.argtypes = [P, C.c_int, C.c_int, C.c_int, D]
         _lib = L
@@ -118,6 +123,8 @@ class Oracle:
         self.n = p.n
         self._params = to_params(p, nsplit, cooling)
         self.h = lib().mcfo_create(C.byref(self._params))
+        if not self.h:
+            raise ValueError("oracle: grid or block split not supported")
         # threads < 1: OpenMP's default (OMP_NUM_THREADS or all cores)
         self.threads = lib().mcfo_set_threads(self.h, threads)
         if self.threads < 1:
@@ -128,6 +135,8 @@ class Oracle:
             self.set_state(state)
             lib().mcfo_cc4(self.h)
             lib().mcfo_init_columns(self.h)
+            if p.ifgrv == 2:  # INITIA: GRAVTY(0,2) after the state is read (M3:481-484)
+                lib().mcfo_grav_solve(self.h)
         lib().mcfo_set_time(self.h, time, ifevo)
 
     def __del__(self):
@@ -164,6 +173,20 @@ class Oracle:
             return None
         return np.stack([np.stack([np.ctypeslib.as_array(lib().mcfo_column(self.h, q, d), shape=self.shape).copy()
                                    for d in range(2)]) for q in range(5)])
+
+    def phi(self) -> np.ndarray | None:
+        """Gravitational potential (ifgrv = 2), the layout of a field."""
+        ptr = lib().mcfo_phi(self.h)
+        if not ptr:
+            return None
+        return np.ctypeslib.as_array(ptr, shape=self.shape).copy()
+
+    def grav_solve(self):
+        if lib().mcfo_grav_solve(self.h) != 0:
+            raise ValueError("oracle: no self-gravity (ifgrv != 2)")
+
+    def grav_kick(self, dt):
+        lib().mcfo_grav_kick(self.h, dt)
 
     def fcool_cell(self, i, j, k, T):
         return lib().mcfo_fcool_cell(self.h, i, j, k, T)

@@ -22,7 +22,18 @@ Edits (each recorded in the fixture manifest written by ref_runner.py):
      reads DTF/D{rank}.dat (M3:405-431) and the x-inflow ghost density is the
      DTF field shifted by -v t (M3:818-826, 854-861).
 
+With the "grav" option the self-gravity file 3DMHD-CfCA/selfG_MPIfull8.f90
+is edited too (patch_sg8): mglin's automatic arrays (SG8:155-158) are set to
+zero on entry and PB's boundary arrays bphi1/bphi2 right after their
+ALLOCATE (SG8:1066, 1080).  The unedited code reads some of them before
+writing them (the y/z ghost planes of the coarsest split level, which
+`divide` skips on the first ranks, SG8:369-382), so its potential depends on
+memory left by earlier calls: over 3 steps at 32^3 on 8 ranks the unedited
+build and the zero-filled one differ by 7.8e-10 in Phi (0.15), and filling
+the arrays with 1e30 instead gives NaN.
+
 usage: patch_ref.py SRC DST NDX [dtf] [chem]
+       patch_ref.py --sg8 SRC DST
 """
 import re
 import sys
@@ -57,10 +68,31 @@ def patch(text: str, ndx: int, dtf: bool = False, chem: bool = False) -> str:
     return text
 
 
+def patch_sg8(text: str) -> str:
+    """Zero-fill mglin's arrays on entry and PB's bphi1/bphi2 after ALLOCATE."""
+    a = "character(3) Nfinal\n\n!Pre-BC for rho is necessary"
+    assert text.count(a) == 1, "mglin declarations not found"
+    text = text.replace(a, "character(3) Nfinal\n"
+                        "cphi1=0.d0; crho1=0.d0; cres1=0.d0; crhs1=0.d0\n"
+                        "cphi2=0.d0; crho2=0.d0; cres2=0.d0; crhs2=0.d0\n"
+                        "!Pre-BC for rho is necessary")
+    for arr in ("bphi1", "bphi2"):
+        b = f"ALLOCATE({arr}(Needb,2))\n"
+        assert text.count(b) == 1, b
+        text = text.replace(b, b + f"{arr}=0.d0\n")
+    return text
+
+
 if __name__ == "__main__":
+    if sys.argv[1] == "--sg8":
+        with open(sys.argv[2]) as f:
+            t = f.read()
+        with open(sys.argv[3], "w") as f:
+            f.write(patch_sg8(t))
+        sys.exit(0)
     src, dst, ndx = sys.argv[1], sys.argv[2], int(sys.argv[3])
     opts = sys.argv[4:]
-    assert all(o in ("dtf", "chem") for o in opts), opts
+    assert all(o in ("dtf", "chem", "grav") for o in opts), opts
     with open(src) as f:
         t = f.read()
     with open(dst, "w") as f:

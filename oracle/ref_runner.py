@@ -26,8 +26,8 @@ REF_BIN = os.path.join(HERE, "_ref", "bin")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
-def ref_binary(ndx: int, dtf: bool = False, chem: bool = False) -> str:
-    opts = (["dtf"] if dtf else []) + (["chem"] if chem else [])
+def ref_binary(ndx: int, dtf: bool = False, chem: bool = False, grav: bool = False) -> str:
+    opts = (["dtf"] if dtf else []) + (["chem"] if chem else []) + (["grav"] if grav else [])
     path = os.path.join(REF_BIN, f"mcf_ref_ndx{ndx}" + "".join("_" + o for o in opts))
     if not os.path.exists(path):
         cmd = ["bash", os.path.join(HERE, "build_ref.sh"), str(ndx)] + opts
@@ -61,14 +61,18 @@ def run_reference(params: rs.Input3D, state: np.ndarray, nsteps: int, nranks: in
     """dtf (float32 [k, j, i], global interior): run the build that reads
     the inhomogeneous inflow density field, with these values.  cooling:
     "ki" runs the build with the Koyama-Inutsuka Fcool, "chem" the build
-    with the reference's own chemistry-network Fcool."""
+    with the reference's own chemistry-network Fcool.  With params.ifgrv = 2
+    the build whose multigrid arrays start at zero runs (patch_ref.py), and
+    the result carries "phi", the potential the restart files hold, over
+    the global 1..N+1 index range like the state."""
     assert cooling in ("ki", "chem"), cooling
     nsplit = rs.NSPLIT_TABLE[nranks]
     nx, ny, nz = params.n
     cx, cy, cz = nx // nsplit[0], ny // nsplit[1], nz // nsplit[2]
     assert cx * nsplit[0] == nx and cy * nsplit[1] == ny and cz * nsplit[2] == nz
     ndx = max(cx, cy, cz) + 2
-    exe = ref_binary(ndx, dtf is not None, cooling == "chem")
+    grav = params.ifgrv == 2
+    exe = ref_binary(ndx, dtf is not None, cooling == "chem", grav)
     run = os.path.join(HERE, "_ref", "runs", tag)
     if os.path.isdir(run):
         shutil.rmtree(run)
@@ -91,11 +95,13 @@ def run_reference(params: rs.Input3D, state: np.ndarray, nsteps: int, nranks: in
         m = re.match(r"^\s*(\d+)\s+([-+0-9.EeDd]+)\s+([-+0-9.EeDd]+)\s*$", ln)
         if m:
             dts.append(float(m.group(3).replace("D", "E").replace("d", "e")))
-    blocks = []
+    blocks, phis = [], []
     for r in range(nranks):
-        b, _ = rs.read_restart_rank(os.path.join(run, f"000{r:03d}.dat"), cx, cy, cz)
+        b, ph = rs.read_restart_rank(os.path.join(run, f"000{r:03d}.dat"), cx, cy, cz)
         blocks.append(b)
+        phis.append(np.broadcast_to(ph[None], (rs.NFIELD,) + ph.shape))
     out = rs.assemble_state(blocks, (nx, ny, nz), nsplit)
+    phi = rs.assemble_state(phis, (nx, ny, nz), nsplit)[0]
     with open(os.path.join(run, "tsave.DAT")) as f:
         t_end = float(f.readline().replace("D", "E"))
-    return {"state": out, "dts": dts, "time": t_end, "stdout": res.stdout, "run_dir": run}
+    return {"state": out, "phi": phi, "dts": dts, "time": t_end, "stdout": res.stdout, "run_dir": run}
