@@ -23,7 +23,7 @@ FDRIVER = os.path.join(HERE, "fdriver")
 EVOLVE = os.path.join(LIBDIR, "mcf_evolve")
 ARCH = os.environ.get("MCF_ARCH", "gfx950")
 
-SOURCES = ["mcf_kernels.hip", "mcf_sweep.hip", "mcf_cool.hip", "mcf_chem.hip", "mcf_host.hip"]
+SOURCES = ["mcf_kernels.hip", "mcf_sweep.hip", "mcf_cool.hip", "mcf_chem.hip", "mcf_grav.hip", "mcf_host.hip"]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result"]

@@ -3,6 +3,7 @@
 // and the stage order of EVOLVE (3DMHD-CfCA/MCF3DSGincfca3.f90, "M3").
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "../../include/mcf_abi.h"
+#include "mcf_grav.h"
 #include "mcf_kernels.h"
 #include "mcf_mlp.h"
 
@@ -123,6 +125,16 @@ struct mcf_ctx {
   // many times each stage ran while profiling
   const char *stage = "other";
   std::map<std::string, int> stage_runs;
+  // self-gravity (ifgrv = 2): multigrid geometry and arrays, the potential
+  // Phi (layout of U) and PB's Fourier work arrays
+  bool grav = false;
+  GravGeo gg{};
+  GravBuf gb{};
+  PbArgs pb{};
+  double *gpool = nullptr; // one allocation for all of it
+  long gpool_n = 0;        // doubles in gpool cleared at each solve
+  double *phi = nullptr;
+  double2 *pbZ = nullptr, *pbS = nullptr, *pbD = nullptr, *pbSD = nullptr;
 };
 
 // Marks the launches made while it lives as one run of a stage of the step.
@@ -870,6 +882,312 @@ static int do_dissip(mcf_ctx *x) {
 }
 
 // ----------------------------------------------------------------------------
+// self-gravity (GRAVTY, 3DMHD-CfCA/selfG_MPIfull8.f90)
+// ----------------------------------------------------------------------------
+static int ilog2i(int v) {
+  int l = 0;
+  while ((1 << (l + 1)) <= v) ++l;
+  return l;
+}
+
+// The grid GRAVTY takes: a cube of 2^NGL cells per axis, power-of-two
+// splits, NGcr = log2(max split) + 1 below NGL; here also NGcr <= 4 (the
+// whole-domain levels live in one workgroup's LDS) and n <= 1024 (FFT
+// lines in LDS).
+static const char *grav_check(const int n[3], const int nsp[3]) {
+  if (n[0] != n[1] || n[0] != n[2]) return "self-gravity needs the same cell count on every axis";
+  if (n[0] < 8 || n[0] > 1024 || (n[0] & (n[0] - 1))) return "self-gravity needs 8..1024 cells per axis, a power of two";
+  int ms = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (nsp[d] & (nsp[d] - 1)) return "self-gravity needs power-of-two block splits";
+    if (n[d] / nsp[d] < 2) return "self-gravity needs at least 2 cells per block";
+    ms = std::max(ms, nsp[d]);
+  }
+  const int NGL = ilog2i(n[0]), NGcr = ilog2i(ms) + 1;
+  if (NGcr >= NGL) return "self-gravity needs more cells per block (NGcr < NGL)";
+  if (NGcr > GRAV_MAX_NGCR) return "self-gravity takes at most 8 blocks along an axis";
+  return nullptr;
+}
+
+// GravGeo and the device arrays (pinter SG8:102-146, PB's pointers
+// SG8:1058-1080), the x planes PB transforms and the rlft3 twiddles
+static int grav_setup(mcf_ctx *x) {
+  GravGeo &G = x->gg;
+  const mcf_config &cfg = x->cfg;
+  for (int d = 0; d < 3; ++d) { G.S[d] = x->nsp[d]; G.C[d] = x->g.n[d] / x->nsp[d]; }
+  G.npe = G.S[0] * G.S[1] * G.S[2];
+  G.NGL = ilog2i(x->g.n[0]);
+  G.NGcr = ilog2i(std::max(G.S[0], std::max(G.S[1], G.S[2]))) + 1;
+  G.Lbox = cfg.ql1x + cfg.ql2x;
+  long tot2 = 0, totb = 0;
+  for (int l = G.NGL; l >= G.NGcr; --l) {
+    for (int d = 0; d < 3; ++d) G.n[l][d] = l == G.NGL ? G.C[d] + 1 : G.n[l + 1][d] / 2 + 1;
+  }
+  for (int l = G.NGcr; l <= G.NGL; ++l) {
+    G.bsz[l] = (long)(G.n[l][0] + 1) * (G.n[l][1] + 1) * (G.n[l][2] + 1);
+    G.off[l] = tot2;
+    tot2 += G.bsz[l] * G.npe;
+    G.boff[l] = totb;
+    totb += 2L * G.npe * (G.n[l][1] + 1) * (G.n[l][2] + 1);
+  }
+  G.p1[1] = 1;
+  for (int l = 1; l <= G.NGcr; ++l) { const long m = (1L << l) + 1; G.p1[l + 1] = G.p1[l] + m * m * m; }
+  G.need1 = G.p1[G.NGcr + 1];
+  G.pb1[1] = 1;
+  for (int l = 1; l <= G.NGcr; ++l) { const long m = (1L << l) + 1; G.pb1[l + 1] = G.pb1[l] + m * m; }
+  G.needb1 = G.pb1[G.NGcr + 1];
+  if (G.need1 > GRAV_NEED1_MAX) { g_err = "self-gravity: whole-domain levels exceed the LDS arrays"; return MCF_E_ARG; }
+  // PB's planes: x planes 1..min(C0, 3P) of every x block (SG8:1106-1125)
+  const int P = G.S[1] * G.S[2], C0 = G.C[0];
+  std::vector<int> gxs;
+  std::vector<double> zp, zq, fg;
+  for (int ist = 0; ist < G.S[0]; ++ist) {
+    const double dzz = x->hdx[0][ist * C0 + 1 + 1];
+    for (int klr = 1; klr <= std::min(C0, 3 * P); ++klr) {
+      const int gx = ist * C0 + klr;
+      const double z = x->hx[0][gx + 1] - 0.5 * dzz;
+      gxs.push_back(gx);
+      zp.push_back(z);
+      zq.push_back(G.Lbox - z);
+      fg.push_back(-GRAV_G4PI * dzz);
+    }
+  }
+  const int npl = (int)gxs.size(), nn1 = x->g.n[1], nn2 = x->g.n[2], H = nn1 / 2, q4 = nn1 / 4;
+  // rlft3's unpacking twiddles: the recurrence in double, each value
+  // rounded to single precision by CMPLX (SG8:1476-1508); index i1 >= 2
+  std::vector<double2> tw[2];
+  for (int s = 0; s < 2; ++s) {
+    const int isign = s == 0 ? 1 : -1;
+    const double theta = 6.28318530717959 / (double)(isign * nn1);
+    const double sh = std::sin(0.5 * theta);
+    const double wpr = -(2.0 * (sh * sh)), wpi = std::sin(theta);
+    double wr = 1.0, wi = 0.0;
+    tw[s].assign(q4 + 2, make_double2(0.0, 0.0));
+    for (int i1 = 1; i1 <= q4; ++i1) {
+      const double wtemp = wr;
+      wr = wr * wpr - wi * wpi + wr;
+      wi = wi * wpr + wtemp * wpi + wi;
+      tw[s][i1 + 1] = make_double2((double)(float)wr, (double)(float)wi);
+    }
+  }
+  // every segment starts at an even double (16-byte alignment for double2)
+  auto ev = [](long c) { return (c + 1) & ~1L; };
+  const long n1 = ev(G.need1 + 1), nb1 = ev(2 * (G.needb1 + 1));
+  const long nz = 4 * ev(tot2) + 4 * n1 + nb1 + ev(totb); // cleared at every solve
+  const long nZ = 2L * npl * nn2 * H, nS = 2L * npl * nn2, nD = 2L * 2 * nn2 * H, nSD = 2L * 2 * nn2;
+  const long ntab = 4L * (q4 + 2);
+  const long total = nz + ev((long)x->ncell) + nZ + nS + nD + nSD + ntab + 4 * ev(npl) + 16;
+  HIPCHK(hipMalloc(&x->gpool, total * sizeof(double)));
+  double *p = x->gpool;
+  auto take = [&](long c) { double *r = p; p += ev(c); return r; };
+  x->gb.phi2 = take(tot2);
+  x->gb.rho2 = take(tot2);
+  x->gb.res2 = take(tot2);
+  x->gb.rhs2 = take(tot2);
+  x->gb.phi1 = take(n1);
+  x->gb.rho1 = take(n1);
+  x->gb.res1 = take(n1);
+  x->gb.rhs1 = take(n1);
+  x->gb.bphi1 = take(nb1);
+  x->gb.bphi2 = take(totb);
+  x->gpool_n = nz;
+  x->phi = take((long)x->ncell);
+  x->pbZ = (double2 *)take(nZ);
+  x->pbS = (double2 *)take(nS);
+  x->pbD = (double2 *)take(nD);
+  x->pbSD = (double2 *)take(nSD);
+  double2 *wf = (double2 *)take(2L * (q4 + 2));
+  double2 *wi = (double2 *)take(2L * (q4 + 2));
+  double *dzp = take(npl);
+  double *dzq = take(npl);
+  double *dfg = take(npl);
+  int *dgx = (int *)take(npl);
+  if (p > x->gpool + total) { g_err = "self-gravity: pool layout"; return MCF_E_ARG; }
+  HIPCHK(hipMemsetAsync(x->gpool, 0, total * sizeof(double), x->stream));
+  HIPCHK(hipMemcpyAsync(wf, tw[0].data(), (q4 + 2) * sizeof(double2), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(wi, tw[1].data(), (q4 + 2) * sizeof(double2), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(dzp, zp.data(), npl * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(dzq, zq.data(), npl * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(dfg, fg.data(), npl * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(dgx, gxs.data(), npl * sizeof(int), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  PbArgs &B = x->pb;
+  B.np = npl;
+  B.nn1 = nn1;
+  B.nn2 = nn2;
+  B.gx = dgx;
+  B.zp = dzp;
+  B.zq = dzq;
+  B.facG = dfg;
+  B.dxx = x->hdx[1][1 + 1];
+  B.dyy = x->hdx[2][1 + 1];
+  B.wf = wf;
+  B.wi = wi;
+  x->grav = true;
+  return MCF_OK;
+}
+
+static unsigned gblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// one exchange of split level l (flags: x low, x high, y low, y high, z low, z high)
+static int grav_exchange(mcf_ctx *x, double *u2, int l, int flags) {
+  const GravGeo &G = x->gg;
+  const int *n = G.n[l];
+  const long per = 2L * (n[0] + 1) * (n[1] + 1) + 2L * (n[0] + 1) * (n[2] - 1) + 2L * (n[1] - 1) * (n[2] - 1);
+  KLAUNCH_N("k_grav_exchange", k_grav_exchange, dim3(gblocks(per * G.npe)), dim3(TPB), 16.0 * per * G.npe, G, u2, l, flags);
+  return MCF_OK;
+}
+static const int XALL = 63, XLOW = 1 | 4 | 16;
+
+static int grav_relax(mcf_ctx *x, int l, int times) {
+  const GravGeo &G = x->gg;
+  for (int t = 0; t < times; ++t) {
+    KLAUNCH_N("k_grav_relax", k_grav_relax, dim3(G.npe), dim3(GRAV_RELAX_THREADS), 16.0 * G.bsz[l] * G.npe, G,
+              x->gb.phi2, x->gb.rhs2, l);
+    CHK(grav_exchange(x, x->gb.phi2, l, XALL));
+  }
+  return MCF_OK;
+}
+
+static int grav_zero(mcf_ctx *x, double *a, long n) {
+  if (n > 0) KLAUNCH_N("k_grav_zero", k_grav_zero, dim3((unsigned)std::min<long>(gblocks(n), 4096)), dim3(TPB), 8.0 * n, a, n);
+  return MCF_OK;
+}
+
+// GRAVTY(dt,2) (SG8:15-79): density ghosts (BC_MPI(1,1), PB's BC_MPI(2,1)),
+// PB, mglin, Phi with its face ghosts
+static int do_grav_solve(mcf_ctx *x) {
+  StageScope scope(x, "grav_solve");
+  const GravGeo &G = x->gg;
+  const GravBuf &B = x->gb;
+  const PbArgs &P = x->pb;
+  static const int l1[1] = {F_RHO};
+  CHK(fillU(x, 7, 1, 1, l1, 1));
+  CHK(fillU(x, 7, 2, 1, l1, 1));
+  CHK(grav_zero(x, x->gpool, x->gpool_n));
+  // PB
+  const int nn1 = P.nn1, nn2 = P.nn2, H = nn1 / 2, NGL = G.NGL, NGcr = G.NGcr;
+  const long nZ = (long)P.np * nn2 * H;
+  KLAUNCH_N("k_pb_pack", k_pb_pack, dim3(gblocks(nZ)), dim3(TPB), 16.0 * nZ, P, x->U[F_RHO], x->g, x->pbZ);
+  const int t1 = std::min(512, std::max(64, H / 2)), t2 = std::min(512, std::max(64, nn2 / 2));
+  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)((long)P.np * nn2)), dim3(t1), 32.0 * nZ, x->pbZ, H, 1L, (long)H,
+            (long)nn2 * H, nn2, 1);
+  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)((long)P.np * H)), dim3(t2), 32.0 * nZ, x->pbZ, nn2, (long)H, 1L,
+            (long)nn2 * H, H, 1);
+  const long nu = 2L * (nn2 / 2 + 1) + (long)(nn1 / 4 - 1) * nn2;
+  KLAUNCH_N("k_pb_unpack", k_pb_unpack, dim3(gblocks(nu * P.np)), dim3(TPB), 32.0 * nZ, P, x->pbZ, x->pbS, P.np, 1);
+  KLAUNCH_N("k_pb_weight", k_pb_weight, dim3(gblocks((long)(H + 1) * nn2)), dim3(TPB), 16.0 * nZ, P, x->pbZ, x->pbS,
+            x->pbD, x->pbSD);
+  KLAUNCH_N("k_pb_unpack", k_pb_unpack, dim3(gblocks(nu * 2)), dim3(TPB), 64.0 * nn2 * H, P, x->pbD, x->pbSD, 2, -1);
+  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)(2L * nn2)), dim3(t1), 64.0 * nn2 * H, x->pbD, H, 1L, (long)H,
+            (long)nn2 * H, nn2, -1);
+  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)(2L * H)), dim3(t2), 64.0 * nn2 * H, x->pbD, nn2, (long)H, 1L,
+            (long)nn2 * H, H, -1);
+  {
+    const long nb = 2L * G.npe * (G.C[1] + 2) * (G.C[2] + 2);
+    KLAUNCH_N("k_pb_bphi", k_pb_bphi, dim3(gblocks(nb)), dim3(TPB), 16.0 * nb, G, P, x->pbD, B.bphi2);
+    for (int lc = NGL - 1; lc >= NGcr; --lc) {
+      const long nr = 2L * G.npe * G.n[lc][1] * G.n[lc][2];
+      KLAUNCH_N("k_pb_restrict", k_pb_restrict, dim3(gblocks(nr)), dim3(TPB), 48.0 * nr, G, B.bphi2, lc);
+    }
+    const long N1 = (1L << NGcr) + 1;
+    KLAUNCH_N("k_pb_collect", k_pb_collect, dim3(gblocks(2 * N1 * N1)), dim3(TPB), 16.0 * N1 * N1, G, B.bphi1, B.bphi2);
+  }
+  // mglin (SG8:151-295)
+  {
+    const long nr = G.bsz[NGL] * G.npe;
+    KLAUNCH_N("k_grav_rhs", k_grav_rhs, dim3(gblocks(nr)), dim3(TPB), 16.0 * nr, G, x->U[F_RHO], x->g, B.rhs2);
+  }
+  auto rstrct = [&](double *uc, const double *uf, int lc, int mode) -> int {
+    const long nr = (long)G.n[lc][0] * G.n[lc][1] * G.n[lc][2] * G.npe;
+    KLAUNCH_N("k_grav_rstrct", k_grav_rstrct, dim3(gblocks(nr)), dim3(TPB), 64.0 * nr, G, uc, uf, lc, mode);
+    return grav_exchange(x, uc, lc, XALL);
+  };
+  auto collect = [&](double *u1, const double *u2) {
+    const long N1 = (1L << NGcr) + 1;
+    KLAUNCH_N("k_grav_collect", k_grav_collect, dim3(gblocks(N1 * N1 * N1)), dim3(TPB), 16.0 * N1 * N1 * N1, G, u1, u2);
+    return MCF_OK;
+  };
+  auto coarse = [&](int prog, int j) {
+    KLAUNCH_N("k_grav_coarse", k_grav_coarse, dim3(1), dim3(GRAV_COARSE_THREADS), 48.0 * G.need1, G, B, prog, j);
+    return MCF_OK;
+  };
+  auto divide = [&]() {
+    const long nr = G.bsz[NGcr] * G.npe;
+    KLAUNCH_N("k_grav_divide", k_grav_divide, dim3(gblocks(nr)), dim3(TPB), 16.0 * nr, G, B.phi2, B.phi1);
+    return MCF_OK;
+  };
+  auto copy_level = [&](double *dst, const double *src, int l) {
+    const long nr = G.bsz[l] * G.npe;
+    KLAUNCH_N("k_copy", k_copy, dim3(gblocks(nr)), dim3(TPB), 16.0 * nr, src + G.off[l], dst + G.off[l], nr);
+    return MCF_OK;
+  };
+  auto prolong = [&](double *uf, const double *uc, int lf, int mode) -> int {
+    const long nr = (long)G.n[lf][0] * G.n[lf][1] * G.n[lf][2] * G.npe;
+    KLAUNCH_N("k_grav_prolong", k_grav_prolong, dim3(gblocks(nr)), dim3(TPB), 16.0 * nr, G, uf, uc, lf, mode, B.bphi2);
+    return grav_exchange(x, uf, lf, XLOW);
+  };
+  CHK(rstrct(B.rho2, B.rhs2, NGL - 1, 0));
+  for (int l = NGL - 2; l >= NGcr; --l) CHK(rstrct(B.rho2, B.rho2, l, 0));
+  CHK(collect(B.rho1, B.rho2));
+  CHK(coarse(0, 0));
+  for (int j = 2; j <= NGL; ++j) {
+    if (j < NGcr) {
+      CHK(coarse(1, j));
+      continue;
+    }
+    if (j == NGcr) {
+      CHK(coarse(2, j));
+      CHK(divide());
+      CHK(copy_level(B.rhs2, B.rho2, NGcr));
+    } else {
+      CHK(prolong(B.phi2, B.phi2, j, 1));
+      if (j != NGL) CHK(copy_level(B.rhs2, B.rho2, j));
+    }
+    for (int cyc = 0; cyc < 2; ++cyc) {
+      for (int jj = j; jj >= NGcr; --jj) {
+        CHK(grav_relax(x, jj, j >= NGL ? 2 : 5));
+        {
+          const long nr = (long)G.n[jj][0] * G.n[jj][1] * G.n[jj][2] * G.npe;
+          KLAUNCH_N("k_grav_resid", k_grav_resid, dim3(gblocks(nr)), dim3(TPB), 24.0 * nr, G, B.res2, B.phi2, B.rhs2, jj);
+          CHK(grav_exchange(x, B.res2, jj, XALL));
+        }
+        if (jj == NGcr) {
+          CHK(collect(B.phi1, B.phi2));
+          CHK(collect(B.res1, B.res2));
+        } else {
+          CHK(rstrct(B.rhs2, B.res2, jj - 1, 1));
+          CHK(grav_zero(x, B.phi2 + G.off[jj - 1], G.bsz[jj - 1] * G.npe));
+        }
+      }
+      CHK(coarse(3, j));
+      CHK(divide());
+      CHK(grav_relax(x, NGcr, 5));
+      for (int jj = NGcr + 1; jj <= j; ++jj) {
+        CHK(prolong(B.res2, B.phi2, jj, 0));
+        const long nr = G.bsz[jj] * G.npe;
+        KLAUNCH_N("k_grav_add", k_grav_add, dim3(gblocks(nr)), dim3(TPB), 24.0 * nr, G, B.phi2, B.res2, jj);
+        CHK(grav_relax(x, jj, j >= NGL ? 2 : 5));
+      }
+    }
+  }
+  const long nc = (long)x->g.n[0] * x->g.n[1] * x->g.n[2];
+  KLAUNCH_N("k_grav_phi", k_grav_phi, dim3(gblocks(nc)), dim3(TPB), 16.0 * nc, G, B.phi2, x->phi, x->g);
+  const long ng = (long)x->g.n[1] * x->g.n[2] + (long)x->g.n[0] * x->g.n[2] + (long)x->g.n[0] * x->g.n[1];
+  KLAUNCH_N("k_grav_phi_ghosts", k_grav_phi_ghosts, dim3(gblocks(ng)), dim3(TPB), 64.0 * ng, x->phi, x->g);
+  return MCF_OK;
+}
+
+// GRAVTY(dt,3) (SG8:82-89)
+static int do_grav_kick(mcf_ctx *x, double dt) {
+  StageScope scope(x, "grav_kick");
+  const long nc = (long)x->g.n[0] * x->g.n[1] * x->g.n[2];
+  KLAUNCH_N("k_grav_kick", k_grav_kick, dim3(gblocks(nc)), dim3(TPB), 56.0 * nc, x->U[F_VX], x->U[F_VY], x->U[F_VZ],
+            x->phi, x->g, x->gdx[0], x->gg.C[0], dt);
+  return MCF_OK;
+}
+
+// ----------------------------------------------------------------------------
 // C ABI
 // ----------------------------------------------------------------------------
 extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
@@ -900,7 +1218,10 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   if (cfg->cooling != MCF_COOL_KI && cfg->cooling != MCF_COOL_SURROGATE && cfg->cooling != MCF_COOL_CHEM) {
     g_err = "unknown cooling mode"; return MCF_E_ARG;
   }
-  if (cfg->ifgrv == 2) { g_err = "self-gravity (ifgrv = 2) is not implemented"; return MCF_E_ARG; }
+  if (cfg->ifgrv == 2) {
+    const char *why = grav_check(n, nsp);
+    if (why) { g_err = why; return MCF_E_ARG; }
+  }
   if (cfg->ifrad == 2 && nsp[0] > SHIELD_MAXB) { g_err = "SHIELD takes at most 64 x blocks"; return MCF_E_ARG; }
   long long vol = (long long)(n[0] + 4) * (n[1] + 4) * (n[2] + 4);
   if (vol >= (1LL << 31)) { g_err = "grid too large for 32-bit indexing"; return MCF_E_ARG; }
@@ -984,6 +1305,15 @@ extern "C" int mcf_init(const mcf_config *cfg, mcf_ctx **out) {
   INITCHK(hipMemsetAsync(x->cool_cnt, 0, 4 * sizeof(unsigned long long), x->stream));
   INITCHK(hipStreamSynchronize(x->stream));
 #undef INITCHK
+  if (cfg->ifgrv == 2) {
+    const int r = grav_setup(x);
+    if (r != MCF_OK) {
+      const std::string why = g_err;
+      (void)mcf_finalize(x);
+      g_err = why;
+      return r;
+    }
+  }
   *out = x;
   return MCF_OK;
 }
@@ -999,6 +1329,7 @@ extern "C" int mcf_finalize(mcf_ctx *x) {
   if (x->cool_cnt) (void)hipFree(x->cool_cnt);
   if (x->hred) (void)hipHostFree(x->hred);
   if (x->dtfd) (void)hipFree(x->dtfd);
+  if (x->gpool) (void)hipFree(x->gpool);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return MCF_OK;
@@ -1020,6 +1351,8 @@ extern "C" int mcf_upload(mcf_ctx *x, const double *U8, const double *sp8) {
     if (x->cfg.ifrad == 2)
       for (int l = 0; l < 20; ++l) CHK(do_shield(x));
   }
+  // GRAVTY(0,2) after the restart read (M3:481-484)
+  if (x->grav) CHK(do_grav_solve(x));
   HIPCHK(hipStreamSynchronize(x->stream));
   return MCF_OK;
 }
@@ -1157,14 +1490,37 @@ extern "C" int mcf_dissip(mcf_ctx *x) {
 // ifEVO sweep orders xyz, yzx, zxy, xzy, yxz, zyx (M3:611-635)
 static const int k_order[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 0}};
 
+extern "C" int mcf_gravty(mcf_ctx *x, double dt, int mode) {
+  if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  if (!x->grav) { g_err = "self-gravity is off (ifgrv != 2)"; return MCF_E_STATE; }
+  if (mode == 1) return MCF_OK;
+  if (mode == 2) return do_grav_solve(x);
+  if (mode == 3) return do_grav_kick(x, dt);
+  g_err = "GRAVTY mode must be 1, 2 or 3";
+  return MCF_E_ARG;
+}
+
+extern "C" int mcf_download_phi(mcf_ctx *x, double *phi) {
+  if (!x || !phi) { g_err = "null argument"; return MCF_E_ARG; }
+  if (!x->grav) { g_err = "self-gravity is off (ifgrv != 2)"; return MCF_E_STATE; }
+  HIPCHK(hipMemcpyAsync(phi, x->phi, x->ncell * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return MCF_OK;
+}
+
 extern "C" int mcf_step(mcf_ctx *x, double dt) {
   if (!x) { g_err = "null argument"; return MCF_E_ARG; }
+  if (x->grav) CHK(do_grav_kick(x, dt)); // M3:607
   CHK(do_source(x, 0.5 * dt));
   const int *o = k_order[x->ifevo - 1];
   for (int t = 0; t < 3; ++t) CHK(do_sweep(x, o[t], dt));
   x->ifevo = x->ifevo % 6 + 1;
   CHK(do_ct(x, dt));
   CHK(do_source(x, 0.5 * dt));
+  if (x->grav) { // M3:653
+    CHK(do_grav_solve(x));
+    CHK(do_grav_kick(x, dt));
+  }
   CHK(do_dissip(x));
   x->time = x->time + dt;
   return MCF_OK;
@@ -1286,6 +1642,10 @@ static double stage_bytes(const mcf_ctx *x, const std::string &st) {
   }
   if (st == "dt") return 8.0 * 13.0 * cells;
   if (st == "dissip") return 8.0 * 6.0 * 20.0 * x->g.n[1] * x->g.n[2];
+  // GRAVTY(dt,2) reads rho and writes Phi; GRAVTY(dt,3) reads Phi and
+  // updates three velocities
+  if (st == "grav_solve") return 8.0 * 2.0 * cells;
+  if (st == "grav_kick") return 8.0 * 7.0 * cells;
   return 0.0;
 }
 

@@ -83,6 +83,8 @@ def lib():
         L.mcf_ct.argtypes = [P, D]
         L.mcf_dissip.argtypes = [P]
         L.mcf_step.argtypes = [P, D]
+        L.mcf_gravty.argtypes = [P, D, I]
+        L.mcf_download_phi.argtypes = [P, pd]
         L.mcf_run.argtypes = [P, I, D, pd, pi]
         L.mcf_sync.argtypes = [P]
         L.mcf_stream.argtypes = [P]
@@ -226,6 +228,16 @@ class Domain:
 
     def step(self, dt):
         _chk(lib().mcf_step(self.h, dt))
+
+    def gravty(self, dt: float, mode: int):
+        """GRAVTY(dt, mode): 2 solves for Phi, 3 kicks the velocities."""
+        _chk(lib().mcf_gravty(self.h, dt, mode))
+
+    def phi(self) -> np.ndarray:
+        """The gravitational potential Phi (ifgrv = 2), layout of a field."""
+        out = np.empty(self.shape)
+        _chk(lib().mcf_download_phi(self.h, _dp(out)))
+        return out
 
     def run(self, nsteps: int, tsave: float = 0.25):
         """EVOLVE inner loop: nsteps of (dt, step) with the tsave clip."""

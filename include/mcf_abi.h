@@ -55,8 +55,10 @@ typedef struct mcf_config {
    * 4 inflow; both x ends alike.  y, z: 3 (periodic) only. */
   int bcx1, bcx2, bcy1, bcy2, bcz1, bcz2;
   /* ifthrm = 2: cooling + conduction; ifchem = 2: the chemistry network;
-   * ifrad = 2: SHIELD column densities; ifgrv = 2 (self-gravity) is
-   * rejected */
+   * ifrad = 2: SHIELD column densities; ifgrv = 2: self-gravity (GRAVTY,
+   * see mcf_gravty), which needs a cube of 2^k <= 1024 cells per axis,
+   * power-of-two splits with at most 8 blocks per axis and
+   * log2(max split) + 1 < k */
   int ifchem, ifthrm, ifrad, ifgrv;
   double pinit_over_kb;                      /* 8.810807d3 */
   double Hini, pini, H2ini, Heini, Hepini, Cini, COini, Cpini;
@@ -130,6 +132,20 @@ int mcf_step(mcf_ctx *ctx, double dt);
 /* mcf_timestep + mcf_step repeated nsteps times (the EVOLVE inner loop,
  * M3:562-657, with the tsave/tfinal stops).  dts may be NULL. */
 int mcf_run(mcf_ctx *ctx, int nsteps, double tsave, double *dts, int *ndone);
+
+/* GRAVTY(dt,mode) of 3DMHD-CfCA/selfG_MPIfull8.f90:1-100 (ifgrv = 2):
+ * mode 2 solves Poisson's equation lap Phi = 4 pi G rho (multigrid, with
+ * the free-space potential of the y-z Fourier modes on the x ends, on the
+ * reference's block split) into Phi, after filling the density ghosts;
+ * mode 3 kicks the velocities by -dt/2 grad Phi (fourth-order
+ * differences); mode 1 (the reference's pointer set-up) does nothing.
+ * mcf_upload runs mode 2 once, as INITIA does (M3:481-484), and mcf_step
+ * calls mode 3 before the first SOURCE and modes 2, 3 after the second
+ * (M3:607, 653).  MCF_E_STATE when self-gravity is off. */
+int mcf_gravty(mcf_ctx *ctx, double dt, int mode);
+/* The potential Phi (M3:37), (-1:N+2)^3 layout: interior cells and face
+ * ghosts hold values. */
+int mcf_download_phi(mcf_ctx *ctx, double *phi);
 
 /* Per-kernel timing: while enabled, every kernel launch of the context is
  * bracketed by HIP events on the context's stream (enabling clears what was
