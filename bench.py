@@ -29,7 +29,12 @@ bounded sample of the same problem as the CPU baseline.
 ``chemistry`` times the same problem with the reference's chemistry network,
 SHIELD column densities and chemistry-network Fcool switched on.
 
+``gravity`` times the same problem with self-gravity (ifgrv = 2): the
+multigrid Poisson solve once per step and two velocity kicks, on the 4x4x4
+split and on the 8x8x8 split of the reference's 512-rank runs.
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pmc] [--no-stiff] [--no-chem]
+                       [--no-grav]
 """
 import argparse
 import csv
@@ -135,6 +140,10 @@ def kernel_stage(name):
         return "dt"
     if name in ("k_dissip_div", "k_dissip_upd"):
         return "dissip"
+    if name == "k_grav_kick":
+        return "grav_kick"
+    if name.startswith("k_grav_") or name.startswith("k_pb_"):
+        return "grav_solve"
     return None
 
 
@@ -297,6 +306,39 @@ def chem_compare(n, steps, warmup):
     return out
 
 
+def grav_compare(n, steps, warmup):
+    """The same converging flow with self-gravity (ifgrv = 2): GRAVTY(dt,3)
+    before the first SOURCE, GRAVTY(dt,2) and (dt,3) after the second
+    (M3:607, 653).  The multigrid follows the reference's block split, so
+    its cost depends on it: the 4x4x4 split of the headline and the 8x8x8
+    split of the reference's 512-rank runs, mixed precision (the MHD part)
+    with KI cooling, and FP64 on 8x8x8."""
+    from code_tohoku_amd import mcf
+    p, st = build_state(n)
+    p.ifgrv = 2
+    out = {"workload": f"converging_flow_{n}^3 with self-gravity (ifgrv = 2), KI cooling", "steps": steps}
+    for tag, nsplit, prec in (("mixed_4x4x4", (4, 4, 4), mcf.PREC_MIXED), ("mixed_8x8x8", (8, 8, 8), mcf.PREC_MIXED),
+                              ("fp64_8x8x8", (8, 8, 8), mcf.PREC_FP64)):
+        d = mcf.Domain(p, nsplit=nsplit, precision=prec)
+        d.upload(st)
+        el, dts, rep, stg, nprof = timed(d, steps, warmup, profile_steps=3)
+        d.close()
+        gs, gk = stg.get("grav_solve"), stg.get("grav_kick")
+        glaunch = sum(v[0] for k, v in rep.items() if kernel_stage(k) == "grav_solve") / nprof
+        top = sorted(((k, v) for k, v in rep.items() if kernel_stage(k) in ("grav_solve", "grav_kick")),
+                     key=lambda kv: -kv[1][1])[:12]
+        out[tag] = {"ms_per_step": 1e3 * el / steps, "zone_updates_per_s": n ** 3 * steps / el,
+                    "grav_solve_ms_per_step": (gs[1] / nprof) if gs else None,
+                    "grav_kick_ms_per_step": (gk[1] / nprof) if gk else None,
+                    "grav_solve_launches_per_step": glaunch,
+                    "kernels": {k: {"avg_us": round(1e3 * v[1] / v[0], 2), "launches_per_step": v[0] / nprof,
+                                    "ms_per_step": round(v[1] / nprof, 3)} for k, v in top},
+                    "dt_mean": float(np.mean(dts))}
+        log(f"[bench] gravity {tag}: {out[tag]['ms_per_step']:.3f} ms/step, solve "
+            f"{out[tag]['grav_solve_ms_per_step']:.3f} ms in {glaunch:.0f} launches")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # HBM traffic of the dominant kernel: child rocprofv3 --pmc runs
 # ---------------------------------------------------------------------------
@@ -397,6 +439,7 @@ def main():
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--no-stiff", action="store_true")
     ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--no-grav", action="store_true")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -473,6 +516,8 @@ def main():
         out["stiff"] = stiff_compare(n, 10, 2, w)
     if not args.no_chem:
         out["chemistry"] = chem_compare(n, 10, 2)
+    if not args.no_grav:
+        out["gravity"] = grav_compare(n, 5, 1)
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle {args.cpu_steps} steps at {args.cpu_n}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)

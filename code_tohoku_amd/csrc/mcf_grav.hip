@@ -40,8 +40,12 @@ __device__ __forceinline__ void g_rank(const GravGeo &G, int r, int c[3]) {
   c[1] = r / G.S[0] - G.S[1] * c[2];
 }
 __device__ __forceinline__ int g_rid(const GravGeo &G, int a, int b, int c) { return a + G.S[0] * (b + G.S[1] * c); }
+// Skewed block layout: node (i,j,k) at j + J ((i+j+k) + T k), J = n1+1,
+// T = n0+n1+n2+1, so that the nodes of one hyperplane i+j+k = t and one k
+// are contiguous in j; the six neighbours sit at +-J (x), +-(1+J) (y) and
+// +-J(1+T) (z).  Slots with i outside 0..n0 are never used.
 __device__ __forceinline__ long g_idx(const int *n, int i, int j, int k) {
-  return i + (long)(n[0] + 1) * (j + (long)(n[1] + 1) * k);
+  return j + (long)(n[1] + 1) * ((i + j + k) + (long)(n[0] + n[1] + n[2] + 1) * k);
 }
 __device__ __forceinline__ long g_tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 
@@ -54,7 +58,7 @@ __global__ void k_grav_zero(double *a, long n) {
 __global__ void k_grav_rhs(GravGeo G, const double *rho, Geo g, double *rhs2) {
   const int L = G.NGL;
   const int *n = G.n[L];
-  const long B = G.bsz[L], t = g_tid();
+  const long B = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1), t = g_tid();
   if (t >= B * G.npe) return;
   const int r = (int)(t / B);
   long q = t - r * B;
@@ -63,7 +67,8 @@ __global__ void k_grav_rhs(GravGeo G, const double *rho, Geo g, double *rhs2) {
   const int j = (int)(q % (n[1] + 1)), k = (int)(q / (n[1] + 1));
   int c[3];
   g_rank(G, r, c);
-  rhs2[G.off[L] + t] = rho[ix3(g, c[0] * G.C[0] + i, c[1] * G.C[1] + j, c[2] * G.C[2] + k)] * GRAV_G4PI;
+  rhs2[G.off[L] + r * G.bsz[L] + g_idx(n, i, j, k)] =
+      rho[ix3(g, c[0] * G.C[0] + i, c[1] * G.C[1] + j, c[2] * G.C[2] + k)] * GRAV_G4PI;
 }
 
 // rstrctMPI (SG8:385-456) into split level lc from lc+1, coarse nodes
@@ -93,14 +98,15 @@ __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc,
     if (ic[d] < (first ? 2 : 1) || ic[d] > nc[d] - 1) inner = false;
   }
   const int fi = 2 * ic[0] - 1, fj = 2 * ic[1] - 1, fk = 2 * ic[2] - 1;
-  const long p = g_idx(nf, fi, fj, fk), sy = nf[0] + 1, sz = (long)(nf[0] + 1) * (nf[1] + 1);
+  const long p = g_idx(nf, fi, fj, fk), J = nf[1] + 1;
+  const long sx = J, sy = 1 + J, sz = J * (1 + nf[0] + nf[1] + nf[2] + 1);
   double v = 0.0;
   bool write = true;
   if (copy) {
     v = uf[p];
   } else if (inner) {
     const double w = 1.0 / 12.0;
-    v = 0.5 * uf[p] + w * (uf[p + 1] + uf[p - 1] + uf[p + sy] + uf[p - sy] + uf[p + sz] + uf[p - sz]);
+    v = 0.5 * uf[p] + w * (uf[p + sx] + uf[p - sx] + uf[p + sy] + uf[p - sy] + uf[p + sz] + uf[p - sz]);
   } else {
     write = false;
   }
@@ -176,16 +182,16 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax(GravGeo G, do
   const double h2 = h * h;
   const int li = c[0] == 0 ? 1 : 0;
   const int m0 = n[0] - 1, m1 = n[1] - 1, m2 = n[2] - 1;
-  const long sy = n[0] + 1, sz = (long)(n[0] + 1) * (n[1] + 1);
+  const long J = n[1] + 1, sx = J, sy = 1 + J, sz = J * (1 + n[0] + n[1] + n[2] + 1);
   for (int t = 3; t <= m0 + m1 + m2; ++t) {
     const int klo = max(1, t - m0 - m1), khi = min(m2, t - 2);
     const int cnt = m1 * (khi - klo + 1);
     for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
       const int j = 1 + q % m1, k = klo + q / m1, i = t - j - k;
       if (i < 1 || i > m0) continue;
-      const long p = i + sy * j + sz * k;
+      const long p = g_idx(n, i, j, k);
       const int ifl = li * (1 / i);
-      const double s = u[p + 1] + u[p - 1] + u[p + sy] + u[p - sy] + u[p + sz] + u[p - sz] - h2 * rhs[p];
+      const double s = u[p + sx] + u[p - sx] + u[p + sy] + u[p - sy] + u[p + sz] + u[p - sz] - h2 * rhs[p];
       u[p] = w * s * (double)(1 - ifl) + (0.5 + copysign(0.5, (double)ifl - 0.5)) * u[p];
     }
     __syncthreads();
@@ -193,15 +199,19 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax(GravGeo G, do
 }
 
 // residMPI (SG8:918-952): nodes 1..n-1, then zero at the x-end nodes
+// (one thread per (j, i+j+k, k) slot, j fastest, so that neighbouring
+// threads touch neighbouring addresses of the skewed layout)
 __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const double *rhs2, int l) {
   const int *n = G.n[l];
-  const long cnt = (long)n[0] * n[1] * n[2], t = g_tid();
+  const int T = n[0] + n[1] + n[2] + 1;
+  const long cnt = (long)n[1] * T * n[2], t = g_tid();
   if (t >= cnt * G.npe) return;
   const int r = (int)(t / cnt);
   long q = t - r * cnt;
-  const int i = 1 + (int)(q % n[0]);
-  q /= n[0];
-  const int j = 1 + (int)(q % n[1]), k = 1 + (int)(q / n[1]);
+  const int j = 1 + (int)(q % n[1]);
+  q /= n[1];
+  const int tt = (int)(q % T), k = 1 + (int)(q / T), i = tt - j - k;
+  if (i < 1 || i > n[0]) return;
   int c[3];
   g_rank(G, r, c);
   const long base = G.off[l] + r * G.bsz[l], p = g_idx(n, i, j, k);
@@ -213,9 +223,9 @@ __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const 
   const double *u = phi2 + base;
   const double h = G.Lbox / (double)((n[0] - 1) * G.S[0]);
   const double h2i = 1.0 / (h * h);
-  const long sy = n[0] + 1, sz = (long)(n[0] + 1) * (n[1] + 1);
+  const long J = n[1] + 1, sx = J, sy = 1 + J, sz = J * (long)(1 + T);
   res2[base + p] =
-      -h2i * (u[p + 1] + u[p - 1] + u[p + sy] + u[p - sy] + u[p + sz] + u[p - sz] - 6.0 * u[p]) + rhs2[base + p];
+      -h2i * (u[p + sx] + u[p - sx] + u[p + sy] + u[p - sy] + u[p + sz] + u[p - sz] - 6.0 * u[p]) + rhs2[base + p];
 }
 
 // Prolongation of interpMPI / interp (SG8:500-599): injection at odd
@@ -291,7 +301,7 @@ __global__ void k_grav_collect(GravGeo G, double *u1, const double *u2) {
 __global__ void k_grav_divide(GravGeo G, double *u2, const double *u1) {
   const int L = G.NGcr, N1 = (1 << L) + 1;
   const int *m = G.n[L];
-  const long B = G.bsz[L], t = g_tid();
+  const long B = (long)(m[0] + 1) * (m[1] + 1) * (m[2] + 1), t = g_tid();
   if (t >= B * G.npe) return;
   const int r = (int)(t / B);
   long q = t - r * B;
@@ -307,7 +317,8 @@ __global__ void k_grav_divide(GravGeo G, double *u2, const double *u1) {
     if (c[d] == 0 && id[d] == 0) return;
     g3[d] = c[d] * (m[d] - 1) + id[d];
   }
-  u2[G.off[L] + t] = u1[G.p1[L] + (g3[0] - 1) + (long)N1 * ((g3[1] - 1) + (long)N1 * (g3[2] - 1))];
+  u2[G.off[L] + r * G.bsz[L] + g_idx(m, id[0], id[1], id[2])] =
+      u1[G.p1[L] + (g3[0] - 1) + (long)N1 * ((g3[1] - 1) + (long)N1 * (g3[2] - 1))];
 }
 
 // ---------------------------------------------------------------------------

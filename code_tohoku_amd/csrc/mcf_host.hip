@@ -924,7 +924,8 @@ static int grav_setup(mcf_ctx *x) {
     for (int d = 0; d < 3; ++d) G.n[l][d] = l == G.NGL ? G.C[d] + 1 : G.n[l + 1][d] / 2 + 1;
   }
   for (int l = G.NGcr; l <= G.NGL; ++l) {
-    G.bsz[l] = (long)(G.n[l][0] + 1) * (G.n[l][1] + 1) * (G.n[l][2] + 1);
+    // skewed block layout (k_grav_relax): (n1+1) x (n0+n1+n2+1) x (n2+1) slots
+    G.bsz[l] = (long)(G.n[l][1] + 1) * (G.n[l][0] + G.n[l][1] + G.n[l][2] + 1) * (G.n[l][2] + 1);
     G.off[l] = tot2;
     tot2 += G.bsz[l] * G.npe;
     G.boff[l] = totb;
@@ -1039,10 +1040,16 @@ static int grav_exchange(mcf_ctx *x, double *u2, int l, int flags) {
 }
 static const int XALL = 63, XLOW = 1 | 4 | 16;
 
+// per-level names of the relaxation launches (mcf_profile)
+static const char *const k_relax_names[GRAV_MAXL] = {
+    "k_grav_relax<0>", "k_grav_relax<1>", "k_grav_relax<2>", "k_grav_relax<3>", "k_grav_relax<4>", "k_grav_relax<5>",
+    "k_grav_relax<6>", "k_grav_relax<7>", "k_grav_relax<8>", "k_grav_relax<9>", "k_grav_relax<10>", "k_grav_relax<11>",
+    "k_grav_relax<12>", "k_grav_relax<13>", "k_grav_relax<14>", "k_grav_relax<15>"};
+
 static int grav_relax(mcf_ctx *x, int l, int times) {
   const GravGeo &G = x->gg;
   for (int t = 0; t < times; ++t) {
-    KLAUNCH_N("k_grav_relax", k_grav_relax, dim3(G.npe), dim3(GRAV_RELAX_THREADS), 16.0 * G.bsz[l] * G.npe, G,
+    KLAUNCH_N(k_relax_names[l], k_grav_relax, dim3(G.npe), dim3(GRAV_RELAX_THREADS), 16.0 * G.bsz[l] * G.npe, G,
               x->gb.phi2, x->gb.rhs2, l);
     CHK(grav_exchange(x, x->gb.phi2, l, XALL));
   }
@@ -1064,9 +1071,14 @@ static int do_grav_solve(mcf_ctx *x) {
   static const int l1[1] = {F_RHO};
   CHK(fillU(x, 7, 1, 1, l1, 1));
   CHK(fillU(x, 7, 2, 1, l1, 1));
-  CHK(grav_zero(x, x->gpool, x->gpool_n));
+  // every array starts the solve at zero, as in the reference's fixture
+  // build; the finest split level needs no clearing: what a solve reads
+  // there before writing it (node 0 of the first x rank) is never written
+  for (double *a : {B.phi2, B.rho2, B.res2, B.rhs2}) CHK(grav_zero(x, a, G.off[G.NGL]));
+  CHK(grav_zero(x, B.phi1, x->gpool_n - (B.phi1 - x->gpool)));
   // PB
   const int nn1 = P.nn1, nn2 = P.nn2, H = nn1 / 2, NGL = G.NGL, NGcr = G.NGcr;
+  (void)nn1;
   const long nZ = (long)P.np * nn2 * H;
   KLAUNCH_N("k_pb_pack", k_pb_pack, dim3(gblocks(nZ)), dim3(TPB), 16.0 * nZ, P, x->U[F_RHO], x->g, x->pbZ);
   const int t1 = std::min(512, std::max(64, H / 2)), t2 = std::min(512, std::max(64, nn2 / 2));
@@ -1095,7 +1107,7 @@ static int do_grav_solve(mcf_ctx *x) {
   }
   // mglin (SG8:151-295)
   {
-    const long nr = G.bsz[NGL] * G.npe;
+    const long nr = (long)(G.n[NGL][0] + 1) * (G.n[NGL][1] + 1) * (G.n[NGL][2] + 1) * G.npe;
     KLAUNCH_N("k_grav_rhs", k_grav_rhs, dim3(gblocks(nr)), dim3(TPB), 16.0 * nr, G, x->U[F_RHO], x->g, B.rhs2);
   }
   auto rstrct = [&](double *uc, const double *uf, int lc, int mode) -> int {
@@ -1113,7 +1125,7 @@ static int do_grav_solve(mcf_ctx *x) {
     return MCF_OK;
   };
   auto divide = [&]() {
-    const long nr = G.bsz[NGcr] * G.npe;
+    const long nr = (long)(G.n[NGcr][0] + 1) * (G.n[NGcr][1] + 1) * (G.n[NGcr][2] + 1) * G.npe;
     KLAUNCH_N("k_grav_divide", k_grav_divide, dim3(gblocks(nr)), dim3(TPB), 16.0 * nr, G, B.phi2, B.phi1);
     return MCF_OK;
   };
@@ -1148,8 +1160,10 @@ static int do_grav_solve(mcf_ctx *x) {
       for (int jj = j; jj >= NGcr; --jj) {
         CHK(grav_relax(x, jj, j >= NGL ? 2 : 5));
         {
-          const long nr = (long)G.n[jj][0] * G.n[jj][1] * G.n[jj][2] * G.npe;
-          KLAUNCH_N("k_grav_resid", k_grav_resid, dim3(gblocks(nr)), dim3(TPB), 24.0 * nr, G, B.res2, B.phi2, B.rhs2, jj);
+          const int *m = G.n[jj];
+          const long nr = (long)m[1] * (m[0] + m[1] + m[2] + 1) * m[2] * G.npe;
+          KLAUNCH_N("k_grav_resid", k_grav_resid, dim3(gblocks(nr)), dim3(TPB), 24.0 * nr / 3.0, G, B.res2, B.phi2,
+                    B.rhs2, jj);
           CHK(grav_exchange(x, B.res2, jj, XALL));
         }
         if (jj == NGcr) {

@@ -123,3 +123,38 @@ def test_gravity_mixed_precision():
     print("mixed", worst(errs), e)
     assert worst(errs) <= 1e-5, errs
     assert e <= 1e-5, e
+
+
+def test_gravity_converging_flow_256():
+    """The converging-flow configuration at its full size (256^3, x inflow,
+    Kolmogorov density) with self-gravity on the 8x8x8 split of the
+    reference's 512-rank runs (four whole-domain levels, the 17^3 one in the
+    solver's LDS workgroup): the initial solve and one step against the
+    oracle at 1e-12.  vy, vz and Bz stay near zero in this flow (max 6e-3,
+    5e-3, 6e-4 against 10 and 8 for vx and Bx), so the velocity and field
+    components are measured against the largest component of their vector;
+    per component the errors of one step are 3e-12, 2.5e-12 and 2.5e-12 (7e-13,
+    5e-13, 5e-13 without gravity), against the vector 2.6e-15 at most."""
+    p = ics.converging_flow_input(256)
+    p.ifgrv = 2
+    st = ics.converging_flow_state(p)
+    o = oracle.Oracle(p, st, nsplit=(8, 8, 8))
+    g = mcf.Domain(p, nsplit=(8, 8, 8))
+    g.upload(st)
+    e0 = rel(g.phi(), o.phi())
+    odts, _ = o.run(1)
+    dts = g.run(1)
+    np.testing.assert_allclose(dts, odts, rtol=1e-13, atol=0)
+    got, ref = interior(g.download(), p), interior(o.state(), p)
+    errs = {}
+    for f in range(16):
+        if f in (1, 2, 3, 5, 6, 7):
+            vec = (1, 2, 3) if f <= 3 else (5, 6, 7)
+            scale = max(np.abs(ref[c]).max() for c in vec)
+        else:
+            scale = np.abs(ref[f]).max()
+        errs[f] = float(np.abs(got[f] - ref[f]).max() / scale)
+    e1 = rel(g.phi(), o.phi())
+    print("256^3 8x8x8: initial phi", e0, "step: state", errs, "phi", e1)
+    assert e0 <= STEP_TOL and e1 <= STEP_TOL, (e0, e1)
+    assert max(errs.values()) <= STEP_TOL, errs
