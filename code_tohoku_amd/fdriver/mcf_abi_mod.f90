@@ -63,6 +63,17 @@ module mcf_abi_mod
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: col10(*)
     end function
+    integer(c_int) function mcf_gravty(ctx, dt, mode) bind(C, name='mcf_gravty')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function mcf_download_phi(ctx, phi) bind(C, name='mcf_download_phi')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: phi(*)
+    end function
     integer(c_int) function mcf_set_time(ctx, time, ifevo) bind(C, name='mcf_set_time')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -11,7 +11,8 @@
 ! tsave2D.DAT (M3:763-794).  The whole domain lives on the GPU; each physics
 ! call of the inner loop is one call through the C ABI (mcf_timestep for
 ! Couran+Stblty+reduction, mcf_step for SOURCE / MHD x3 / CT / SOURCE /
-! DISSIP).
+! DISSIP, with GRAVTY's kicks and solve when ifgrv = 2; the restart and
+! snapshot files then carry the potential Phi).
 !
 ! Environment: MCF_WALL_HOURS overrides the 11.7-hour wall-clock limit;
 ! MCF_COOLING=chem selects the reference's chemistry-network Fcool (default:
@@ -26,7 +27,7 @@ program mcf_evolve
 
   type(mcf_config) :: cfg
   type(c_ptr) :: ctx
-  real(c_double), allocatable, target :: U(:,:,:,:), SP(:,:,:,:), BC(:,:,:,:)
+  real(c_double), allocatable, target :: U(:,:,:,:), SP(:,:,:,:), BC(:,:,:,:), PH(:,:,:)
   character(len=512) :: rundir, arg
   integer :: npe, nsx, nsy, nsz, nx, ny, nz, nunit, nunit2D, st, itime, in10, in20, ifevo
   integer(c_int) :: rc, tag
@@ -97,6 +98,8 @@ program mcf_evolve
 10 close(3)
 
   allocate(U(-1:nx+2, -1:ny+2, -1:nz+2, 8), SP(-1:nx+2, -1:ny+2, -1:nz+2, 8), BC(-1:nx+2, -1:ny+2, -1:nz+2, 3))
+  allocate(PH(-1:nx+2, -1:ny+2, -1:nz+2))
+  PH = 0.d0
   U = 0.d0; SP = 0.d0
   if (nunit == 1) then
     call shock_tube_ic(cfg, U, SP, nx, ny, nz)
@@ -157,7 +160,8 @@ contains
     integer, intent(in) :: msig
     character(len=7), parameter :: stb(2) = ['FastSpd', 'Conduct']
     call mcf_check(mcf_download(ctx, U, SP), 'mcf_download')
-    call write_rank_files(trim(rundir), U, SP, nx, ny, nz, nsx, nsy, nsz, nunit)
+    if (cfg%ifgrv == 2) call mcf_check(mcf_download_phi(ctx, PH), 'mcf_download_phi')
+    call write_rank_files(trim(rundir), U, SP, PH, nx, ny, nz, nsx, nsy, nsz, nunit)
     t(nunit) = time
     open(3, file=trim(rundir)//'/time.DAT')
     do i = 1, nunit
@@ -179,7 +183,8 @@ contains
   subroutine saveu2d()
     call mcf_check(mcf_download(ctx, U, SP), 'mcf_download')
     call mcf_check(mcf_download_bcc(ctx, BC), 'mcf_download_bcc')
-    call write_rank_slices(trim(rundir), U, SP, BC, nx, ny, nz, nsx, nsy, nsz, nunit2D)
+    if (cfg%ifgrv == 2) call mcf_check(mcf_download_phi(ctx, PH), 'mcf_download_phi')
+    call write_rank_slices(trim(rundir), U, SP, BC, PH, nx, ny, nz, nsx, nsy, nsz, nunit2D)
     nunit2D = nunit2D + 1
     open(3, file=trim(rundir)//'/tsave2D.DAT')
     write(3, *) nunit2D
@@ -312,11 +317,13 @@ end subroutine read_restart
 
 
 ! Per-rank FP32 snapshot {nunit}{rank}.dat and FP64 restart 000{rank}.dat
-! (M3:701-717, 749-757); Phi is written as zero (self-gravity is off).
-subroutine write_rank_files(dir, U, SP, nx, ny, nz, sx, sy, sz, nunit)
+! (M3:701-717, 749-757); the 17th value is the potential Phi (zero without
+! self-gravity).
+subroutine write_rank_files(dir, U, SP, PH, nx, ny, nz, sx, sy, sz, nunit)
   character(*), intent(in) :: dir
   integer, intent(in) :: nx, ny, nz, sx, sy, sz, nunit
   double precision, intent(in) :: U(-1:nx+2, -1:ny+2, -1:nz+2, 8), SP(-1:nx+2, -1:ny+2, -1:nz+2, 8)
+  double precision, intent(in) :: PH(-1:nx+2, -1:ny+2, -1:nz+2)
   integer :: r, ist, jst, kst, cx, cy, cz, i, j, k, m, gi, gj, gk
   double precision, allocatable :: rec(:,:)
   character(len=3) :: rs, ns
@@ -337,7 +344,7 @@ subroutine write_rank_files(dir, U, SP, nx, ny, nz, sx, sy, sz, nunit)
             rec(m, i) = U(gi, gj, gk, m)
             rec(8 + m, i) = SP(gi, gj, gk, m)
           end do
-          rec(17, i) = 0.d0
+          rec(17, i) = PH(gi, gj, gk)
         end do
         write(10) real(rec, 4)
         write(8) rec
@@ -389,12 +396,12 @@ end subroutine write_cdnt
 
 ! Per-rank 2D slices 2D{nunit2D}{rank}.dat (M3:763-783): the rank's local
 ! k = 1 plane, one FP32 record of 17 values per cell (rho, v, p, Bcc,
-! species, Phi = 0) for i = 1..Ncellx, per j = 1..Ncelly
-subroutine write_rank_slices(dir, U, SP, BC, nx, ny, nz, sx, sy, sz, nunit2D)
+! species, Phi) for i = 1..Ncellx, per j = 1..Ncelly
+subroutine write_rank_slices(dir, U, SP, BC, PH, nx, ny, nz, sx, sy, sz, nunit2D)
   character(*), intent(in) :: dir
   integer, intent(in) :: nx, ny, nz, sx, sy, sz, nunit2D
   double precision, intent(in) :: U(-1:nx+2, -1:ny+2, -1:nz+2, 8), SP(-1:nx+2, -1:ny+2, -1:nz+2, 8)
-  double precision, intent(in) :: BC(-1:nx+2, -1:ny+2, -1:nz+2, 3)
+  double precision, intent(in) :: BC(-1:nx+2, -1:ny+2, -1:nz+2, 3), PH(-1:nx+2, -1:ny+2, -1:nz+2)
   integer :: r, ist, jst, kst, cx, cy, cz, i, j, m, gi, gj, gk
   real(4), allocatable :: rec(:,:)
   character(len=3) :: rs, ns
@@ -419,7 +426,7 @@ subroutine write_rank_slices(dir, U, SP, BC, nx, ny, nz, sx, sy, sz, nunit2D)
         do m = 1, 8
           rec(8 + m, i) = real(SP(gi, gj, gk, m), 4)
         end do
-        rec(17, i) = 0.0
+        rec(17, i) = real(PH(gi, gj, gk), 4)
       end do
       write(11) rec
     end do

@@ -110,3 +110,36 @@ def test_fortran_driver_chemistry(tmp_path):
     errs = rel_linf(interior(out, p), interior(expand(arr["out6"], p), p))
     print("fortran chem worst", worst(errs))
     assert worst(errs) <= 1e-11, errs
+
+
+def test_fortran_driver_gravity(tmp_path):
+    """The driver with self-gravity (ifgrv = 2 in INPUT3D.DAT) on the 32^3
+    8-rank gravity fixture: the potential Phi it writes as the restart
+    files' 17th value, and rho, v, p after 3 steps, against the reference."""
+    from test_oracle_grav import load
+    z, meta, p, st = load("grav_32_8")
+    nsplit = tuple(meta["nsplit"])
+    run = str(tmp_path)
+    q = rs.Input3D(**p.__dict__)
+    q.maxstp, q.nitera = 1, 3
+    with open(os.path.join(run, "INPUT3D.DAT"), "w") as f:
+        f.write(q.to_text())
+    rs.write_counters(run, 0.0, 2)
+    for r, b in enumerate(rs.split_state(st, nsplit)):
+        rs.write_restart_rank(os.path.join(run, f"000{r:03d}.dat"), b)
+    res = subprocess.run([build.EVOLVE, run, "8"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    n = p.n[0]
+    c = n // 2
+    blocks, phis = [], []
+    for r in range(8):
+        b, ph = rs.read_restart_rank(os.path.join(run, f"000{r:03d}.dat"), c, c, c)
+        blocks.append(b)
+        phis.append(np.broadcast_to(ph[None], (rs.NFIELD,) + ph.shape))
+    out = rs.assemble_state(blocks, p.n, nsplit)
+    phi = rs.assemble_state(phis, p.n, nsplit)[0][2:n + 2, 2:n + 2, 2:n + 2]
+    ref = z["phi3"]
+    assert np.abs(phi - ref).max() <= 1e-11 * np.abs(ref).max()
+    for t, f in enumerate(meta["state_fields"]):
+        a, b = out[f, 2:n + 2, 2:n + 2, 2:n + 2], z["state3"][t]
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max(), f
