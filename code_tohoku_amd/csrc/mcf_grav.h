@@ -46,6 +46,7 @@ struct GravBuf {
   double *phi1, *rho1, *res1, *rhs1; // whole-domain levels, need1+1 entries (1-based)
   double *bphi1;                     // 2 x (needb1+1), 1-based
   double *bphi2;
+  double *ex[2]; // boundary planes of consecutive sweeps (k_grav_relax_col), 6 per rank block
 };
 
 // PB (SG8:1026-1463): the x planes whose Fourier modes make the end-plane
@@ -67,6 +68,9 @@ __global__ void k_grav_rhs(GravGeo G, const double *rho, Geo g, double *rhs2);
 __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc, int mode);
 __global__ void k_grav_exchange(GravGeo G, double *u2, int l, int flags);
 __global__ void k_grav_relax(GravGeo G, double *phi2, const double *rhs2, int l);
+template <int PP, int LMAX>
+__global__ void k_grav_relax_col(GravGeo G, double *phi2, const double *rhs2, int l, const double *Ein, double *Eout,
+                                 long EP);
 __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const double *rhs2, int l);
 __global__ void k_grav_prolong(GravGeo G, double *uf2, const double *uc2, int lf, int mode, const double *bphi2);
 __global__ void k_grav_add(GravGeo G, double *uf2, const double *res2, int l);

@@ -40,12 +40,13 @@ __device__ __forceinline__ void g_rank(const GravGeo &G, int r, int c[3]) {
   c[1] = r / G.S[0] - G.S[1] * c[2];
 }
 __device__ __forceinline__ int g_rid(const GravGeo &G, int a, int b, int c) { return a + G.S[0] * (b + G.S[1] * c); }
-// Skewed block layout: node (i,j,k) at j + J ((i+j+k) + T k), J = n1+1,
-// T = n0+n1+n2+1, so that the nodes of one hyperplane i+j+k = t and one k
-// are contiguous in j; the six neighbours sit at +-J (x), +-(1+J) (y) and
-// +-J(1+T) (z).  Slots with i outside 0..n0 are never used.
+// Skewed block layout, hyperplane-major: node (i,j,k) at
+// j + J (k + K (i+j+k)), J = n1+1, K = n2+1, so that one hyperplane
+// i+j+k = t is one contiguous J*K slab, rows of j; the upper neighbours
+// sit at +JK (x), +1+JK (y), +J+JK (z), the lower ones at the negatives.
+// Slots with i outside 0..n0 are never used.
 __device__ __forceinline__ long g_idx(const int *n, int i, int j, int k) {
-  return j + (long)(n[1] + 1) * ((i + j + k) + (long)(n[0] + n[1] + n[2] + 1) * k);
+  return j + (long)(n[1] + 1) * (k + (long)(n[2] + 1) * (i + j + k));
 }
 __device__ __forceinline__ long g_tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 
@@ -98,8 +99,8 @@ __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc,
     if (ic[d] < (first ? 2 : 1) || ic[d] > nc[d] - 1) inner = false;
   }
   const int fi = 2 * ic[0] - 1, fj = 2 * ic[1] - 1, fk = 2 * ic[2] - 1;
-  const long p = g_idx(nf, fi, fj, fk), J = nf[1] + 1;
-  const long sx = J, sy = 1 + J, sz = J * (1 + nf[0] + nf[1] + nf[2] + 1);
+  const long p = g_idx(nf, fi, fj, fk), J = nf[1] + 1, JK = J * (nf[2] + 1);
+  const long sx = JK, sy = 1 + JK, sz = J + JK;
   double v = 0.0;
   bool write = true;
   if (copy) {
@@ -117,52 +118,64 @@ __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc,
   if (write) uc[g_idx(nc, ic[0], ic[1], ic[2])] = v;
 }
 
+// Where BCsgr_MPI's six phases take ghost node (i,j,k) of rank c from: walks
+// the phases backwards (z, y, x); returns false when no phase moves it.
+__device__ __forceinline__ bool g_resolve(const GravGeo &G, const int *n, int c[3], int &i, int &j, int &k, int flags) {
+  const int i0 = i, j0 = j, k0 = k;
+  if (k == 0 && (flags & 16)) { c[2] = (c[2] + G.S[2] - 1) % G.S[2]; k = n[2] - 1; }
+  else if (k == n[2] && (flags & 32)) { c[2] = (c[2] + 1) % G.S[2]; k = 1; }
+  if (j == 0 && (flags & 4)) { c[1] = (c[1] + G.S[1] - 1) % G.S[1]; j = n[1] - 1; }
+  else if (j == n[1] && (flags & 8)) { c[1] = (c[1] + 1) % G.S[1]; j = 1; }
+  if (i == 0 && (flags & 1) && c[0] > 0) { c[0] -= 1; i = n[0] - 1; }
+  else if (i == n[0] && (flags & 2) && c[0] < G.S[0] - 1) { c[0] += 1; i = 1; }
+  return i != i0 || j != j0 || k != k0;
+}
+
+// Ghost node q (0 <= q < ghosts per block) of a block: faces counted once,
+// z planes whole, y planes without their z rows, x planes without both.
+__device__ __forceinline__ long g_nghost(const int *n) {
+  return 2L * (n[0] + 1) * (n[1] + 1) + 2L * (n[0] + 1) * (n[2] - 1) + 2L * (n[1] - 1) * (n[2] - 1);
+}
+__device__ __forceinline__ void g_ghost(const int *n, int q, int &i, int &j, int &k) {
+  const int gz = 2 * (n[0] + 1) * (n[1] + 1), gy = 2 * (n[0] + 1) * (n[2] - 1);
+  if (q < gz) {
+    const int h = (n[0] + 1) * (n[1] + 1);
+    const int s = q / h, m = q - s * h;
+    i = m % (n[0] + 1);
+    j = m / (n[0] + 1);
+    k = s ? n[2] : 0;
+  } else if (q < gz + gy) {
+    q -= gz;
+    const int h = (n[0] + 1) * (n[2] - 1);
+    const int s = q / h, m = q - s * h;
+    i = m % (n[0] + 1);
+    k = 1 + m / (n[0] + 1);
+    j = s ? n[1] : 0;
+  } else {
+    q -= gz + gy;
+    const int h = (n[1] - 1) * (n[2] - 1);
+    const int s = q / h, m = q - s * h;
+    j = 1 + m % (n[1] - 1);
+    k = 1 + m / (n[1] - 1);
+    i = s ? n[0] : 0;
+  }
+}
+
 // BCsgr_MPI (SG8:298-336) at split level l.  flags: bit 0 x low, 1 x high,
 // 2 y low, 3 y high, 4 z low, 5 z high (the reference's lx1..lz2).  One
 // thread per ghost node (a node with a coordinate 0 or n); faces counted
 // once: z planes whole, y planes without their z rows, x planes without both.
 __global__ void k_grav_exchange(GravGeo G, double *u2, int l, int flags) {
   const int *n = G.n[l];
-  const long gz = 2L * (n[0] + 1) * (n[1] + 1), gy = 2L * (n[0] + 1) * (n[2] - 1), gx = 2L * (n[1] - 1) * (n[2] - 1);
-  const long per = gz + gy + gx, t = g_tid();
+  const long per = g_nghost(n), t = g_tid();
   if (t >= per * G.npe) return;
   const int r = (int)(t / per);
-  long q = t - r * per;
   int i, j, k;
-  if (q < gz) {
-    const long h = (long)(n[0] + 1) * (n[1] + 1);
-    const int s = (int)(q / h);
-    const long m = q - s * h;
-    i = (int)(m % (n[0] + 1));
-    j = (int)(m / (n[0] + 1));
-    k = s ? n[2] : 0;
-  } else if (q < gz + gy) {
-    q -= gz;
-    const long h = (long)(n[0] + 1) * (n[2] - 1);
-    const int s = (int)(q / h);
-    const long m = q - s * h;
-    i = (int)(m % (n[0] + 1));
-    k = 1 + (int)(m / (n[0] + 1));
-    j = s ? n[1] : 0;
-  } else {
-    q -= gz + gy;
-    const long h = (long)(n[1] - 1) * (n[2] - 1);
-    const int s = (int)(q / h);
-    const long m = q - s * h;
-    j = 1 + (int)(m % (n[1] - 1));
-    k = 1 + (int)(m / (n[1] - 1));
-    i = s ? n[0] : 0;
-  }
+  g_ghost(n, (int)(t - r * per), i, j, k);
   int c[3];
   g_rank(G, r, c);
   int si = i, sj = j, sk = k;
-  if (sk == 0 && (flags & 16)) { c[2] = (c[2] + G.S[2] - 1) % G.S[2]; sk = n[2] - 1; }
-  else if (sk == n[2] && (flags & 32)) { c[2] = (c[2] + 1) % G.S[2]; sk = 1; }
-  if (sj == 0 && (flags & 4)) { c[1] = (c[1] + G.S[1] - 1) % G.S[1]; sj = n[1] - 1; }
-  else if (sj == n[1] && (flags & 8)) { c[1] = (c[1] + 1) % G.S[1]; sj = 1; }
-  if (si == 0 && (flags & 1) && c[0] > 0) { c[0] -= 1; si = n[0] - 1; }
-  else if (si == n[0] && (flags & 2) && c[0] < G.S[0] - 1) { c[0] += 1; si = 1; }
-  if (si == i && sj == j && sk == k) return;
+  if (!g_resolve(G, n, c, si, sj, sk, flags)) return;
   const int rs = g_rid(G, c[0], c[1], c[2]);
   u2[G.off[l] + r * G.bsz[l] + g_idx(n, i, j, k)] = u2[G.off[l] + rs * G.bsz[l] + g_idx(n, si, sj, sk)];
 }
@@ -182,7 +195,7 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax(GravGeo G, do
   const double h2 = h * h;
   const int li = c[0] == 0 ? 1 : 0;
   const int m0 = n[0] - 1, m1 = n[1] - 1, m2 = n[2] - 1;
-  const long J = n[1] + 1, sx = J, sy = 1 + J, sz = J * (1 + n[0] + n[1] + n[2] + 1);
+  const long J = n[1] + 1, JK = J * (n[2] + 1), sx = JK, sy = 1 + JK, sz = J + JK;
   for (int t = 3; t <= m0 + m1 + m2; ++t) {
     const int klo = max(1, t - m0 - m1), khi = min(m2, t - 2);
     const int cnt = m1 * (khi - klo + 1);
@@ -198,8 +211,131 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax(GravGeo G, do
   }
 }
 
+// Barrier that orders LDS accesses only: the workgroup's threads exchange
+// values through LDS, and global loads still in flight (the next plane's
+// prefetch) need not complete first, as they would under __syncthreads.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The same sweep with each thread owning fixed (j,k) columns of the block
+// (PP of them) and walking i = t-j-k along them: the node's lower x
+// neighbour is the thread's own result of the previous plane (a register),
+// its lower y and z neighbours are the neighbouring columns' results of the
+// previous plane (LDS, two buffers by plane parity), and its upper
+// neighbours, own old value, right-hand side and the fixed ghost values are
+// loaded one plane ahead.  LMAX bounds (n1+1)(n2+1).
+//
+// Sweeps in a row exchange halos through boundary planes instead of a
+// separate launch: with Eout, every node the sweep computes on the six
+// planes next to the block's faces (node 1 or n-1 along an axis) is also
+// written there; with Ein, the ghost nodes are first set from the
+// neighbours' planes of the previous sweep, resolved as BCsgr_MPI would
+// resolve them (a source at a closed x end, never written, is read from the
+// neighbour's block itself).  Plane layout per block: 6 planes of EP values
+// (x planes indexed by (j,k), y by (i,k), z by (i,j)).
+__device__ __forceinline__ long g_plane_at(const int *n, long EP, int i, int j, int k) {
+  if (i == 1) return 0 * EP + j + (long)(n[1] + 1) * k;
+  if (i == n[0] - 1) return 1 * EP + j + (long)(n[1] + 1) * k;
+  if (j == 1) return 2 * EP + i + (long)(n[0] + 1) * k;
+  if (j == n[1] - 1) return 3 * EP + i + (long)(n[0] + 1) * k;
+  if (k == 1) return 4 * EP + i + (long)(n[0] + 1) * j;
+  return 5 * EP + i + (long)(n[0] + 1) * j;
+}
+
+template <int PP, int LMAX>
+__global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax_col(GravGeo G, double *phi2, const double *rhs2,
+                                                                       int l, const double *Ein, double *Eout, long EP) {
+  __shared__ double lds[2 * LMAX];
+  const int r = blockIdx.x;
+  const int *n = G.n[l];
+  int c[3];
+  g_rank(G, r, c);
+  double *u = phi2 + G.off[l] + r * G.bsz[l];
+  const double *rhs = rhs2 + G.off[l] + r * G.bsz[l];
+  if (Ein) {
+    const int ng = (int)g_nghost(n);
+    for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+      int i, j, k, cc[3] = {c[0], c[1], c[2]};
+      g_ghost(n, q, i, j, k);
+      int si = i, sj = j, sk = k;
+      if (!g_resolve(G, n, cc, si, sj, sk, 63)) continue;
+      const int rs = g_rid(G, cc[0], cc[1], cc[2]);
+      u[g_idx(n, i, j, k)] = (si == 0 || si == n[0]) ? phi2[G.off[l] + rs * G.bsz[l] + g_idx(n, si, sj, sk)]
+                                                     : Ein[rs * 6 * EP + g_plane_at(n, EP, si, sj, sk)];
+    }
+    __syncthreads();
+  }
+  const double w = 1.0 / 6.0;
+  const double h = G.Lbox / (double)((n[0] - 1) * G.S[0]);
+  const double h2 = h * h;
+  const int li = c[0] == 0 ? 1 : 0;
+  const int m0 = n[0] - 1, m1 = n[1] - 1, m2 = n[2] - 1, LJ = n[1] + 1;
+  const long J = n[1] + 1, JK = J * (n[2] + 1), sx = JK, sy = 1 + JK, sz = J + JK;
+  int pj[PP], pk[PP];
+  bool on[PP];
+  double fx[PP], fy[PP], fz[PP], fo[PP], fr[PP], gx[PP], gy[PP], gz[PP], prev[PP];
+#pragma unroll
+  for (int p = 0; p < PP; ++p) {
+    const int q = threadIdx.x + p * blockDim.x;
+    on[p] = q < m1 * m2;
+    pj[p] = on[p] ? 1 + q % m1 : 1;
+    pk[p] = on[p] ? 1 + q / m1 : 1;
+    fx[p] = fy[p] = fz[p] = fo[p] = fr[p] = gx[p] = gy[p] = gz[p] = prev[p] = 0.0;
+  }
+  auto fetch = [&](int p, int i) {
+    if (!on[p] || i < 1 || i > m0) return;
+    const long q = g_idx(n, i, pj[p], pk[p]);
+    fx[p] = u[q + sx];
+    fy[p] = u[q + sy];
+    fz[p] = u[q + sz];
+    fo[p] = u[q];
+    fr[p] = rhs[q];
+    if (i == 1) gx[p] = u[q - sx];
+    if (pj[p] == 1) gy[p] = u[q - sy];
+    if (pk[p] == 1) gz[p] = u[q - sz];
+  };
+#pragma unroll
+  for (int p = 0; p < PP; ++p) fetch(p, 3 - pj[p] - pk[p]);
+  for (int t = 3; t <= m0 + m1 + m2; ++t) {
+    double *Lc = lds + (t & 1) * LMAX;
+    const double *Lp = lds + ((t + 1) & 1) * LMAX;
+#pragma unroll
+    for (int p = 0; p < PP; ++p) {
+      const int i = t - pj[p] - pk[p];
+      if (on[p] && i >= 1 && i <= m0) {
+        const double xm = i == 1 ? gx[p] : prev[p];
+        const double ym = pj[p] == 1 ? gy[p] : Lp[(pj[p] - 1) + LJ * pk[p]];
+        const double zm = pk[p] == 1 ? gz[p] : Lp[pj[p] + LJ * (pk[p] - 1)];
+        const int ifl = li * (1 / i);
+        const double s = fx[p] + xm + fy[p] + ym + fz[p] + zm - h2 * fr[p];
+        const double v = w * s * (double)(1 - ifl) + (0.5 + copysign(0.5, (double)ifl - 0.5)) * fo[p];
+        u[g_idx(n, i, pj[p], pk[p])] = v;
+        Lc[pj[p] + LJ * pk[p]] = v;
+        prev[p] = v;
+        if (Eout) {
+          double *E = Eout + r * 6 * EP;
+          if (i == 1) E[0 * EP + pj[p] + LJ * pk[p]] = v;
+          if (i == m0) E[1 * EP + pj[p] + LJ * pk[p]] = v;
+          if (pj[p] == 1) E[2 * EP + i + (long)(n[0] + 1) * pk[p]] = v;
+          if (pj[p] == m1) E[3 * EP + i + (long)(n[0] + 1) * pk[p]] = v;
+          if (pk[p] == 1) E[4 * EP + i + (long)(n[0] + 1) * pj[p]] = v;
+          if (pk[p] == m2) E[5 * EP + i + (long)(n[0] + 1) * pj[p]] = v;
+        }
+      }
+      fetch(p, i + 1);
+    }
+    lds_barrier();
+  }
+}
+template __global__ void k_grav_relax_col<1, 1156>(GravGeo, double *, const double *, int, const double *, double *, long);
+template __global__ void k_grav_relax_col<2, 1156>(GravGeo, double *, const double *, int, const double *, double *, long);
+template __global__ void k_grav_relax_col<4, 4356>(GravGeo, double *, const double *, int, const double *, double *, long);
+
 // residMPI (SG8:918-952): nodes 1..n-1, then zero at the x-end nodes
-// (one thread per (j, i+j+k, k) slot, j fastest, so that neighbouring
+// (one thread per (j, k, i+j+k) slot, j fastest, so that neighbouring
 // threads touch neighbouring addresses of the skewed layout)
 __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const double *rhs2, int l) {
   const int *n = G.n[l];
@@ -210,7 +346,7 @@ __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const 
   long q = t - r * cnt;
   const int j = 1 + (int)(q % n[1]);
   q /= n[1];
-  const int tt = (int)(q % T), k = 1 + (int)(q / T), i = tt - j - k;
+  const int k = 1 + (int)(q % n[2]), tt = (int)(q / n[2]), i = tt - j - k;
   if (i < 1 || i > n[0]) return;
   int c[3];
   g_rank(G, r, c);
@@ -223,7 +359,7 @@ __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const 
   const double *u = phi2 + base;
   const double h = G.Lbox / (double)((n[0] - 1) * G.S[0]);
   const double h2i = 1.0 / (h * h);
-  const long J = n[1] + 1, sx = J, sy = 1 + J, sz = J * (long)(1 + T);
+  const long J = n[1] + 1, JK = J * (n[2] + 1), sx = JK, sy = 1 + JK, sz = J + JK;
   res2[base + p] =
       -h2i * (u[p + sx] + u[p - sx] + u[p + sy] + u[p - sy] + u[p + sz] + u[p - sz] - 6.0 * u[p]) + rhs2[base + p];
 }
@@ -518,11 +654,30 @@ __device__ __forceinline__ double plane_restrict(const double *bf, long kf, long
 
 __global__ __launch_bounds__(GRAV_COARSE_THREADS) void k_grav_coarse(GravGeo G, GravBuf B, int prog, int jst) {
   __shared__ double sphi[GRAV_NEED1_MAX + 1], srhs[GRAV_NEED1_MAX + 1], sres[GRAV_NEED1_MAX + 1];
+  // all loads of the three arrays in flight at once (GRAV_COARSE_THREADS
+  // threads, GRAV_NEED1_MAX + 1 entries each)
+  constexpr int CU = (GRAV_NEED1_MAX + GRAV_COARSE_THREADS) / GRAV_COARSE_THREADS;
   const long need = G.need1;
-  for (long p = threadIdx.x; p <= need; p += blockDim.x) {
-    sphi[p] = B.phi1[p];
-    srhs[p] = B.rhs1[p];
-    sres[p] = B.res1[p];
+  {
+    double a[CU], b[CU], c[CU];
+#pragma unroll
+    for (int q = 0; q < CU; ++q) {
+      const long p = threadIdx.x + (long)q * GRAV_COARSE_THREADS;
+      if (p <= need) {
+        a[q] = B.phi1[p];
+        b[q] = B.rhs1[p];
+        c[q] = B.res1[p];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CU; ++q) {
+      const long p = threadIdx.x + (long)q * GRAV_COARSE_THREADS;
+      if (p <= need) {
+        sphi[p] = a[q];
+        srhs[p] = b[q];
+        sres[p] = c[q];
+      }
+    }
   }
   __syncthreads();
   const Coarse S{G, B, sphi, srhs, sres};

@@ -977,7 +977,14 @@ static int grav_setup(mcf_ctx *x) {
   const long nz = 4 * ev(tot2) + 4 * n1 + nb1 + ev(totb); // cleared at every solve
   const long nZ = 2L * npl * nn2 * H, nS = 2L * npl * nn2, nD = 2L * 2 * nn2 * H, nSD = 2L * 2 * nn2;
   const long ntab = 4L * (q4 + 2);
-  const long total = nz + ev((long)x->ncell) + nZ + nS + nD + nSD + ntab + 4 * ev(npl) + 16;
+  // boundary planes of consecutive sweeps: 6 per rank block, two sets
+  long nex = 0;
+  for (int l = G.NGcr; l <= G.NGL; ++l) {
+    const int *m = G.n[l];
+    const long ep = std::max((long)(m[1] + 1) * (m[2] + 1), std::max((long)(m[0] + 1) * (m[2] + 1), (long)(m[0] + 1) * (m[1] + 1)));
+    nex = std::max(nex, 6L * G.npe * ep);
+  }
+  const long total = nz + ev((long)x->ncell) + nZ + nS + nD + nSD + ntab + 4 * ev(npl) + 2 * ev(nex) + 16;
   HIPCHK(hipMalloc(&x->gpool, total * sizeof(double)));
   double *p = x->gpool;
   auto take = [&](long c) { double *r = p; p += ev(c); return r; };
@@ -1003,6 +1010,8 @@ static int grav_setup(mcf_ctx *x) {
   double *dzq = take(npl);
   double *dfg = take(npl);
   int *dgx = (int *)take(npl);
+  x->gb.ex[0] = take(nex);
+  x->gb.ex[1] = take(nex);
   if (p > x->gpool + total) { g_err = "self-gravity: pool layout"; return MCF_E_ARG; }
   HIPCHK(hipMemsetAsync(x->gpool, 0, total * sizeof(double), x->stream));
   HIPCHK(hipMemcpyAsync(wf, tw[0].data(), (q4 + 2) * sizeof(double2), hipMemcpyHostToDevice, x->stream));
@@ -1046,14 +1055,48 @@ static const char *const k_relax_names[GRAV_MAXL] = {
     "k_grav_relax<6>", "k_grav_relax<7>", "k_grav_relax<8>", "k_grav_relax<9>", "k_grav_relax<10>", "k_grav_relax<11>",
     "k_grav_relax<12>", "k_grav_relax<13>", "k_grav_relax<14>", "k_grav_relax<15>"};
 
+// Gauss-Seidel sweeps at split level l: the column kernel (k_grav_relax_col)
+// when the block's (j,k) columns fit 1, 2 or 4 per thread of 1024 and its
+// y-z plane fits the kernel's LDS buffers, else the plane-by-plane kernel
 static int grav_relax(mcf_ctx *x, int l, int times) {
   const GravGeo &G = x->gg;
-  for (int t = 0; t < times; ++t) {
-    KLAUNCH_N(k_relax_names[l], k_grav_relax, dim3(G.npe), dim3(GRAV_RELAX_THREADS), 16.0 * G.bsz[l] * G.npe, G,
-              x->gb.phi2, x->gb.rhs2, l);
-    CHK(grav_exchange(x, x->gb.phi2, l, XALL));
+  const int *n = G.n[l];
+  const long cols = (long)(n[1] - 1) * (n[2] - 1), plane = (long)(n[1] + 1) * (n[2] + 1);
+  const double bytes = 8.0 * 8.0 * (double)(n[0] - 1) * cols * G.npe;
+  const unsigned tb = (unsigned)std::min<long>(GRAV_RELAX_THREADS, std::max<long>(64, (cols + 63) / 64 * 64));
+  const long ep = std::max(plane, std::max((long)(n[0] + 1) * (n[2] + 1), (long)(n[0] + 1) * (n[1] + 1)));
+  const int kind = (cols <= GRAV_RELAX_THREADS && plane <= 1156) ? 1
+                   : (cols <= 2 * GRAV_RELAX_THREADS && plane <= 1156) ? 2
+                   : (cols <= 4 * GRAV_RELAX_THREADS && plane <= 4356) ? 4 : 0;
+  if (kind == 0) {
+    for (int t = 0; t < times; ++t) {
+      KLAUNCH_N(k_relax_names[l], k_grav_relax, dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G, x->gb.phi2,
+                x->gb.rhs2, l);
+      CHK(grav_exchange(x, x->gb.phi2, l, XALL));
+    }
+    return MCF_OK;
   }
-  return MCF_OK;
+  // Blocks of at most 9 nodes per side pass the halo between the sweeps of
+  // the sequence through boundary planes and exchange once after the last
+  // one; larger blocks exchange after every sweep (256^3, 8x8x8: the planes
+  // saved 9 us of launch per sweep at 5^3-node blocks but cost 130 us per
+  // sweep at 33^3).
+  const bool planes = std::max(n[0], std::max(n[1], n[2])) <= 9;
+  for (int t = 0; t < times; ++t) {
+    const double *ein = planes && t > 0 ? x->gb.ex[(t - 1) & 1] : nullptr;
+    double *eout = planes && t < times - 1 ? x->gb.ex[t & 1] : nullptr;
+    if (!planes && t > 0) CHK(grav_exchange(x, x->gb.phi2, l, XALL));
+    if (kind == 1)
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156>), dim3(G.npe), dim3(tb), bytes, G, x->gb.phi2,
+                x->gb.rhs2, l, ein, eout, ep);
+    else if (kind == 2)
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<2, 1156>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
+                x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);
+    else
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<4, 4356>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
+                x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);
+  }
+  return grav_exchange(x, x->gb.phi2, l, XALL);
 }
 
 static int grav_zero(mcf_ctx *x, double *a, long n) {
