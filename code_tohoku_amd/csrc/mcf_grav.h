@@ -24,6 +24,8 @@ constexpr int GRAV_MAX_NGCR = 4;
 constexpr int GRAV_NEED1_MAX = 5795;
 constexpr int GRAV_COARSE_THREADS = 1024;
 constexpr int GRAV_RELAX_THREADS = 1024;
+// planes of loads in flight in k_grav_relax_col with 1, 2 and 4 columns per thread
+constexpr int GRAV_RELAX_D1 = 4, GRAV_RELAX_D2 = 2, GRAV_RELAX_D4 = 1;
 constexpr int GRAV_FFT_MAXL = 1024; // longest FFT line (nn1/2 and nn2)
 // 4 pi G (M3:35: G = 1.11142d-4, G4pi = 12.56637d0*G)
 constexpr double GRAV_G4PI = 12.56637 * 1.11142e-4;
@@ -68,7 +70,7 @@ __global__ void k_grav_rhs(GravGeo G, const double *rho, Geo g, double *rhs2);
 __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc, int mode);
 __global__ void k_grav_exchange(GravGeo G, double *u2, int l, int flags);
 __global__ void k_grav_relax(GravGeo G, double *phi2, const double *rhs2, int l);
-template <int PP, int LMAX>
+template <int PP, int LMAX, int D>
 __global__ void k_grav_relax_col(GravGeo G, double *phi2, const double *rhs2, int l, const double *Ein, double *Eout,
                                  long EP);
 __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const double *rhs2, int l);

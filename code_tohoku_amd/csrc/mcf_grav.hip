@@ -224,9 +224,17 @@ __device__ __forceinline__ void lds_barrier() {
 // (PP of them) and walking i = t-j-k along them: the node's lower x
 // neighbour is the thread's own result of the previous plane (a register),
 // its lower y and z neighbours are the neighbouring columns' results of the
-// previous plane (LDS, two buffers by plane parity), and its upper
-// neighbours, own old value, right-hand side and the fixed ghost values are
-// loaded one plane ahead.  LMAX bounds (n1+1)(n2+1).
+// previous plane (LDS, two buffers by plane parity) or, on the j = 0 and
+// k = 0 faces, fixed ghost values staged in LDS at the start.  Everything
+// else a node reads is a value from before the sweep -- its upper
+// neighbours, the right-hand side, and its own old value (the upper x
+// neighbour loaded one plane earlier) -- and is loaded D planes ahead into a
+// ring of registers, so that D planes of loads are in flight behind the
+// plane being computed: a node on plane t+D+1 is written only after the
+// barrier that ends plane t+D, by which time its old value has arrived.
+// Every load is issued, for a node off the column at an unused slot of the
+// same plane, so that the waits count the same loads on every path.  LMAX
+// bounds (n1+1)(n2+1), (n0+1)(n2+1) and (n0+1)(n1+1).
 //
 // Sweeps in a row exchange halos through boundary planes instead of a
 // separate launch: with Eout, every node the sweep computes on the six
@@ -245,10 +253,11 @@ __device__ __forceinline__ long g_plane_at(const int *n, long EP, int i, int j, 
   return 5 * EP + i + (long)(n[0] + 1) * j;
 }
 
-template <int PP, int LMAX>
+template <int PP, int LMAX, int D>
 __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax_col(GravGeo G, double *phi2, const double *rhs2,
                                                                        int l, const double *Ein, double *Eout, long EP) {
-  __shared__ double lds[2 * LMAX];
+  // two plane buffers, then the fixed ghost faces j = 0 and k = 0
+  __shared__ double lds[4 * LMAX];
   const int r = blockIdx.x;
   const int *n = G.n[l];
   int c[3];
@@ -272,67 +281,89 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax_col(GravGeo G
   const double h = G.Lbox / (double)((n[0] - 1) * G.S[0]);
   const double h2 = h * h;
   const int li = c[0] == 0 ? 1 : 0;
-  const int m0 = n[0] - 1, m1 = n[1] - 1, m2 = n[2] - 1, LJ = n[1] + 1;
+  const int m0 = n[0] - 1, m1 = n[1] - 1, m2 = n[2] - 1, LJ = n[1] + 1, T = m0 + m1 + m2, I1 = n[0] + 1;
   const long J = n[1] + 1, JK = J * (n[2] + 1), sx = JK, sy = 1 + JK, sz = J + JK;
+  double *gyf = lds + 2 * LMAX, *gzf = lds + 3 * LMAX;
+  for (int q = threadIdx.x; q < I1 * (n[2] + 1); q += blockDim.x) gyf[q] = u[g_idx(n, q % I1, 0, q / I1)];
+  for (int q = threadIdx.x; q < I1 * (n[1] + 1); q += blockDim.x) gzf[q] = u[g_idx(n, q % I1, q / I1, 0)];
+  __syncthreads();
   int pj[PP], pk[PP];
   bool on[PP];
-  double fx[PP], fy[PP], fz[PP], fo[PP], fr[PP], gx[PP], gy[PP], gz[PP], prev[PP];
+  // per column: the lower x neighbour (the ghost u(0,j,k), then the last
+  // result) and the old value of the node to compute (u(1,j,k) at first)
+  double prev[PP], fon[PP];
+  // ring slot s: the loads of the node on plane t = 3 + s (mod D)
+  double rx[D][PP], ry[D][PP], rz[D][PP], rr[D][PP];
 #pragma unroll
   for (int p = 0; p < PP; ++p) {
     const int q = threadIdx.x + p * blockDim.x;
     on[p] = q < m1 * m2;
     pj[p] = on[p] ? 1 + q % m1 : 1;
     pk[p] = on[p] ? 1 + q / m1 : 1;
-    fx[p] = fy[p] = fz[p] = fo[p] = fr[p] = gx[p] = gy[p] = gz[p] = prev[p] = 0.0;
+    const long q1 = g_idx(n, 1, pj[p], pk[p]);
+    prev[p] = u[q1 - sx];
+    fon[p] = u[q1];
   }
-  auto fetch = [&](int p, int i) {
-    if (!on[p] || i < 1 || i > m0) return;
-    const long q = g_idx(n, i, pj[p], pk[p]);
-    fx[p] = u[q + sx];
-    fy[p] = u[q + sy];
-    fz[p] = u[q + sz];
-    fo[p] = u[q];
-    fr[p] = rhs[q];
-    if (i == 1) gx[p] = u[q - sx];
-    if (pj[p] == 1) gy[p] = u[q - sy];
-    if (pk[p] == 1) gz[p] = u[q - sz];
+  // 32-bit byte offsets from the block's (uniform) base: one register per address
+  const unsigned bx = 8u * (unsigned)sx, by = 8u * (unsigned)sy, bz = 8u * (unsigned)sz;
+  auto at = [](const double *base, unsigned off) { return *(const double *)((const char *)base + off); };
+  auto fetch = [&](int s, int p, int i) {
+    // node i's slot on its plane, the plane held to 3..T: outside the
+    // column's nodes this is an unused slot of the skewed layout, on the same
+    // cache lines as the active columns' loads of that plane
+    const int tq = min(i + pj[p] + pk[p], T);
+    const unsigned q = 8u * (unsigned)(pj[p] + LJ * (pk[p] + (n[2] + 1) * tq));
+    rx[s][p] = at(u, q + bx);
+    ry[s][p] = at(u, q + by);
+    rz[s][p] = at(u, q + bz);
+    rr[s][p] = at(rhs, q);
   };
 #pragma unroll
-  for (int p = 0; p < PP; ++p) fetch(p, 3 - pj[p] - pk[p]);
-  for (int t = 3; t <= m0 + m1 + m2; ++t) {
-    double *Lc = lds + (t & 1) * LMAX;
-    const double *Lp = lds + ((t + 1) & 1) * LMAX;
+  for (int s = 0; s < D; ++s)
 #pragma unroll
-    for (int p = 0; p < PP; ++p) {
-      const int i = t - pj[p] - pk[p];
-      if (on[p] && i >= 1 && i <= m0) {
-        const double xm = i == 1 ? gx[p] : prev[p];
-        const double ym = pj[p] == 1 ? gy[p] : Lp[(pj[p] - 1) + LJ * pk[p]];
-        const double zm = pk[p] == 1 ? gz[p] : Lp[pj[p] + LJ * (pk[p] - 1)];
-        const int ifl = li * (1 / i);
-        const double s = fx[p] + xm + fy[p] + ym + fz[p] + zm - h2 * fr[p];
-        const double v = w * s * (double)(1 - ifl) + (0.5 + copysign(0.5, (double)ifl - 0.5)) * fo[p];
-        u[g_idx(n, i, pj[p], pk[p])] = v;
-        Lc[pj[p] + LJ * pk[p]] = v;
-        prev[p] = v;
-        if (Eout) {
-          double *E = Eout + r * 6 * EP;
-          if (i == 1) E[0 * EP + pj[p] + LJ * pk[p]] = v;
-          if (i == m0) E[1 * EP + pj[p] + LJ * pk[p]] = v;
-          if (pj[p] == 1) E[2 * EP + i + (long)(n[0] + 1) * pk[p]] = v;
-          if (pj[p] == m1) E[3 * EP + i + (long)(n[0] + 1) * pk[p]] = v;
-          if (pk[p] == 1) E[4 * EP + i + (long)(n[0] + 1) * pj[p]] = v;
-          if (pk[p] == m2) E[5 * EP + i + (long)(n[0] + 1) * pj[p]] = v;
+    for (int p = 0; p < PP; ++p) fetch(s, p, 3 + s - pj[p] - pk[p]);
+  for (int tb = 3; tb <= T; tb += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      const int t = tb + s;
+      if (t > T) break;
+      double *Lc = lds + (t & 1) * LMAX;
+      const double *Lp = lds + ((t + 1) & 1) * LMAX;
+#pragma unroll
+      for (int p = 0; p < PP; ++p) {
+        const int i = t - pj[p] - pk[p];
+        if (on[p] && i >= 1 && i <= m0) {
+          const double ym = pj[p] == 1 ? gyf[i + I1 * pk[p]] : Lp[(pj[p] - 1) + LJ * pk[p]];
+          const double zm = pk[p] == 1 ? gzf[i + I1 * pj[p]] : Lp[pj[p] + LJ * (pk[p] - 1)];
+          const int ifl = li * (1 / i);
+          const double sum = rx[s][p] + prev[p] + ry[s][p] + ym + rz[s][p] + zm - h2 * rr[s][p];
+          const double v = w * sum * (double)(1 - ifl) + (0.5 + copysign(0.5, (double)ifl - 0.5)) * fon[p];
+          u[g_idx(n, i, pj[p], pk[p])] = v;
+          Lc[pj[p] + LJ * pk[p]] = v;
+          prev[p] = v;
+          if (Eout) {
+            double *E = Eout + r * 6 * EP;
+            if (i == 1) E[0 * EP + pj[p] + LJ * pk[p]] = v;
+            if (i == m0) E[1 * EP + pj[p] + LJ * pk[p]] = v;
+            if (pj[p] == 1) E[2 * EP + i + (long)(n[0] + 1) * pk[p]] = v;
+            if (pj[p] == m1) E[3 * EP + i + (long)(n[0] + 1) * pk[p]] = v;
+            if (pk[p] == 1) E[4 * EP + i + (long)(n[0] + 1) * pj[p]] = v;
+            if (pk[p] == m2) E[5 * EP + i + (long)(n[0] + 1) * pj[p]] = v;
+          }
         }
+        if (i >= 1) fon[p] = rx[s][p];
+        fetch(s, p, i + D);
       }
-      fetch(p, i + 1);
+      lds_barrier();
     }
-    lds_barrier();
   }
 }
-template __global__ void k_grav_relax_col<1, 1156>(GravGeo, double *, const double *, int, const double *, double *, long);
-template __global__ void k_grav_relax_col<2, 1156>(GravGeo, double *, const double *, int, const double *, double *, long);
-template __global__ void k_grav_relax_col<4, 4356>(GravGeo, double *, const double *, int, const double *, double *, long);
+template __global__ void k_grav_relax_col<1, 1156, GRAV_RELAX_D1>(GravGeo, double *, const double *, int,
+                                                                  const double *, double *, long);
+template __global__ void k_grav_relax_col<2, 1156, GRAV_RELAX_D2>(GravGeo, double *, const double *, int,
+                                                                  const double *, double *, long);
+template __global__ void k_grav_relax_col<4, 4356, GRAV_RELAX_D4>(GravGeo, double *, const double *, int,
+                                                                  const double *, double *, long);
 
 // residMPI (SG8:918-952): nodes 1..n-1, then zero at the x-end nodes
 // (one thread per (j, k, i+j+k) slot, j fastest, so that neighbouring

@@ -1057,11 +1057,14 @@ static const char *const k_relax_names[GRAV_MAXL] = {
 
 // Gauss-Seidel sweeps at split level l: the column kernel (k_grav_relax_col)
 // when the block's (j,k) columns fit 1, 2 or 4 per thread of 1024 and its
-// y-z plane fits the kernel's LDS buffers, else the plane-by-plane kernel
+// y-z plane and ghost faces fit the kernel's LDS buffers, else the
+// plane-by-plane kernel
 static int grav_relax(mcf_ctx *x, int l, int times) {
   const GravGeo &G = x->gg;
   const int *n = G.n[l];
-  const long cols = (long)(n[1] - 1) * (n[2] - 1), plane = (long)(n[1] + 1) * (n[2] + 1);
+  const long cols = (long)(n[1] - 1) * (n[2] - 1);
+  const long plane = std::max((long)(n[1] + 1) * (n[2] + 1),
+                              (long)(n[0] + 1) * std::max(n[1] + 1, n[2] + 1)); // the LDS planes and ghost faces
   const double bytes = 8.0 * 8.0 * (double)(n[0] - 1) * cols * G.npe;
   const unsigned tb = (unsigned)std::min<long>(GRAV_RELAX_THREADS, std::max<long>(64, (cols + 63) / 64 * 64));
   const long ep = std::max(plane, std::max((long)(n[0] + 1) * (n[2] + 1), (long)(n[0] + 1) * (n[1] + 1)));
@@ -1087,13 +1090,13 @@ static int grav_relax(mcf_ctx *x, int l, int times) {
     double *eout = planes && t < times - 1 ? x->gb.ex[t & 1] : nullptr;
     if (!planes && t > 0) CHK(grav_exchange(x, x->gb.phi2, l, XALL));
     if (kind == 1)
-      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156>), dim3(G.npe), dim3(tb), bytes, G, x->gb.phi2,
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156, GRAV_RELAX_D1>), dim3(G.npe), dim3(tb), bytes, G, x->gb.phi2,
                 x->gb.rhs2, l, ein, eout, ep);
     else if (kind == 2)
-      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<2, 1156>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<2, 1156, GRAV_RELAX_D2>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
                 x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);
     else
-      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<4, 4356>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<4, 4356, GRAV_RELAX_D4>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
                 x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);
   }
   return grav_exchange(x, x->gb.phi2, l, XALL);
