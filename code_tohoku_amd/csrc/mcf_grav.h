@@ -51,6 +51,13 @@ struct GravBuf {
   double *ex[2]; // boundary planes of consecutive sweeps (k_grav_relax_col), 6 per rank block
 };
 
+// threads of an exchange per rank block at a split level of nodes 0..n
+// (k_grav_exchange): the two z planes walked as (i+j, j), the y planes
+// without their z rows, the x planes without both
+__host__ __device__ inline long grav_nghost(const int *n) {
+  return 2L * (n[0] + n[1] + 1) * (n[1] + 1) + 2L * (n[0] + 1) * (n[2] - 1) + 2L * (n[1] - 1) * (n[2] - 1);
+}
+
 // PB (SG8:1026-1463): the x planes whose Fourier modes make the end-plane
 // potential, with their distances from the two ends and 4 pi G dx factors
 struct PbArgs {

@@ -131,34 +131,37 @@ __device__ __forceinline__ bool g_resolve(const GravGeo &G, const int *n, int c[
   return i != i0 || j != j0 || k != k0;
 }
 
-// Ghost node q (0 <= q < ghosts per block) of a block: faces counted once,
-// z planes whole, y planes without their z rows, x planes without both.
-__device__ __forceinline__ long g_nghost(const int *n) {
-  return 2L * (n[0] + 1) * (n[1] + 1) + 2L * (n[0] + 1) * (n[2] - 1) + 2L * (n[1] - 1) * (n[2] - 1);
-}
-__device__ __forceinline__ void g_ghost(const int *n, int q, int &i, int &j, int &k) {
-  const int gz = 2 * (n[0] + 1) * (n[1] + 1), gy = 2 * (n[0] + 1) * (n[2] - 1);
+// Ghost node q (0 <= q < grav_nghost(n)) of a block: faces counted once,
+// z planes whole, y planes without their z rows, x planes without both.  A
+// z plane is walked as (i+j, j) with j fastest, so that neighbouring threads
+// touch neighbouring slots of the skewed layout (there and at the source);
+// the slots of that walk with i outside 0..n0 are not nodes (false).
+__device__ __forceinline__ bool g_ghost(const int *n, int q, int &i, int &j, int &k) {
+  const int gz = 2 * (n[0] + n[1] + 1) * (n[1] + 1), gy = 2 * (n[0] + 1) * (n[2] - 1);
   if (q < gz) {
-    const int h = (n[0] + 1) * (n[1] + 1);
+    const int h = (n[0] + n[1] + 1) * (n[1] + 1);
     const int s = q / h, m = q - s * h;
-    i = m % (n[0] + 1);
-    j = m / (n[0] + 1);
+    j = m % (n[1] + 1);
+    i = m / (n[1] + 1) - j;
     k = s ? n[2] : 0;
-  } else if (q < gz + gy) {
+    return i >= 0 && i <= n[0];
+  }
+  if (q < gz + gy) {
     q -= gz;
     const int h = (n[0] + 1) * (n[2] - 1);
     const int s = q / h, m = q - s * h;
     i = m % (n[0] + 1);
     k = 1 + m / (n[0] + 1);
     j = s ? n[1] : 0;
-  } else {
-    q -= gz + gy;
-    const int h = (n[1] - 1) * (n[2] - 1);
-    const int s = q / h, m = q - s * h;
-    j = 1 + m % (n[1] - 1);
-    k = 1 + m / (n[1] - 1);
-    i = s ? n[0] : 0;
+    return true;
   }
+  q -= gz + gy;
+  const int h = (n[1] - 1) * (n[2] - 1);
+  const int s = q / h, m = q - s * h;
+  j = 1 + m % (n[1] - 1);
+  k = 1 + m / (n[1] - 1);
+  i = s ? n[0] : 0;
+  return true;
 }
 
 // BCsgr_MPI (SG8:298-336) at split level l.  flags: bit 0 x low, 1 x high,
@@ -167,11 +170,11 @@ __device__ __forceinline__ void g_ghost(const int *n, int q, int &i, int &j, int
 // once: z planes whole, y planes without their z rows, x planes without both.
 __global__ void k_grav_exchange(GravGeo G, double *u2, int l, int flags) {
   const int *n = G.n[l];
-  const long per = g_nghost(n), t = g_tid();
+  const long per = grav_nghost(n), t = g_tid();
   if (t >= per * G.npe) return;
   const int r = (int)(t / per);
   int i, j, k;
-  g_ghost(n, (int)(t - r * per), i, j, k);
+  if (!g_ghost(n, (int)(t - r * per), i, j, k)) return;
   int c[3];
   g_rank(G, r, c);
   int si = i, sj = j, sk = k;
@@ -265,10 +268,10 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax_col(GravGeo G
   double *u = phi2 + G.off[l] + r * G.bsz[l];
   const double *rhs = rhs2 + G.off[l] + r * G.bsz[l];
   if (Ein) {
-    const int ng = (int)g_nghost(n);
+    const int ng = (int)grav_nghost(n);
     for (int q = threadIdx.x; q < ng; q += blockDim.x) {
       int i, j, k, cc[3] = {c[0], c[1], c[2]};
-      g_ghost(n, q, i, j, k);
+      if (!g_ghost(n, q, i, j, k)) continue;
       int si = i, sj = j, sk = k;
       if (!g_resolve(G, n, cc, si, sj, sk, 63)) continue;
       const int rs = g_rid(G, cc[0], cc[1], cc[2]);

@@ -1039,12 +1039,19 @@ static int grav_setup(mcf_ctx *x) {
 
 static unsigned gblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// per-level names of the exchange launches (mcf_profile)
+static const char *const k_exchange_names[GRAV_MAXL] = {
+    "k_grav_exchange<0>", "k_grav_exchange<1>", "k_grav_exchange<2>", "k_grav_exchange<3>",
+    "k_grav_exchange<4>", "k_grav_exchange<5>", "k_grav_exchange<6>", "k_grav_exchange<7>",
+    "k_grav_exchange<8>", "k_grav_exchange<9>", "k_grav_exchange<10>", "k_grav_exchange<11>",
+    "k_grav_exchange<12>", "k_grav_exchange<13>", "k_grav_exchange<14>", "k_grav_exchange<15>"};
+
 // one exchange of split level l (flags: x low, x high, y low, y high, z low, z high)
 static int grav_exchange(mcf_ctx *x, double *u2, int l, int flags) {
   const GravGeo &G = x->gg;
   const int *n = G.n[l];
-  const long per = 2L * (n[0] + 1) * (n[1] + 1) + 2L * (n[0] + 1) * (n[2] - 1) + 2L * (n[1] - 1) * (n[2] - 1);
-  KLAUNCH_N("k_grav_exchange", k_grav_exchange, dim3(gblocks(per * G.npe)), dim3(TPB), 16.0 * per * G.npe, G, u2, l, flags);
+  const long per = grav_nghost(n);
+  KLAUNCH_N(k_exchange_names[l], k_grav_exchange, dim3(gblocks(per * G.npe)), dim3(TPB), 16.0 * per * G.npe, G, u2, l, flags);
   return MCF_OK;
 }
 static const int XALL = 63, XLOW = 1 | 4 | 16;
