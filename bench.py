@@ -334,6 +334,19 @@ def grav_compare(n, steps, warmup):
                     "kernels": {k: {"avg_us": round(1e3 * v[1] / v[0], 2), "launches_per_step": v[0] / nprof,
                                     "ms_per_step": round(v[1] / nprof, 3)} for k, v in top},
                     "dt_mean": float(np.mean(dts))}
+        # the finest level's Gauss-Seidel sweep: compulsory bytes (old phi
+        # and rhs read, new phi written: 24 B per node, n^3 nodes) against
+        # HBM peak, and the time per hyperplane of the block's march (3C-5
+        # planes for C cells per block and axis), the latency that bounds it
+        fin = rep.get(f"k_grav_relax<{int(round(np.log2(n)))}>")
+        if fin:
+            avg = fin[1] / fin[0] * 1e-3
+            cmax = n // min(nsplit)
+            out[tag]["relax_finest"] = {"avg_us": round(avg * 1e6, 2), "compulsory_bytes": 24 * n ** 3,
+                                        "achieved_GBps": round(24 * n ** 3 / avg / 1e9, 1),
+                                        "frac_of_hbm_peak": round(24 * n ** 3 / avg / 1e9 / HBM_PEAK_GBS, 4),
+                                        "planes_per_sweep": 3 * cmax - 5,
+                                        "us_per_plane": round(avg * 1e6 / (3 * cmax - 5), 3)}
         log(f"[bench] gravity {tag}: {out[tag]['ms_per_step']:.3f} ms/step, solve "
             f"{out[tag]['grav_solve_ms_per_step']:.3f} ms in {glaunch:.0f} launches")
     return out
