@@ -24,8 +24,8 @@ constexpr int GRAV_MAX_NGCR = 4;
 constexpr int GRAV_NEED1_MAX = 5795;
 constexpr int GRAV_COARSE_THREADS = 1024;
 constexpr int GRAV_RELAX_THREADS = 1024;
-// planes of loads in flight in k_grav_relax_col with 1, 2 and 4 columns per thread
-constexpr int GRAV_RELAX_D1 = 4, GRAV_RELAX_D2 = 2, GRAV_RELAX_D4 = 1;
+// planes of loads in flight in k_grav_relax_col with 1 and 4 columns per thread
+constexpr int GRAV_RELAX_D1 = 4, GRAV_RELAX_D4 = 1;
 constexpr int GRAV_FFT_MAXL = 1024; // longest FFT line (nn1/2 and nn2)
 // 4 pi G (M3:35: G = 1.11142d-4, G4pi = 12.56637d0*G)
 constexpr double GRAV_G4PI = 12.56637 * 1.11142e-4;

@@ -363,8 +363,6 @@ __global__ __launch_bounds__(GRAV_RELAX_THREADS) void k_grav_relax_col(GravGeo G
 }
 template __global__ void k_grav_relax_col<1, 1156, GRAV_RELAX_D1>(GravGeo, double *, const double *, int,
                                                                   const double *, double *, long);
-template __global__ void k_grav_relax_col<2, 1156, GRAV_RELAX_D2>(GravGeo, double *, const double *, int,
-                                                                  const double *, double *, long);
 template __global__ void k_grav_relax_col<4, 4356, GRAV_RELAX_D4>(GravGeo, double *, const double *, int,
                                                                   const double *, double *, long);
 

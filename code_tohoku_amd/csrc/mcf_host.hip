@@ -1063,7 +1063,7 @@ static const char *const k_relax_names[GRAV_MAXL] = {
     "k_grav_relax<12>", "k_grav_relax<13>", "k_grav_relax<14>", "k_grav_relax<15>"};
 
 // Gauss-Seidel sweeps at split level l: the column kernel (k_grav_relax_col)
-// when the block's (j,k) columns fit 1, 2 or 4 per thread of 1024 and its
+// when the block's (j,k) columns fit 1 or 4 per thread of 1024 and its
 // y-z plane and ghost faces fit the kernel's LDS buffers, else the
 // plane-by-plane kernel
 static int grav_relax(mcf_ctx *x, int l, int times) {
@@ -1076,7 +1076,6 @@ static int grav_relax(mcf_ctx *x, int l, int times) {
   const unsigned tb = (unsigned)std::min<long>(GRAV_RELAX_THREADS, std::max<long>(64, (cols + 63) / 64 * 64));
   const long ep = std::max(plane, std::max((long)(n[0] + 1) * (n[2] + 1), (long)(n[0] + 1) * (n[1] + 1)));
   const int kind = (cols <= GRAV_RELAX_THREADS && plane <= 1156) ? 1
-                   : (cols <= 2 * GRAV_RELAX_THREADS && plane <= 1156) ? 2
                    : (cols <= 4 * GRAV_RELAX_THREADS && plane <= 4356) ? 4 : 0;
   if (kind == 0) {
     for (int t = 0; t < times; ++t) {
@@ -1099,9 +1098,6 @@ static int grav_relax(mcf_ctx *x, int l, int times) {
     if (kind == 1)
       KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156, GRAV_RELAX_D1>), dim3(G.npe), dim3(tb), bytes, G, x->gb.phi2,
                 x->gb.rhs2, l, ein, eout, ep);
-    else if (kind == 2)
-      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<2, 1156, GRAV_RELAX_D2>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
-                x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);
     else
       KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<4, 4356, GRAV_RELAX_D4>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
                 x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);

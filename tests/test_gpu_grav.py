@@ -86,11 +86,15 @@ def test_gravity_pieces_vs_oracle():
 
 
 @pytest.mark.parametrize("n,nsplit,steps", [(64, (4, 2, 2), 2), (64, (2, 4, 2), 1), (128, (2, 2, 2), 1),
-                                            (32, (2, 2, 1), 2)])
+                                            (32, (2, 2, 1), 2), (128, (2, 1, 1), 1)])
 def test_gravity_vs_oracle(n, nsplit, steps):
     """Splits with one, two and three whole-domain levels below the split
     ones (NGcr = 2, 3) and more split levels (128^3): state and Phi after
-    whole steps."""
+    whole steps.  The relaxation kernel depends on the block: one column per
+    thread up to 33^2 nodes per y-z plane (most cases), four per thread up to
+    65^2 (128^3 on 2x2x2; 128^3 on 2x1x1 below its finest level), and the
+    plane-by-plane kernel beyond (128^3 on 2x1x1: blocks of 65 x 129 x 129
+    nodes at the finest level)."""
     p = cube(n)
     st = ics.random_state(p, seed=23)
     o = oracle.Oracle(p, st, nsplit=nsplit)
