@@ -543,20 +543,28 @@ __device__ void c_interp(const Coarse &S, double *uf, long offf, const double *u
 }
 
 // relax (SG8:873-915): x nodes 2..n-1, y and z wrapped with period n
+// Hyperplanes i+j+k = t; each thread keeps one (j,k) column for the whole
+// sweep (n^2 <= 289 columns, fewer than the workgroup's threads), with its
+// wrapped neighbours' offsets formed once.
+static_assert(((1 << GRAV_MAX_NGCR) + 1) * ((1 << GRAV_MAX_NGCR) + 1) <= GRAV_COARSE_THREADS,
+              "one thread per column of the whole-domain levels");
 __device__ void c_relax(const Coarse &S, double *u, long off, const double *rhs, int n) {
   const double w = 1.0 / 6.0;
   const double h = S.G.Lbox / (double)(n - 1);
   const double h2 = h * h;
-  const long nn = (long)n * n;
+  const int q = threadIdx.x;
+  const bool on = q < n * n;
+  const int j = 1 + q % n, k = 1 + q / n;
+  const int jm = j == 1 ? n : j - 1, jp = j == n ? 1 : j + 1;
+  const int km = k == 1 ? n : k - 1, kp = k == n ? 1 : k + 1;
+  const long base = c_at(off, n, 0, j, k);
+  const long oyp = (long)n * (jp - j), oym = (long)n * (jm - j), ozp = (long)n * n * (kp - k),
+             ozm = (long)n * n * (km - k);
   for (int t = 4; t <= (n - 1) + 2 * n; ++t) {
-    for (long q = threadIdx.x; q < nn; q += blockDim.x) {
-      const int j = 1 + (int)(q % n), k = 1 + (int)(q / n), i = t - j - k;
-      if (i < 2 || i > n - 1) continue;
-      const int jm = j == 1 ? n : j - 1, jp = j == n ? 1 : j + 1;
-      const int km = k == 1 ? n : k - 1, kp = k == n ? 1 : k + 1;
-      const long p = c_at(off, n, i, j, k);
-      u[p] = w * (u[p + 1] + u[p - 1] + u[c_at(off, n, i, jp, k)] + u[c_at(off, n, i, jm, k)] + u[c_at(off, n, i, j, kp)] +
-                  u[c_at(off, n, i, j, km)] - h2 * rhs[p]);
+    const int i = t - j - k;
+    if (on && i >= 2 && i <= n - 1) {
+      const long p = base + i;
+      u[p] = w * (u[p + 1] + u[p - 1] + u[p + oyp] + u[p + oym] + u[p + ozp] + u[p + ozm] - h2 * rhs[p]);
     }
     __syncthreads();
   }
