@@ -77,7 +77,7 @@ __global__ void k_grav_rhs(GravGeo G, const double *rho, Geo g, double *rhs2);
 __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc, int mode);
 __global__ void k_grav_exchange(GravGeo G, double *u2, int l, int flags);
 __global__ void k_grav_relax(GravGeo G, double *phi2, const double *rhs2, int l);
-template <int PP, int LMAX, int D>
+template <int PP, int LMAX, int D, bool PL>
 __global__ void k_grav_relax_col(GravGeo G, double *phi2, const double *rhs2, int l, const double *Ein, double *Eout,
                                  long EP);
 __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const double *rhs2, int l);

@@ -1095,12 +1095,15 @@ static int grav_relax(mcf_ctx *x, int l, int times) {
     const double *ein = planes && t > 0 ? x->gb.ex[(t - 1) & 1] : nullptr;
     double *eout = planes && t < times - 1 ? x->gb.ex[t & 1] : nullptr;
     if (!planes && t > 0) CHK(grav_exchange(x, x->gb.phi2, l, XALL));
-    if (kind == 1)
-      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156, GRAV_RELAX_D1>), dim3(G.npe), dim3(tb), bytes, G, x->gb.phi2,
-                x->gb.rhs2, l, ein, eout, ep);
-    else
-      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<4, 4356, GRAV_RELAX_D4>), dim3(G.npe), dim3(GRAV_RELAX_THREADS), bytes, G,
+    if (kind == 1 && planes)
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156, GRAV_RELAX_D1, true>), dim3(G.npe), dim3(tb), bytes, G,
                 x->gb.phi2, x->gb.rhs2, l, ein, eout, ep);
+    else if (kind == 1)
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<1, 1156, GRAV_RELAX_D1, false>), dim3(G.npe), dim3(tb), bytes, G,
+                x->gb.phi2, x->gb.rhs2, l, nullptr, nullptr, ep);
+    else
+      KLAUNCH_N(k_relax_names[l], (k_grav_relax_col<4, 4356, GRAV_RELAX_D4, true>), dim3(G.npe),
+                dim3(GRAV_RELAX_THREADS), bytes, G, x->gb.phi2, x->gb.rhs2, l, nullptr, nullptr, ep);
   }
   return grav_exchange(x, x->gb.phi2, l, XALL);
 }
