@@ -31,7 +31,9 @@ SHIELD column densities and chemistry-network Fcool switched on.
 
 ``gravity`` times the same problem with self-gravity (ifgrv = 2): the
 multigrid Poisson solve once per step and two velocity kicks, on the 4x4x4
-split and on the 8x8x8 split of the reference's 512-rank runs.
+split and on the 8x8x8 split of the reference's 512-rank runs; with the CPU
+baseline on, the oracle's solve of the same density on 8x8x8 is timed on
+the host beside it.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pmc] [--no-stiff] [--no-chem]
                        [--no-grav]
@@ -227,6 +229,25 @@ def cpu_baseline(n: int, steps: int):
     return {"value": n ** 3 * len(dts) / el, "unit": "zone-updates/s", "cores": o.threads, "kind": "port",
             "sample": f"CPU restatement (oracle/, C + OpenMP, {o.threads} threads), {len(dts)} FP64 steps of the "
                       f"same converging-flow problem at {n}^3, {el:.1f} s"}
+
+
+def cpu_grav(n, nsplit):
+    """The oracle's GRAVTY(dt,2) (one multigrid Poisson solve with the FFT
+    boundary potential) on the host, for the gravity leg: the same 256^3
+    converging-flow density on the same block split."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    p, st = build_state(n)
+    p.ifgrv = 2
+    o = oracle.Oracle(p, st, nsplit=nsplit)
+    del st
+    t0 = time.perf_counter()
+    o.grav_solve()
+    el = time.perf_counter() - t0
+    return {"solve_ms": round(1e3 * el, 1), "threads": o.threads, "kind": "port",
+            "sample": f"CPU restatement (oracle/mcf_grav.c, C + OpenMP, {o.threads} threads), one GRAVTY(dt,2) "
+                      f"solve at {n}^3 on the {'x'.join(map(str, nsplit))} split"}
 
 
 def stiff_state(n):
@@ -534,6 +555,13 @@ def main():
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle {args.cpu_steps} steps at {args.cpu_n}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
+        if "gravity" in out:
+            log(f"[bench] CPU baseline: oracle gravity solve at {n}^3")
+            cg = cpu_grav(n, (8, 8, 8))
+            gs = out["gravity"]["mixed_8x8x8"]["grav_solve_ms_per_step"]
+            cg["gpu_solve_ms"] = gs
+            cg["gpu_speedup"] = round(cg["solve_ms"] / gs, 1) if gs else None
+            out["gravity"]["cpu_baseline"] = cg
     print(json.dumps(out), flush=True)
 
 
