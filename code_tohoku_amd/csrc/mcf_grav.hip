@@ -449,9 +449,16 @@ __global__ void k_grav_prolong(GravGeo G, double *uf2, const double *uc2, int lf
 }
 
 // addintMPI's update (SG8:602-624): every node 0..n, uf += res
+// (over the slots of the skewed layout, the unused ones skipped: a third of
+// the slots are nodes)
 __global__ void k_grav_add(GravGeo G, double *uf2, const double *res2, int l) {
-  const long n = G.bsz[l] * G.npe, t = g_tid();
-  if (t >= n) return;
+  const int *n = G.n[l];
+  const long n_ = G.bsz[l] * G.npe, t = g_tid();
+  if (t >= n_) return;
+  const int J = n[1] + 1, K = n[2] + 1;
+  const int s = (int)(t % G.bsz[l]);
+  const int j = s % J, k = (s / J) % K, i = s / (J * K) - j - k;
+  if (i < 0 || i > n[0]) return;
   uf2[G.off[l] + t] = uf2[G.off[l] + t] + res2[G.off[l] + t];
 }
 
