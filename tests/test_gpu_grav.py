@@ -111,6 +111,27 @@ def test_gravity_vs_oracle(n, nsplit, steps):
     assert e <= tol, e
 
 
+def test_gravity_with_chemistry():
+    """Self-gravity beside the chemistry network, SHIELD and the chemistry
+    Fcool (ifchem = ifrad = 2, ifgrv = 2): state and Phi after 2 steps at
+    32^3 on 8 blocks."""
+    p = cube(32)
+    p.ifchem, p.ifrad = 2, 2
+    st = ics.random_state(p, seed=27)
+    o = oracle.Oracle(p, st, nsplit=(2, 2, 2), cooling="chem")
+    odts, _ = o.run(2)
+    g = mcf.Domain(p, nsplit=(2, 2, 2), cooling=mcf.COOL_CHEM)
+    g.upload(st)
+    dts = g.run(2)
+    np.testing.assert_allclose(dts, odts, rtol=1e-13, atol=0)
+    errs = rel_linf(interior(g.download(), p), interior(o.state(), p))
+    e = rel(g.phi(), o.phi())
+    print("chemistry + gravity", worst(errs), e)
+    assert worst(errs) <= MULTI_TOL, errs
+    assert e <= MULTI_TOL, e
+    g.close()
+
+
 def test_gravity_mixed_precision():
     """The mixed-precision MHD path with FP64 gravity: within 1e-5 of the
     oracle after 2 steps at 32^3."""
