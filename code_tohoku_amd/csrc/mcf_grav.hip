@@ -50,6 +50,46 @@ __device__ __forceinline__ long g_idx(const int *n, int i, int j, int k) {
 }
 __device__ __forceinline__ long g_tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 
+// Node q (0 <= q < A B) of an A x B rectangle of nodes (a, b) in
+// anti-diagonal order: a+b ascending, then b.  With a = i-1, b = j-1 of one
+// k plane, neighbouring q are neighbouring slots of the skewed layout (j up,
+// i down, i+j+k fixed).  The diagonals' lengths rise 1..m, stay at m, fall
+// m-1..1 (m = min(A, B)); the rising and falling parts invert a triangular
+// number, the float square root corrected to the exact integer.
+__device__ __forceinline__ int g_tri(int q) { // largest s with s(s+1)/2 <= q
+  int s = (int)((__builtin_amdgcn_sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+  while (s * (s + 1) / 2 > q) --s;
+  while ((s + 1) * (s + 2) / 2 <= q) ++s;
+  return s;
+}
+__device__ __forceinline__ void g_diag(int A, int B, int q, int &a, int &b) {
+  const int m = min(A, B), M = max(A, B), T1 = m * (m + 1) / 2, F = (M - m) * m;
+  int s, o;
+  if (q < T1) {
+    s = g_tri(q);
+    o = q - s * (s + 1) / 2;
+  } else if (q < T1 + F) {
+    s = m + (q - T1) / m;
+    o = (q - T1) % m;
+  } else {
+    const int r = A * B - 1 - q, u = g_tri(r);
+    s = A + B - 2 - u;
+    o = u - (r - u * (u + 1) / 2);
+  }
+  b = max(0, s - (A - 1)) + o;
+  a = s - b;
+}
+// node q of a block's nodes 1..n0 x 1..n1 x 1..n2: k planes, each in
+// anti-diagonal order
+__device__ __forceinline__ void g_node(const int *n, long q, int &i, int &j, int &k) {
+  const long AB = (long)n[0] * n[1];
+  k = 1 + (int)(q / AB);
+  int a, b;
+  g_diag(n[0], n[1], (int)(q - (k - 1) * AB), a, b);
+  i = 1 + a;
+  j = 1 + b;
+}
+
 __global__ void k_grav_zero(double *a, long n) {
   for (long t = g_tid(); t < n; t += (long)gridDim.x * blockDim.x) a[t] = 0.0;
 }
@@ -82,12 +122,8 @@ __global__ void k_grav_rstrct(GravGeo G, double *uc2, const double *uf2, int lc,
   const long cnt = (long)nc[0] * nc[1] * nc[2], t = g_tid();
   if (t >= cnt * G.npe) return;
   const int r = (int)(t / cnt);
-  long q = t - r * cnt;
   int ic[3];
-  ic[0] = 1 + (int)(q % nc[0]);
-  q /= nc[0];
-  ic[1] = 1 + (int)(q % nc[1]);
-  ic[2] = 1 + (int)(q / nc[1]);
+  g_node(nc, t - r * cnt, ic[0], ic[1], ic[2]);
   int c[3];
   g_rank(G, r, c);
   const double *uf = uf2 + G.off[lc + 1] + r * G.bsz[lc + 1];
@@ -376,19 +412,15 @@ template __global__ void k_grav_relax_col<4, 4356, GRAV_RELAX_D4, true>(GravGeo,
                                                                         const double *, double *, long);
 
 // residMPI (SG8:918-952): nodes 1..n-1, then zero at the x-end nodes
-// (one thread per (j, k, i+j+k) slot, j fastest, so that neighbouring
-// threads touch neighbouring addresses of the skewed layout)
+// (nodes in g_node's order, so that neighbouring threads touch neighbouring
+// addresses of the skewed layout)
 __global__ void k_grav_resid(GravGeo G, double *res2, const double *phi2, const double *rhs2, int l) {
   const int *n = G.n[l];
-  const int T = n[0] + n[1] + n[2] + 1;
-  const long cnt = (long)n[1] * T * n[2], t = g_tid();
+  const long cnt = (long)n[0] * n[1] * n[2], t = g_tid();
   if (t >= cnt * G.npe) return;
   const int r = (int)(t / cnt);
-  long q = t - r * cnt;
-  const int j = 1 + (int)(q % n[1]);
-  q /= n[1];
-  const int k = 1 + (int)(q % n[2]), tt = (int)(q / n[2]), i = tt - j - k;
-  if (i < 1 || i > n[0]) return;
+  int i, j, k;
+  g_node(n, t - r * cnt, i, j, k);
   int c[3];
   g_rank(G, r, c);
   const long base = G.off[l] + r * G.bsz[l], p = g_idx(n, i, j, k);
@@ -430,10 +462,8 @@ __global__ void k_grav_prolong(GravGeo G, double *uf2, const double *uc2, int lf
   const long cnt = (long)nf[0] * nf[1] * nf[2], t = g_tid();
   if (t >= cnt * G.npe) return;
   const int r = (int)(t / cnt);
-  long q = t - r * cnt;
-  const int i = 1 + (int)(q % nf[0]);
-  q /= nf[0];
-  const int j = 1 + (int)(q % nf[1]), k = 1 + (int)(q / nf[1]);
+  int i, j, k;
+  g_node(nf, t - r * cnt, i, j, k);
   int c[3];
   g_rank(G, r, c);
   const double *uc = uc2 + G.off[lf - 1] + r * G.bsz[lf - 1];

@@ -1213,8 +1213,8 @@ static int do_grav_solve(mcf_ctx *x) {
         CHK(grav_relax(x, jj, j >= NGL ? 2 : 5));
         {
           const int *m = G.n[jj];
-          const long nr = (long)m[1] * (m[0] + m[1] + m[2] + 1) * m[2] * G.npe;
-          KLAUNCH_N("k_grav_resid", k_grav_resid, dim3(gblocks(nr)), dim3(TPB), 24.0 * nr / 3.0, G, B.res2, B.phi2,
+          const long nr = (long)m[0] * m[1] * m[2] * G.npe;
+          KLAUNCH_N("k_grav_resid", k_grav_resid, dim3(gblocks(nr)), dim3(TPB), 24.0 * nr, G, B.res2, B.phi2,
                     B.rhs2, jj);
           CHK(grav_exchange(x, B.res2, jj, XALL));
         }
