@@ -1050,15 +1050,16 @@ __global__ void k_pb_collect(GravGeo G, double *bphi1, const double *bphi2) {
 }
 
 // Phi at the cells: each rank's nodes 1..C of the finest level (SG8:282-291)
+// (a rank block's cells in g_node's order: the reads of the skewed layout
+// contiguous, the writes scattered)
 __global__ void k_grav_phi(GravGeo G, const double *phi2, double *Phi, Geo g) {
-  const long N0 = g.n[0], N1 = g.n[1];
-  const long t = g_tid();
-  if (t >= N0 * N1 * g.n[2]) return;
-  const int i = 1 + (int)(t % N0), j = 1 + (int)((t / N0) % N1), k = 1 + (int)(t / (N0 * N1));
-  const int c0 = (i - 1) / G.C[0], c1 = (j - 1) / G.C[1], c2 = (k - 1) / G.C[2];
-  const int r = g_rid(G, c0, c1, c2), L = G.NGL;
-  Phi[ix3(g, i, j, k)] =
-      phi2[G.off[L] + r * G.bsz[L] + g_idx(G.n[L], i - c0 * G.C[0], j - c1 * G.C[1], k - c2 * G.C[2])];
+  const long cnt = (long)G.C[0] * G.C[1] * G.C[2], t = g_tid();
+  if (t >= cnt * G.npe) return;
+  const int r = (int)(t / cnt), L = G.NGL;
+  int i, j, k, c[3];
+  g_node(G.C, t - r * cnt, i, j, k);
+  g_rank(G, r, c);
+  Phi[ix3(g, c[0] * G.C[0] + i, c[1] * G.C[1] + j, c[2] * G.C[2] + k)] = phi2[G.off[L] + r * G.bsz[L] + g_idx(G.n[L], i, j, k)];
 }
 
 // Phi's face ghosts (SG8:22-67): y and z periodic, the x ends copied from
