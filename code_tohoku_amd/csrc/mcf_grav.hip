@@ -955,9 +955,14 @@ __global__ void k_pb_unpack(PbArgs P, double2 *Z, double2 *S, int nplanes, int i
 // -0.5 z facG; the Nyquist column of y (the speq values) takes
 // kappa = sqrt(sqrt(kappa^2) + 1e-20) for m <= nn2/2+1, as the reference.
 // One thread per mode (pp = 0..nn1/2, q = 0..nn2-1).
+// One wave per mode: the lanes take the planes in turn and the four sums
+// are reduced across the wave (the reference's sum runs over each rank's
+// planes and then over the ranks in the MPI library's order, so no order
+// is the reference's bit for bit).
 __global__ void k_pb_weight(PbArgs P, const double2 *Z, const double2 *S, double2 *D, double2 *SD) {
   const int H = P.nn1 / 2, nn1 = P.nn1, nn2 = P.nn2;
-  const long t = g_tid();
+  const long t = g_tid() >> 6;
+  const int lane = threadIdx.x & 63;
   if (t >= (long)(H + 1) * nn2) return;
   const int pp = (int)(t % (H + 1)), q = (int)(t / (H + 1));
   const double pi = 3.14159265359;
@@ -974,7 +979,7 @@ __global__ void k_pb_weight(PbArgs P, const double2 *Z, const double2 *S, double
     kap = m2 <= nn2 / 2 + 1 ? sqrt(sqrt(k2(l, m2)) + 1.0e-20) : sqrt(k2(l, nn2 + 2 - m2));
   }
   double ar[2] = {0.0, 0.0}, ai[2] = {0.0, 0.0};
-  for (int pl = 0; pl < P.np; ++pl) {
+  for (int pl = lane; pl < P.np; pl += 64) {
     const double2 X = pp < H ? Z[((long)pl * nn2 + q) * H + pp] : S[(long)pl * nn2 + q];
     const double fg = P.facG[pl];
     const double z[2] = {P.zp[pl], P.zq[pl]};
@@ -989,6 +994,14 @@ __global__ void k_pb_weight(PbArgs P, const double2 *Z, const double2 *S, double
       }
     }
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      ar[s] += __shfl_xor(ar[s], o);
+      ai[s] += __shfl_xor(ai[s], o);
+    }
+  if (lane != 0) return;
   for (int s = 0; s < 2; ++s) {
     if (pp < H) D[((long)s * nn2 + q) * H + pp] = make_double2(ar[s], ai[s]);
     else SD[(long)s * nn2 + q] = make_double2(ar[s], ai[s]);

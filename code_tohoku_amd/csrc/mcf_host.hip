@@ -1140,7 +1140,7 @@ static int do_grav_solve(mcf_ctx *x) {
             (long)nn2 * H, H, 1);
   const long nu = 2L * (nn2 / 2 + 1) + (long)(nn1 / 4 - 1) * nn2;
   KLAUNCH_N("k_pb_unpack", k_pb_unpack, dim3(gblocks(nu * P.np)), dim3(TPB), 32.0 * nZ, P, x->pbZ, x->pbS, P.np, 1);
-  KLAUNCH_N("k_pb_weight", k_pb_weight, dim3(gblocks((long)(H + 1) * nn2)), dim3(TPB), 16.0 * nZ, P, x->pbZ, x->pbS,
+  KLAUNCH_N("k_pb_weight", k_pb_weight, dim3(gblocks(64L * (H + 1) * nn2)), dim3(TPB), 16.0 * nZ, P, x->pbZ, x->pbS,
             x->pbD, x->pbSD);
   KLAUNCH_N("k_pb_unpack", k_pb_unpack, dim3(gblocks(nu * 2)), dim3(TPB), 64.0 * nn2 * H, P, x->pbD, x->pbSD, 2, -1);
   KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)(2L * nn2)), dim3(t1), 64.0 * nn2 * H, x->pbD, H, 1L, (long)H,
