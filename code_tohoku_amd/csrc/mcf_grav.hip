@@ -822,15 +822,26 @@ __global__ __launch_bounds__(GRAV_COARSE_THREADS) void k_grav_coarse(GravGeo G, 
 // ---------------------------------------------------------------------------
 // the real plane of global x index gx packed as nn1/2 complex values per
 // row (y = 2k-1 real part, 2k imaginary part), rows along z
-__global__ void k_pb_pack(PbArgs P, const double *rho, Geo g, double2 *Z) {
-  const int H = P.nn1 / 2;
-  const long per = (long)H * P.nn2, t = g_tid();
-  if (t >= per * P.np) return;
-  const int pl = (int)(t / per);
-  const long m = t - pl * per;
-  const int pp = (int)(m % H), q = (int)(m / H);
-  const int gx = P.gx[pl];
-  Z[t] = make_double2(rho[ix3(g, gx, 2 * pp + 1, q + 1)], rho[ix3(g, gx, 2 * pp + 2, q + 1)]);
+// Through an LDS tile of 64 planes x 64 rows at one z: the reads run along
+// x (the planes, consecutive x within a block), the writes along the row
+// pairs.  Grid (plane tiles, pair tiles of 32, nn2), 256 threads.
+__global__ __launch_bounds__(256) void k_pb_pack(PbArgs P, const double *rho, Geo g, double2 *Z) {
+  __shared__ double T[64][65];
+  const int H = P.nn1 / 2, q = blockIdx.z;
+  const int pl0 = 64 * blockIdx.x, pp0 = 32 * blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int pl = pl0 + lane;
+  const int gx = pl < P.np ? P.gx[pl] : 0;
+  for (int r = w; r < 64; r += 4) { // row j = 2 pp0 + 1 + r
+    const int pp = pp0 + (r >> 1);
+    T[r][lane] = (pl < P.np && pp < H) ? rho[ix3(g, gx, 2 * pp0 + 1 + r, q + 1)] : 0.0;
+  }
+  __syncthreads();
+  const int e = threadIdx.x & 31, h = threadIdx.x >> 5; // pair pp0 + e, planes h, h + 8, ...
+  for (int c = h; c < 64; c += 8) {
+    const int p2 = pl0 + c, pp = pp0 + e;
+    if (p2 < P.np && pp < H) Z[((long)p2 * P.nn2 + q) * H + pp] = make_double2(T[2 * e][c], T[2 * e + 1][c]);
+  }
 }
 
 // radix-2 FFT of lines of L complex values in LDS: X_k = sum_j z_j

@@ -1132,7 +1132,8 @@ static int do_grav_solve(mcf_ctx *x) {
   const int nn1 = P.nn1, nn2 = P.nn2, H = nn1 / 2, NGL = G.NGL, NGcr = G.NGcr;
   (void)nn1;
   const long nZ = (long)P.np * nn2 * H;
-  KLAUNCH_N("k_pb_pack", k_pb_pack, dim3(gblocks(nZ)), dim3(TPB), 16.0 * nZ, P, x->U[F_RHO], x->g, x->pbZ);
+  KLAUNCH_N("k_pb_pack", k_pb_pack, dim3((unsigned)((P.np + 63) / 64), (unsigned)((H + 31) / 32), (unsigned)nn2),
+            dim3(256), 16.0 * nZ, P, x->U[F_RHO], x->g, x->pbZ);
   const int t1 = std::min(512, std::max(64, H / 2)), t2 = std::min(512, std::max(64, nn2 / 2));
   KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)((long)P.np * nn2)), dim3(t1), 32.0 * nZ, x->pbZ, H, 1L, (long)H,
             (long)nn2 * H, nn2, 1);
