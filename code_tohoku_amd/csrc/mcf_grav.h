@@ -90,7 +90,8 @@ __global__ void k_grav_divide(GravGeo G, double *u2, const double *u1);
 // bottom of a V-cycle below the coarsest split level)
 __global__ void k_grav_coarse(GravGeo G, GravBuf B, int prog, int j);
 __global__ void k_pb_pack(PbArgs P, const double *rho, Geo g, double2 *Z);
-__global__ void k_pb_fft(double2 *Z, int L, long es, long inner_stride, long outer_stride, int ninner, int sign);
+__global__ void k_pb_fft(double2 *Z, int L, long es, long inner_stride, long outer_stride, int ninner, int sign,
+                         int lcw);
 __global__ void k_pb_unpack(PbArgs P, double2 *Z, double2 *S, int nplanes, int isign);
 __global__ void k_pb_weight(PbArgs P, const double2 *Z, const double2 *S, double2 *D, double2 *SD);
 __global__ void k_pb_bphi(GravGeo G, PbArgs P, const double2 *D, double *bphi2);

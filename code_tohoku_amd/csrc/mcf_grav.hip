@@ -846,35 +846,53 @@ __global__ __launch_bounds__(256) void k_pb_pack(PbArgs P, const double *rho, Ge
 
 // radix-2 FFT of lines of L complex values in LDS: X_k = sum_j z_j
 // exp(sign 2 pi i j k / L).  Line id -> (outer, inner); element e of the
-// line at outer*outer_stride + inner*inner_stride + e*es.
+// line at outer*outer_stride + inner*inner_stride + e*es.  A workgroup
+// transforms 2^lcw lines of consecutive inner index (ninner a multiple of
+// it, 2^lcw L <= 2 GRAV_FFT_MAXL): with inner_stride 1 and es > 1 (the
+// columns of a plane) its loads and stores run over 2^lcw neighbouring
+// columns at once.
 __global__ __launch_bounds__(512) void k_pb_fft(double2 *Z, int L, long es, long inner_stride, long outer_stride,
-                                                 int ninner, int sign) {
-  __shared__ double2 buf[GRAV_FFT_MAXL];
-  const long line = blockIdx.x;
-  double2 *z = Z + (line / ninner) * outer_stride + (line % ninner) * inner_stride;
+                                                 int ninner, int sign, int lcw) {
+  __shared__ double2 buf[2 * GRAV_FFT_MAXL];
+  // the twiddles exp(sign 2 pi i k / L), k < L/2, once per workgroup: stage
+  // len takes k = pos L/len, whose sincospi argument 2k/L is the same double
+  // as 2 pos/len
+  __shared__ double2 tw[GRAV_FFT_MAXL / 2];
+  const int CW = 1 << lcw, H2 = L / 2;
+  const long line0 = (long)blockIdx.x << lcw;
+  double2 *z = Z + (line0 / ninner) * outer_stride + (line0 % ninner) * inner_stride;
   int lg = 0;
   while ((1 << lg) < L) ++lg;
-  for (int e = threadIdx.x; e < L; e += blockDim.x) {
+  for (int idx = threadIdx.x; idx < CW * L; idx += blockDim.x) {
+    const int e = idx >> lcw, c = idx & (CW - 1);
     const int rev = (int)(__brev((unsigned)e) >> (32 - lg));
-    buf[rev] = z[e * es];
+    buf[c * L + rev] = z[c * inner_stride + e * es];
+  }
+  for (int k = threadIdx.x; k < H2; k += blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * (double)k / (double)L, &sn, &cs);
+    tw[k] = make_double2(cs, sign > 0 ? sn : -sn);
   }
   __syncthreads();
   for (int len = 2; len <= L; len <<= 1) {
     const int half = len >> 1;
-    for (int b = threadIdx.x; b < L / 2; b += blockDim.x) {
-      const int grp = b / half, pos = b - grp * half;
-      const int i0 = grp * len + pos, i1 = i0 + half;
-      double sn, cs;
-      sincospi(2.0 * (double)pos / (double)len, &sn, &cs);
-      sn = sign > 0 ? sn : -sn;
-      const double2 a = buf[i0], c = buf[i1];
-      const double tr = cs * c.x - sn * c.y, ti = cs * c.y + sn * c.x;
+    for (int b = threadIdx.x; b < CW * H2; b += blockDim.x) {
+      const int c = b / H2, bb = b - c * H2;
+      const int grp = bb / half, pos = bb - grp * half;
+      const int i0 = c * L + grp * len + pos, i1 = i0 + half;
+      const double2 W = tw[pos * (L / len)];
+      const double cs = W.x, sn = W.y;
+      const double2 a = buf[i0], d = buf[i1];
+      const double tr = cs * d.x - sn * d.y, ti = cs * d.y + sn * d.x;
       buf[i1] = make_double2(a.x - tr, a.y - ti);
       buf[i0] = make_double2(a.x + tr, a.y + ti);
     }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < L; e += blockDim.x) z[e * es] = buf[e];
+  for (int idx = threadIdx.x; idx < CW * L; idx += blockDim.x) {
+    const int e = idx >> lcw, c = idx & (CW - 1);
+    z[c * inner_stride + e * es] = buf[c * L + e];
+  }
 }
 
 // one rlft3 unpacking step: h1 = c1 (a + b), h2 = c2 (a - b) with

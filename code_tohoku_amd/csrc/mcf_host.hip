@@ -1134,20 +1134,23 @@ static int do_grav_solve(mcf_ctx *x) {
   const long nZ = (long)P.np * nn2 * H;
   KLAUNCH_N("k_pb_pack", k_pb_pack, dim3((unsigned)((P.np + 63) / 64), (unsigned)((H + 31) / 32), (unsigned)nn2),
             dim3(256), 16.0 * nZ, P, x->U[F_RHO], x->g, x->pbZ);
-  const int t1 = std::min(512, std::max(64, H / 2)), t2 = std::min(512, std::max(64, nn2 / 2));
+  // rows: one line per workgroup; columns: 8 neighbouring columns per
+  // workgroup where H allows, so that each load covers 128 bytes
+  const int lcw = (H % 8 == 0 && 8 * nn2 <= 2 * GRAV_FFT_MAXL) ? 3 : 0;
+  const int t1 = std::min(512, std::max(64, H / 2)), t2 = std::min(512, std::max(64, (nn2 << lcw) / 2));
   KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)((long)P.np * nn2)), dim3(t1), 32.0 * nZ, x->pbZ, H, 1L, (long)H,
-            (long)nn2 * H, nn2, 1);
-  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)((long)P.np * H)), dim3(t2), 32.0 * nZ, x->pbZ, nn2, (long)H, 1L,
-            (long)nn2 * H, H, 1);
+            (long)nn2 * H, nn2, 1, 0);
+  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)(((long)P.np * H) >> lcw)), dim3(t2), 32.0 * nZ, x->pbZ, nn2, (long)H,
+            1L, (long)nn2 * H, H, 1, lcw);
   const long nu = 2L * (nn2 / 2 + 1) + (long)(nn1 / 4 - 1) * nn2;
   KLAUNCH_N("k_pb_unpack", k_pb_unpack, dim3(gblocks(nu * P.np)), dim3(TPB), 32.0 * nZ, P, x->pbZ, x->pbS, P.np, 1);
   KLAUNCH_N("k_pb_weight", k_pb_weight, dim3(gblocks(64L * (H + 1) * nn2)), dim3(TPB), 16.0 * nZ, P, x->pbZ, x->pbS,
             x->pbD, x->pbSD);
   KLAUNCH_N("k_pb_unpack", k_pb_unpack, dim3(gblocks(nu * 2)), dim3(TPB), 64.0 * nn2 * H, P, x->pbD, x->pbSD, 2, -1);
   KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)(2L * nn2)), dim3(t1), 64.0 * nn2 * H, x->pbD, H, 1L, (long)H,
-            (long)nn2 * H, nn2, -1);
-  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)(2L * H)), dim3(t2), 64.0 * nn2 * H, x->pbD, nn2, (long)H, 1L,
-            (long)nn2 * H, H, -1);
+            (long)nn2 * H, nn2, -1, 0);
+  KLAUNCH_N("k_pb_fft", k_pb_fft, dim3((unsigned)((2L * H) >> lcw)), dim3(t2), 64.0 * nn2 * H, x->pbD, nn2, (long)H,
+            1L, (long)nn2 * H, H, -1, lcw);
   {
     const long nb = 2L * G.npe * (G.C[1] + 2) * (G.C[2] + 2);
     KLAUNCH_N("k_pb_bphi", k_pb_bphi, dim3(gblocks(nb)), dim3(TPB), 16.0 * nb, G, P, x->pbD, B.bphi2);
